@@ -1,0 +1,237 @@
+/*
+ * mc_abi.h -- C ABI of the MI355X drift-correction library (libmc.so).
+ *
+ * Plain C: POD types, device pointers as void*, byte counts as size_t, int
+ * status returns (MC_OK = 0).  No HIP, CUDA or cuFFT types appear here.
+ *
+ * Level 0 replaces the device seam of the reference, src-1125/cufunc.h:16-49
+ * (the functions the CDFAlign worker calls, DFAlign.cu:456-1355), one entry
+ * point per function with the same argument meaning and order.  Buffers use
+ * the same in-place padded spectrum layout as the reference (n rows of n+2
+ * floats = n x (n/2+1) complex, row-major).
+ *
+ * Level 1 is the resident movie pipeline: all spectra stay in HBM, the pair
+ * stage computes only the peak-search box of every correlation map, and one
+ * call returns the per-frame shifts and the aligned sum.
+ */
+#ifndef MC_ABI_H
+#define MC_ABI_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define MC_OK 0
+#define MC_ERR_ARG -1      /* invalid argument or unsupported size        */
+#define MC_ERR_DEVICE -2   /* HIP runtime error (message: mc_last_error) */
+#define MC_ERR_NOMEM -3    /* device allocation failed                    */
+#define MC_ERR_SOLVE -4    /* least-squares system singular               */
+#define MC_ERR_STATE -5    /* call out of order (e.g. no spectra loaded)  */
+
+/* Element types of frame data.  Codes 0, 1, 2, 6 are MRC modes
+ * (src-1125/mrcheader.h:8-14); mode 0 is read as signed bytes as the
+ * reference does (mrc.cpp:338-348).  100 is unsigned 8-bit (8-bit TIFF). */
+#define MC_DTYPE_INT8 0
+#define MC_DTYPE_INT16 1
+#define MC_DTYPE_FLOAT32 2
+#define MC_DTYPE_UINT16 6
+#define MC_DTYPE_UINT8 100
+
+/* Text of the last error on the calling thread. */
+const char *mc_last_error(void);
+/* Library version string. */
+const char *mc_version(void);
+
+/* ======================================================================
+ * Level 0: cufunc.h seam (src-1125/cufunc.h:16-49)
+ * ==================================================================== */
+typedef struct mc_plan_s *mc_plan;
+
+/* initGPU (cufunc.cu:79-114): select device; MC_ERR_ARG if out of range. */
+int mc_init_gpu(int gpu);
+/* ResetGPU (cufunc.cu:116-120). */
+int mc_reset_gpu(void);
+/* getGPUList (cufunc.cu:129-152): number of devices. */
+int mc_device_count(void);
+/* GPUMemCheck (cufunc.cu:154-158). */
+int mc_mem_check(size_t *free_bytes, size_t *total_bytes);
+/* GPUMemAlloc/Zero/Free/H2D/D2H/D2D (cufunc.cu:160-194); size_t counts. */
+int mc_mem_alloc(void **buf, size_t bytes);
+int mc_mem_zero(void *buf, size_t bytes);
+int mc_mem_free(void *buf);
+int mc_mem_h2d(void *dst, const void *src, size_t bytes);
+int mc_mem_d2h(void *dst, const void *src, size_t bytes);
+int mc_mem_d2d(void *dst, const void *src, size_t bytes);
+/* GPUMemBinD2D / GPUMemBinD2H (cufunc.cu:196-225): Fourier crop of a padded
+ * spectrum of size src_n to dst_n (rows 0..dst_n/2-1 and the last dst_n/2
+ * rows, dst_n+2 floats each). */
+int mc_mem_bin_d2d(float *dst, const float *src, int dst_n, int src_n);
+int mc_mem_bin_d2h(float *dst_host, const float *src, int dst_n, int src_n);
+/* GPUFFTPlan / GPUIFFTPlan (cufunc.cu:251-264): n x n in-place plans.
+ * Supported n: powers of two from 64 to 8192. */
+int mc_fft_plan(mc_plan *plan, int n);
+int mc_ifft_plan(mc_plan *plan, int n);
+/* GPUFFT2d / GPUIFFT2d (cufunc.cu:272-282): unnormalised, in place. */
+int mc_fft2d(float *buf, mc_plan plan);
+int mc_ifft2d(float *buf, mc_plan plan);
+/* GPUFFTDestroy (cufunc.cu:266-270). */
+int mc_fft_destroy(mc_plan plan);
+/* GPUSync (cufunc.cu:284-289). */
+int mc_sync(void);
+/* GPUAdd / GPUMultiplyNum (cufunc.cu:292-312): counts in floats. */
+int mc_add(float *dst, const float *src, size_t nfloats);
+int mc_multiply_num(float *dst, float num, size_t nfloats);
+/* GPUFFTLogModulus (cufunc.cu:227-237): dmod[i] = log(1+|F_i|*scale). */
+int mc_fft_log_modulus(float *dmod, const float *dfft, int n, float scale);
+/* MASK of cufunc.h:9-14; MkPosList(MASK*, n, bfactor) (cufunc.cu:358-390)
+ * fills a host table of n*(n/2+1) entries. */
+typedef struct {
+  int x;
+  int y;
+  float z;
+} mc_mask;
+int mc_make_pos_list(mc_mask *list, int n, float bfactor);
+/* GPUShiftCC (cufunc.cu:392-419): dfft <- dsum*conj(dfft*e^{i th})*z/sizec. */
+int mc_shift_cc(float *dfft, const float *dsum, const mc_mask *dposlist, float sx, float sy,
+                int n);
+/* GPUShift (cufunc.cu:421-443): dfft <- dfft * e^{i th}, th = (2 pi/n)(sx x + sy y'). */
+int mc_shift(float *dfft, const mc_mask *dposlist, float sx, float sy, int n);
+/* FindShift (cufunc.cu:445-546): box read-out of a C2R output (pitch n+2),
+ * integer and 1/32-pixel peak (FP64).  hboxmap (host, box*box floats) gets
+ * the box; returns 0 and writes cc. */
+int mc_find_shift(const float *dsrc, int n, float *hboxmap, int box, float *sx, float *sy,
+                  float *cc, int wnoise);
+
+/* ======================================================================
+ * Level 1: resident movie pipeline
+ * ==================================================================== */
+typedef struct mc_context_s *mc_context;
+
+/* A context owns a HIP stream and the device buffers of one movie at a
+ * time (grown as needed, kept across calls). */
+int mc_context_create(int device, mc_context *ctx);
+int mc_context_destroy(mc_context ctx);
+
+/* Frames of one movie in device memory. */
+typedef struct {
+  const void *frames;  /* nz frames; element type dtype                    */
+  const void *frames2; /* second dark/gain-corrected copy used as the
+                          conjugated operand of the pair stage (the hbuf2
+                          stack, DFAlign.cu:724-749,889-890) or NULL      */
+  int dtype;           /* MC_DTYPE_*                                      */
+  int nx, ny, nz;      /* frame width, height, count in the buffer        */
+  size_t row_pitch;    /* bytes between rows (0: nx * element size)       */
+  size_t frame_pitch;  /* bytes between frames (0: ny * row_pitch)        */
+  int nz_file;         /* frames in the stack file; RefFrame = nz_file/2+1
+                          (DFAlign.cu:1133); 0: nz                        */
+} mc_movie_desc;
+
+/* Alignment parameters: APARA (DFAlign.h:16-63) and CDFAlign::m_iter /
+ * m_iterthres (DFAlign.h:75-76), plus the new geometry and form selectors.
+ * Frame indices are absolute (0-based) and already clamped as VerifyPara
+ * does (dosefgpu_driftcorr.cpp:60-118). */
+typedef struct {
+  int crop_offsetx, crop_offsety, crop_nsam; /* -crx -cry -crd            */
+  float bin;                                 /* -bin (1..2)               */
+  int nStart, nEnd;                          /* -nst -ned                 */
+  int nStartSum, nEndSum, nEndSum2;          /* -nss -nes -2es            */
+  float bfactor;                             /* -bft                      */
+  int CCPeakSearchDim;                       /* -pbx                      */
+  int FrameDistOffset;                       /* -fod                      */
+  int NoisePeakSize;                         /* -nps                      */
+  float kiThresh;                            /* -kit                      */
+  int bAlignToMid;                           /* -atm                      */
+  int iter;                                  /* -itr                      */
+  float iterthres;                           /* -ith                      */
+  int pad;        /* new: 0 = reference square crop; N = pad the region
+                     [crx,nx) x [cry,ny) into an N x N canvas filled with
+                     the region mean                                       */
+  int precision;  /* MC_PREC_*                                          */
+  int grad_steps; /* new: adjoint sharpness refinement steps (0 = off)  */
+  float grad_step_px; /* new: trust-region step of the refinement (px)  */
+} mc_params;
+
+#define MC_PREC_FP32 0
+
+/* Optional results (host pointers; NULL to skip).  Sizes use
+ * F = nEnd-nStart+1, K = F-1, P = pair count, B = CCPeakSearchDim. */
+typedef struct {
+  int *pairs;          /* [2P] frame positions (i, j) of each pair          */
+  double *pair_b;      /* [2P] measured shift of each pair                   */
+  double *pair_cc;     /* [P]                                                */
+  float *boxmaps;      /* [P*B*B] box of each correlation map                */
+  double *shift1;      /* [2K] first least-squares increments               */
+  double *ki1;         /* [P] residuals                                      */
+  int *good;           /* [P] rows kept by outlier rejection (ascending)    */
+  double *shift2;      /* [2K] increments after outlier rejection           */
+  double *ki2;         /* [P] residuals of the kept rows                     */
+  double *iter_meas;   /* [iter*2F] per-round measured positions            */
+  double *rmsd;        /* [iter] accumulated RMSD recursion                  */
+  double *final_shift; /* [2K] increments after iteration                   */
+  double *frame_shift; /* [2F] shift applied to each frame in the sum (NaN:
+                          not summed)                                        */
+  double *frame_mean;  /* [F] DC/n^2 of each spectrum (the per-frame log)   */
+  float *sum_image;    /* [n*n] aligned sum (host)                           */
+  float *sum2_image;   /* [n*n] second sum (-2es) (host)                     */
+  float *rawsum_image; /* [n*n] unaligned sum (host)                         */
+  void *sum_image_dev; /* [n*n] aligned sum left in device memory            */
+} mc_outputs;
+
+typedef struct {
+  int n, nsamUnbin, nframe, npairs, ngood, niter, converged;
+  int sum2_written, rawsum_written;
+  double rmsd;
+  float ms_total, ms_spectra, ms_pairs, ms_peaks, ms_solve, ms_iter, ms_sum;
+} mc_report;
+
+/* Whole movie: DFAlign.cu:456-1355 numerics (without file I/O and logs). */
+int mc_align_movie(mc_context ctx, const mc_movie_desc *movie, const mc_params *p,
+                   mc_outputs *out, mc_report *rep);
+
+/* Stage entry points on the context's resident state ------------------- */
+/* Stage 1 only: frames -> resident spectra (sizes written to rep). */
+int mc_load_frames(mc_context ctx, const mc_movie_desc *movie, const mc_params *p,
+                   mc_report *rep);
+/* Replace the resident spectra with host spectra in the padded layout
+ * (nframe x n x (n+2) floats). */
+int mc_set_spectra(mc_context ctx, const float *host_padded, int n, int nframe);
+/* Copy the resident spectra out in the padded layout. */
+int mc_get_spectra(mc_context ctx, float *host_padded);
+/* Pair stage on resident spectra: pairs[2P] index resident frames (operand
+ * i kept, operand j conjugated); writes b (2P), cc (P) and optionally the
+ * box maps (P*box*box). */
+int mc_pair_shifts(mc_context ctx, const int *pairs, int npairs, float bfactor, int box,
+                   int wnoise, double *b, double *cc, float *boxmaps);
+/* Peak search on given box maps (host, P*box*box floats). */
+int mc_find_peaks(mc_context ctx, const float *boxmaps, int npairs, int box, int n,
+                  int wnoise, double *b, double *cc);
+/* Phase-shift-and-sum of the resident spectra with per-frame shifts
+ * (2F doubles; NaN x = frame skipped) and inverse to an n x n image scaled
+ * by 1/n^2 (host). */
+int mc_sum_image(mc_context ctx, const double *frame_shift, float *image);
+
+/* Host FP64 drift model (mfunc.h:202-298, DFAlign.cu:914-948). */
+int mc_pair_list(int nframe, int fod, int *pairs, int cap);
+int mc_lsq(const int *pairs, int npairs, int nframe, const double *b, double *s);
+int mc_threshold(const int *pairs, int npairs, int nframe, const double *ki, double kithresh,
+                 int *good);
+
+/* Kernel timing over a region (HIP events on the context's stream around
+ * every launch of the named kernel group). */
+#define MC_KT_SPECTRA 0
+#define MC_KT_PAIR_COLS 1
+#define MC_KT_PAIR_ROWS 2
+#define MC_KT_PEAKS 3
+#define MC_KT_SUM 4
+#define MC_KT_IMAGE 5
+#define MC_KT_COUNT 6
+int mc_kernel_timing(mc_context ctx, int enable);
+int mc_kernel_timing_get(mc_context ctx, int group, int *count, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,858 @@
+// mc_align.cpp -- host side of libmc: the C ABI (include/mc_abi.h), the
+// resident movie pipeline and the FP64 drift model.
+//
+// Control flow follows CDFAlign::ThreadFunc_cuAlign (reference
+// src-1125/DFAlign.cu:456-1355) with the device work on one HIP stream and
+// every spectrum resident in HBM: stage 1 (DFAlign.cu:705-824), pairwise
+// shifts (:875-905), least squares with outlier rejection (:914-948),
+// iterative align-to-sum (:950-1071) and the final phase-shift-and-sum
+// (:1114-1292).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mc_abi.h"
+#include "mc_kernels.h"
+
+using mc::R2CGeom;
+
+// ------------------------------------------------------------- errors
+static thread_local std::string g_err;
+
+static int set_err(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                       \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return set_err(e_ == hipErrorOutOfMemory ? MC_ERR_NOMEM : MC_ERR_DEVICE, "%s: %s (%s:%d)", \
+                     #expr, hipGetErrorString(e_), __FILE__, __LINE__);                     \
+  } while (0)
+
+#define MC_TRY(expr)        \
+  do {                      \
+    int r_ = (expr);        \
+    if (r_ != MC_OK) return r_; \
+  } while (0)
+
+int mc_internal_set_error(int code, const char* msg) {
+  g_err = msg;
+  return code;
+}
+
+extern "C" const char* mc_last_error(void) { return g_err.c_str(); }
+extern "C" const char* mc_version(void) { return "motioncorr1exp_amd 0.1 (gfx950)"; }
+
+static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+static bool fft_size_ok(int n) { return is_pow2(n) && n >= 64 && n <= 8192; }
+
+// ------------------------------------------------------- device buffers
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e == hipSuccess) cap = bytes;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+struct KTimer {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+};
+
+struct mc_context_s {
+  int device = 0;
+  hipStream_t st = nullptr;
+  DevBuf spec, rt, rowsum, fill, padP, pairs_d, tbuf, box, peaks, shifts, image, tmp, means,
+      sum2img;
+  // resident spectra
+  int n = 0, nframe = 0, nslots = 0, off2 = 0, sum_slot = 0;
+  size_t spec_stride = 0;
+  int padP_nu = -1, padP_w = -1, padP_nkx = -1;
+  // kernel timing
+  bool kt_on = false;
+  KTimer kt[MC_KT_COUNT];
+  std::vector<hipEvent_t> pool;
+  size_t pool_used = 0;
+
+  hipEvent_t ev() {
+    if (pool_used < pool.size()) return pool[pool_used++];
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    pool.push_back(e);
+    pool_used++;
+    return e;
+  }
+};
+
+// Wraps one kernel group with events when timing is enabled.
+struct KScope {
+  mc_context_s* c;
+  int g;
+  hipEvent_t a = nullptr;
+  KScope(mc_context_s* c_, int g_) : c(c_), g(g_) {
+    if (c->kt_on) {
+      a = c->ev();
+      if (a) (void)hipEventRecord(a, c->st);
+    }
+  }
+  ~KScope() {
+    if (c->kt_on && a) {
+      hipEvent_t b = c->ev();
+      if (b) {
+        (void)hipEventRecord(b, c->st);
+        c->kt[g].ev.push_back({a, b});
+      }
+    }
+  }
+};
+
+extern "C" int mc_context_create(int device, mc_context* ctx) {
+  if (!ctx) return set_err(MC_ERR_ARG, "ctx is NULL");
+  int nd = 0;
+  HIP_TRY(hipGetDeviceCount(&nd));
+  if (device < 0 || device >= nd)
+    return set_err(MC_ERR_ARG, "GPU ID %d is out of range(%d).", device, nd);
+  HIP_TRY(hipSetDevice(device));
+  mc_context_s* c = new mc_context_s;
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return set_err(MC_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  *ctx = c;
+  return MC_OK;
+}
+
+extern "C" int mc_context_destroy(mc_context ctx) {
+  if (!ctx) return MC_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->st);
+  DevBuf* bufs[] = {&ctx->spec, &ctx->rt, &ctx->rowsum, &ctx->fill, &ctx->padP,
+                    &ctx->pairs_d, &ctx->tbuf, &ctx->box, &ctx->peaks, &ctx->shifts,
+                    &ctx->image, &ctx->tmp, &ctx->means, &ctx->sum2img};
+  for (DevBuf* b : bufs) b->release();
+  for (hipEvent_t e : ctx->pool) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(ctx->st);
+  delete ctx;
+  return MC_OK;
+}
+
+extern "C" int mc_kernel_timing(mc_context ctx, int enable) {
+  if (!ctx) return set_err(MC_ERR_ARG, "ctx is NULL");
+  HIP_TRY(hipStreamSynchronize(ctx->st));
+  for (auto& k : ctx->kt) k.ev.clear();
+  ctx->pool_used = 0;
+  ctx->kt_on = enable != 0;
+  return MC_OK;
+}
+
+extern "C" int mc_kernel_timing_get(mc_context ctx, int group, int* count, double* total_ms) {
+  if (!ctx || group < 0 || group >= MC_KT_COUNT) return set_err(MC_ERR_ARG, "bad group");
+  HIP_TRY(hipStreamSynchronize(ctx->st));
+  double t = 0;
+  for (auto& pr : ctx->kt[group].ev) {
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, pr.first, pr.second));
+    t += ms;
+  }
+  if (count) *count = (int)ctx->kt[group].ev.size();
+  if (total_ms) *total_ms = t;
+  return MC_OK;
+}
+
+// ============================================================ geometry
+// func.cpp:80-103
+static int verify_crop_size(int nx, int ny, int offx, int offy, int nsamout, float bin) {
+  if (offx < 0 || offx >= nx || offy < 0 || offy >= ny || bin <= 0) return 0;
+  if (nsamout <= 0) nsamout = std::min(nx - offx, ny - offy);
+  if (nx - offx < nsamout) nsamout = nx - offx;
+  if (ny - offy < nsamout) nsamout = ny - offy;
+  if (std::fabs(bin - 1.0) < 0.0001) nsamout = nsamout / 2 * 2;
+  else if (std::fabs(bin - 2.0) < 0.0001) nsamout = nsamout / 4 * 4;
+  else nsamout = nsamout / 2 * 2;
+  return nsamout;
+}
+
+// DFAlign.cu:515
+static int binned_size(int nsu, float bin) { return (int)((float)nsu / bin) / 2 * 2; }
+
+static size_t dtype_size(int dt) {
+  switch (dt) {
+    case MC_DTYPE_INT8:
+    case MC_DTYPE_UINT8: return 1;
+    case MC_DTYPE_INT16:
+    case MC_DTYPE_UINT16: return 2;
+    case MC_DTYPE_FLOAT32: return 4;
+    default: return 0;
+  }
+}
+
+struct Geom {
+  int nsu, n, w, h, pad_mode;
+  size_t row_pitch, frame_pitch;
+};
+
+static int movie_geometry(const mc_movie_desc* m, const mc_params* p, Geom* g) {
+  const size_t es = dtype_size(m->dtype);
+  if (!es) return set_err(MC_ERR_ARG, "unsupported dtype %d", m->dtype);
+  if (m->nx <= 0 || m->ny <= 0 || m->nz <= 0) return set_err(MC_ERR_ARG, "empty stack");
+  g->row_pitch = m->row_pitch ? m->row_pitch : (size_t)m->nx * es;
+  g->frame_pitch = m->frame_pitch ? m->frame_pitch : g->row_pitch * m->ny;
+  if (p->bin <= 0) return set_err(MC_ERR_ARG, "bin must be > 0");
+  if (p->pad > 0) {
+    if (p->crop_offsetx < 0 || p->crop_offsetx >= m->nx || p->crop_offsety < 0 ||
+        p->crop_offsety >= m->ny)
+      return set_err(MC_ERR_ARG, "Error: Wrong image Size.");
+    g->nsu = p->pad;
+    g->w = std::min(m->nx - p->crop_offsetx, p->pad);
+    g->h = std::min(m->ny - p->crop_offsety, p->pad);
+    g->pad_mode = (g->w < p->pad || g->h < p->pad) ? 1 : 0;
+  } else {
+    g->nsu = verify_crop_size(m->nx, m->ny, p->crop_offsetx, p->crop_offsety, p->crop_nsam,
+                              p->bin);
+    if (g->nsu <= 0) return set_err(MC_ERR_ARG, "Error: Wrong image Size.");
+    g->w = g->h = g->nsu;
+    g->pad_mode = 0;
+  }
+  g->n = binned_size(g->nsu, p->bin);
+  if (!fft_size_ok(g->nsu) || !fft_size_ok(g->n))
+    return set_err(MC_ERR_ARG,
+                   "transform size %d (binned %d) unsupported: sizes must be powers of two "
+                   "in [64, 8192]; use the pad geometry",
+                   g->nsu, g->n);
+  return MC_OK;
+}
+
+// ========================================================= drift model
+// mfunc.h:202-236
+extern "C" int mc_pair_list(int nframe, int fod, int* pairs, int cap) {
+  int dist = nframe / 2 - fod;
+  if (dist <= 0) dist = 1;
+  if (dist > nframe / 2) dist = nframe / 2;
+  int c = 0;
+  for (int i = 0; i < nframe; i++)
+    for (int j = i + dist; j < nframe; j++) {
+      if (pairs && c < cap) {
+        pairs[2 * c] = i;
+        pairs[2 * c + 1] = j;
+      }
+      c++;
+    }
+  return c;
+}
+
+// Least squares of the pair design matrix (A[p][k] = 1 for i <= k < j,
+// mfunc.h:228-233) by the FP64 normal equations A^T A s = A^T b, Cholesky.
+extern "C" int mc_lsq(const int* pairs, int np, int nframe, const double* b, double* s) {
+  const int K = nframe - 1;
+  if (K <= 0 || np < K) return set_err(MC_ERR_ARG, "least squares needs >= %d rows", K);
+  std::vector<double> N((size_t)K * K, 0.0), r((size_t)2 * K, 0.0);
+  for (int q = 0; q < np; q++) {
+    const int i = pairs[2 * q], j = pairs[2 * q + 1];
+    for (int k = i; k < j; k++) {
+      r[2 * k] += b[2 * q];
+      r[2 * k + 1] += b[2 * q + 1];
+      for (int l = i; l < j; l++) N[(size_t)k * K + l] += 1.0;
+    }
+  }
+  // Cholesky N = L L^T (lower, in place)
+  for (int k = 0; k < K; k++) {
+    double d = N[(size_t)k * K + k];
+    for (int m = 0; m < k; m++) d -= N[(size_t)k * K + m] * N[(size_t)k * K + m];
+    if (!(d > 1e-12)) return set_err(MC_ERR_SOLVE, "singular drift system (column %d)", k);
+    d = std::sqrt(d);
+    N[(size_t)k * K + k] = d;
+    for (int i = k + 1; i < K; i++) {
+      double v = N[(size_t)i * K + k];
+      for (int m = 0; m < k; m++) v -= N[(size_t)i * K + m] * N[(size_t)k * K + m];
+      N[(size_t)i * K + k] = v / d;
+    }
+  }
+  for (int c = 0; c < 2; c++) {
+    std::vector<double> y(K);
+    for (int i = 0; i < K; i++) {
+      double v = r[2 * i + c];
+      for (int m = 0; m < i; m++) v -= N[(size_t)i * K + m] * y[m];
+      y[i] = v / N[(size_t)i * K + i];
+    }
+    for (int i = K - 1; i >= 0; i--) {
+      double v = y[i];
+      for (int m = i + 1; m < K; m++) v -= N[(size_t)m * K + i] * s[2 * m + c];
+      s[2 * i + c] = v / N[(size_t)i * K + i];
+    }
+  }
+  return MC_OK;
+}
+
+static void residuals(const int* pairs, int np, const double* s, const double* b, double* ki) {
+  for (int q = 0; q < np; q++) {
+    double x = -b[2 * q], y = -b[2 * q + 1];
+    for (int k = pairs[2 * q]; k < pairs[2 * q + 1]; k++) {
+      x += s[2 * k];
+      y += s[2 * k + 1];
+    }
+    ki[q] = std::sqrt(x * x + y * y);
+  }
+}
+
+// rank(A restricted to rows) == nframe-1  <=>  the pair graph is connected
+static bool connected(const int* pairs, const int* sel, int nsel, int nframe) {
+  std::vector<int> par(nframe);
+  for (int i = 0; i < nframe; i++) par[i] = i;
+  auto find = [&](int a) {
+    while (par[a] != a) a = par[a] = par[par[a]];
+    return a;
+  };
+  int comps = nframe;
+  for (int q = 0; q < nsel; q++) {
+    int a = find(pairs[2 * sel[q]]), b = find(pairs[2 * sel[q] + 1]);
+    if (a != b) {
+      par[a] = b;
+      comps--;
+    }
+  }
+  return comps == 1;
+}
+
+// OD_Threshold (mfunc.h:253-298)
+extern "C" int mc_threshold(const int* pairs, int np, int nframe, const double* ki,
+                            double kithresh, int* good) {
+  const int K = nframe - 1;
+  std::vector<int> order(np);
+  for (int i = 0; i < np; i++) order[i] = i;
+  // ascending ki, ties by lower row index (repeated first-minimum selection)
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ki[a] < ki[b]; });
+  int i;
+  for (i = np - 1; i >= K; i--)
+    if (ki[order[i]] < kithresh) break;
+  int ii = i, nsel;
+  do {
+    nsel = ii + 1;
+    for (int q = 0; q < nsel; q++) good[q] = order[q];
+    std::sort(good, good + nsel);
+    ii++;
+    if (connected(pairs, good, nsel, nframe)) break;
+  } while (ii < np);
+  return nsel;
+}
+
+// ========================================================== resident state
+static int ensure_padP(mc_context_s* c, int nu, int w, int nkx) {
+  if (c->padP_nu == nu && c->padP_w == w && c->padP_nkx == nkx) return MC_OK;
+  // P[kx] = sum_{x=w}^{nu-1} e^{-2 pi i kx x/nu}: spectrum of the fill mask row
+  std::vector<float2> h(nkx);
+  for (int k = 0; k < nkx; k++) {
+    double re = 0, im = 0;
+    if (k == 0) {
+      re = nu - w;
+    } else {
+      // (e^{-i t w} - e^{-i t nu}) / (1 - e^{-i t}), t = 2 pi k / nu
+      const double t = 2.0 * M_PI * k / nu;
+      const double ar = std::cos(t * w) - 1.0, ai = -std::sin(t * w);
+      const double br = 1.0 - std::cos(t), bi = std::sin(t);
+      const double den = br * br + bi * bi;
+      re = (ar * br + ai * bi) / den;
+      im = (ai * br - ar * bi) / den;
+    }
+    h[k] = make_float2((float)re, (float)im);
+  }
+  HIP_TRY(c->padP.ensure(sizeof(float2) * nkx));
+  HIP_TRY(hipMemcpyAsync(c->padP.p, h.data(), sizeof(float2) * nkx, hipMemcpyHostToDevice, c->st));
+  HIP_TRY(hipStreamSynchronize(c->st));
+  c->padP_nu = nu;
+  c->padP_w = w;
+  c->padP_nkx = nkx;
+  return MC_OK;
+}
+
+static int ensure_spectra(mc_context_s* c, int n, int nframe, int two) {
+  const size_t stride = (size_t)(n / 2 + 1) * n;
+  const int nslots = nframe * (two ? 2 : 1) + 1;
+  HIP_TRY(c->spec.ensure(sizeof(float2) * stride * nslots));
+  c->n = n;
+  c->nframe = nframe;
+  c->nslots = nslots;
+  c->off2 = two ? nframe : 0;
+  c->sum_slot = nslots - 1;
+  c->spec_stride = stride;
+  return MC_OK;
+}
+
+static float2* slot(mc_context_s* c, int s) { return c->spec.as<float2>() + c->spec_stride * s; }
+
+// Stage 1 for frames [first, first+count) of a buffer into slots [s0, ...).
+static int run_stage1(mc_context_s* c, const void* frames, const mc_movie_desc* m,
+                      const mc_params* p, const Geom& g, int first, int count, int s0) {
+  const int nkx = g.n / 2 + 1;
+  const size_t rt_per = (size_t)nkx * g.nsu;
+  int chunk = (int)std::max<size_t>(1, ((size_t)512 << 20) / (rt_per * sizeof(float2)));
+  chunk = std::min(chunk, count);
+  HIP_TRY(c->rt.ensure(sizeof(float2) * rt_per * chunk));
+  HIP_TRY(c->rowsum.ensure(sizeof(float) * (size_t)g.nsu * chunk));
+  HIP_TRY(c->fill.ensure(sizeof(float) * chunk));
+  if (g.pad_mode) MC_TRY(ensure_padP(c, g.nsu, g.w, nkx));
+  R2CGeom rg;
+  rg.src = static_cast<const char*>(frames) + (size_t)first * g.frame_pitch;
+  rg.dtype = m->dtype;
+  rg.row_pitch = g.row_pitch;
+  rg.frame_pitch = g.frame_pitch;
+  rg.offx = p->crop_offsetx;
+  rg.offy = p->crop_offsety;
+  rg.w = g.w;
+  rg.h = g.h;
+  rg.nu = g.nsu;
+  rg.n = g.n;
+  rg.pad_mode = g.pad_mode;
+  for (int f = 0; f < count; f += chunk) {
+    const int cnt = std::min(chunk, count - f);
+    HIP_TRY(mc::launch_frames_to_spectra(rg, f, cnt, slot(c, s0 + f), c->spec_stride,
+                                         c->rt.as<float2>(), c->rowsum.as<float>(),
+                                         c->fill.as<float>(), c->padP.as<float2>(), c->st));
+  }
+  return MC_OK;
+}
+
+static int load_frames(mc_context_s* c, const mc_movie_desc* m, const mc_params* p, Geom* g,
+                       int* nframe_out) {
+  MC_TRY(movie_geometry(m, p, g));
+  const int nframe = p->nEnd - p->nStart + 1;
+  if (p->nStart < 0 || p->nEnd >= m->nz || nframe < 2)
+    return set_err(MC_ERR_ARG, "frame range [%d, %d] invalid for %d frames", p->nStart, p->nEnd,
+                   m->nz);
+  if (!m->frames) return set_err(MC_ERR_ARG, "frames is NULL");
+  MC_TRY(ensure_spectra(c, g->n, nframe, m->frames2 != nullptr));
+  {
+    KScope ks(c, MC_KT_SPECTRA);
+    MC_TRY(run_stage1(c, m->frames, m, p, *g, p->nStart, nframe, 0));
+    if (m->frames2) MC_TRY(run_stage1(c, m->frames2, m, p, *g, p->nStart, nframe, c->off2));
+  }
+  *nframe_out = nframe;
+  return MC_OK;
+}
+
+// Pair stage on resident slots: pairs (a, b) are slot indices.
+static int pair_stage(mc_context_s* c, const std::vector<int>& slots, float bfactor, int box,
+                      int wnoise, std::vector<double>& res, float* boxmaps_host) {
+  const int np = (int)slots.size() / 2;
+  const int n = c->n;
+  const int H = n / 2 + 1;
+  HIP_TRY(c->pairs_d.ensure(sizeof(int) * slots.size()));
+  HIP_TRY(c->tbuf.ensure(sizeof(float2) * (size_t)np * H * box));
+  HIP_TRY(c->box.ensure(sizeof(float) * (size_t)np * box * box));
+  HIP_TRY(c->peaks.ensure(sizeof(double) * 3 * np));
+  HIP_TRY(hipMemcpyAsync(c->pairs_d.p, slots.data(), sizeof(int) * slots.size(),
+                         hipMemcpyHostToDevice, c->st));
+  {
+    KScope ks(c, MC_KT_PAIR_COLS);
+    HIP_TRY(mc::launch_pair_cols(c->spec.as<float2>(), c->spec_stride, c->pairs_d.as<int2>(), np,
+                                 n, bfactor, box, c->tbuf.as<float2>(), c->st));
+  }
+  {
+    KScope ks(c, MC_KT_PAIR_ROWS);
+    HIP_TRY(mc::launch_pair_rows(c->tbuf.as<float2>(), np, n, box, c->box.as<float>(), c->st));
+  }
+  {
+    KScope ks(c, MC_KT_PEAKS);
+    HIP_TRY(mc::launch_find_peaks(c->box.as<float>(), np, box, n, wnoise, c->peaks.as<double>(),
+                                  c->st));
+  }
+  res.resize(3 * (size_t)np);
+  HIP_TRY(hipMemcpyAsync(res.data(), c->peaks.p, sizeof(double) * 3 * np, hipMemcpyDeviceToHost,
+                         c->st));
+  if (boxmaps_host)
+    HIP_TRY(hipMemcpyAsync(boxmaps_host, c->box.p, sizeof(float) * (size_t)np * box * box,
+                           hipMemcpyDeviceToHost, c->st));
+  HIP_TRY(hipStreamSynchronize(c->st));
+  return MC_OK;
+}
+
+// Shift-and-sum of slots [0, nframe) with per-frame shifts into `dst` slot
+// (and optionally sum2 for frames <= f2_last), all on device.
+static int shift_sum(mc_context_s* c, const std::vector<double>& sh, float2* dst, float2* dst2,
+                     int f2_last) {
+  HIP_TRY(c->shifts.ensure(sizeof(double) * sh.size()));
+  HIP_TRY(hipMemcpyAsync(c->shifts.p, sh.data(), sizeof(double) * sh.size(),
+                         hipMemcpyHostToDevice, c->st));
+  KScope ks(c, MC_KT_SUM);
+  HIP_TRY(mc::launch_shift_sum(c->spec.as<float2>(), c->spec_stride, c->nframe, c->n,
+                               c->shifts.as<double2>(), dst, dst2, f2_last, c->st));
+  return MC_OK;
+}
+
+static int to_image(mc_context_s* c, const float2* spec, float* dev_img) {
+  const int n = c->n;
+  HIP_TRY(c->tmp.ensure(sizeof(float2) * (size_t)(n / 2 + 1) * n));
+  KScope ks(c, MC_KT_IMAGE);
+  // DFAlign.cu:1289-1290: C2R then x 1/size (float 1.0/size)
+  HIP_TRY(mc::launch_spectrum_to_image(spec, n, (float)(1.0 / ((double)n * n)), dev_img, n,
+                                       c->tmp.as<float2>(), c->st));
+  return MC_OK;
+}
+
+// ================================================================ level 1
+extern "C" int mc_load_frames(mc_context ctx, const mc_movie_desc* m, const mc_params* p,
+                              mc_report* rep) {
+  if (!ctx || !m || !p) return set_err(MC_ERR_ARG, "NULL argument");
+  HIP_TRY(hipSetDevice(ctx->device));
+  Geom g;
+  int nframe = 0;
+  MC_TRY(load_frames(ctx, m, p, &g, &nframe));
+  HIP_TRY(hipStreamSynchronize(ctx->st));
+  if (rep) {
+    memset(rep, 0, sizeof *rep);
+    rep->n = g.n;
+    rep->nsamUnbin = g.nsu;
+    rep->nframe = nframe;
+  }
+  return MC_OK;
+}
+
+extern "C" int mc_set_spectra(mc_context ctx, const float* host_padded, int n, int nframe) {
+  if (!ctx || !host_padded || nframe < 1 || !fft_size_ok(n))
+    return set_err(MC_ERR_ARG, "bad spectra arguments");
+  HIP_TRY(hipSetDevice(ctx->device));
+  MC_TRY(ensure_spectra(ctx, n, nframe, 0));
+  const size_t per = (size_t)(n / 2 + 1) * n;
+  HIP_TRY(ctx->tmp.ensure(sizeof(float2) * per));
+  for (int f = 0; f < nframe; f++) {
+    HIP_TRY(hipMemcpyAsync(ctx->tmp.p, host_padded + (size_t)f * n * (n + 2),
+                           sizeof(float2) * per, hipMemcpyHostToDevice, ctx->st));
+    HIP_TRY(mc::launch_padded_to_spec(ctx->tmp.as<float2>(), n, slot(ctx, f), ctx->st));
+  }
+  HIP_TRY(hipStreamSynchronize(ctx->st));
+  return MC_OK;
+}
+
+extern "C" int mc_get_spectra(mc_context ctx, float* host_padded) {
+  if (!ctx || !host_padded) return set_err(MC_ERR_ARG, "NULL argument");
+  if (!ctx->nframe) return set_err(MC_ERR_STATE, "no spectra loaded");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int n = ctx->n;
+  const size_t per = (size_t)(n / 2 + 1) * n;
+  HIP_TRY(ctx->tmp.ensure(sizeof(float2) * per));
+  for (int f = 0; f < ctx->nframe; f++) {
+    HIP_TRY(mc::launch_spec_to_padded(slot(ctx, f), n, ctx->tmp.as<float2>(), ctx->st));
+    HIP_TRY(hipMemcpyAsync(host_padded + (size_t)f * n * (n + 2), ctx->tmp.p,
+                           sizeof(float2) * per, hipMemcpyDeviceToHost, ctx->st));
+    HIP_TRY(hipStreamSynchronize(ctx->st));
+  }
+  return MC_OK;
+}
+
+extern "C" int mc_pair_shifts(mc_context ctx, const int* pairs, int npairs, float bfactor,
+                              int box, int wnoise, double* b, double* cc, float* boxmaps) {
+  if (!ctx || !pairs || npairs < 1) return set_err(MC_ERR_ARG, "bad pair arguments");
+  if (!ctx->nframe) return set_err(MC_ERR_STATE, "no spectra loaded");
+  if (box < 16 || box > ctx->n || box > 256)
+    return set_err(MC_ERR_ARG, "peak box %d must be in [16, min(n, 256)]", box);
+  HIP_TRY(hipSetDevice(ctx->device));
+  std::vector<int> s(2 * (size_t)npairs);
+  for (int q = 0; q < npairs; q++) {
+    if (pairs[2 * q] < 0 || pairs[2 * q] >= ctx->nframe || pairs[2 * q + 1] < 0 ||
+        pairs[2 * q + 1] >= ctx->nframe)
+      return set_err(MC_ERR_ARG, "pair %d out of range", q);
+    s[2 * q] = pairs[2 * q];
+    s[2 * q + 1] = pairs[2 * q + 1] + ctx->off2;
+  }
+  std::vector<double> res;
+  MC_TRY(pair_stage(ctx, s, bfactor, box, wnoise, res, boxmaps));
+  for (int q = 0; q < npairs; q++) {
+    if (b) {
+      b[2 * q] = res[3 * q];
+      b[2 * q + 1] = res[3 * q + 1];
+    }
+    if (cc) cc[q] = res[3 * q + 2];
+  }
+  return MC_OK;
+}
+
+extern "C" int mc_find_peaks(mc_context ctx, const float* boxmaps, int npairs, int box, int n,
+                             int wnoise, double* b, double* cc) {
+  if (!ctx || !boxmaps || npairs < 1 || box < 16) return set_err(MC_ERR_ARG, "bad arguments");
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(ctx->box.ensure(sizeof(float) * (size_t)npairs * box * box));
+  HIP_TRY(ctx->peaks.ensure(sizeof(double) * 3 * npairs));
+  HIP_TRY(hipMemcpyAsync(ctx->box.p, boxmaps, sizeof(float) * (size_t)npairs * box * box,
+                         hipMemcpyHostToDevice, ctx->st));
+  HIP_TRY(mc::launch_find_peaks(ctx->box.as<float>(), npairs, box, n, wnoise,
+                                ctx->peaks.as<double>(), ctx->st));
+  std::vector<double> res(3 * (size_t)npairs);
+  HIP_TRY(hipMemcpyAsync(res.data(), ctx->peaks.p, sizeof(double) * res.size(),
+                         hipMemcpyDeviceToHost, ctx->st));
+  HIP_TRY(hipStreamSynchronize(ctx->st));
+  for (int q = 0; q < npairs; q++) {
+    if (b) {
+      b[2 * q] = res[3 * q];
+      b[2 * q + 1] = res[3 * q + 1];
+    }
+    if (cc) cc[q] = res[3 * q + 2];
+  }
+  return MC_OK;
+}
+
+extern "C" int mc_sum_image(mc_context ctx, const double* frame_shift, float* image) {
+  if (!ctx || !frame_shift || !image) return set_err(MC_ERR_ARG, "NULL argument");
+  if (!ctx->nframe) return set_err(MC_ERR_STATE, "no spectra loaded");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int n = ctx->n;
+  std::vector<double> sh(frame_shift, frame_shift + 2 * (size_t)ctx->nframe);
+  MC_TRY(shift_sum(ctx, sh, slot(ctx, ctx->sum_slot), nullptr, -1));
+  HIP_TRY(ctx->image.ensure(sizeof(float) * (size_t)n * n));
+  MC_TRY(to_image(ctx, slot(ctx, ctx->sum_slot), ctx->image.as<float>()));
+  HIP_TRY(hipMemcpyAsync(image, ctx->image.p, sizeof(float) * (size_t)n * n,
+                         hipMemcpyDeviceToHost, ctx->st));
+  HIP_TRY(hipStreamSynchronize(ctx->st));
+  return MC_OK;
+}
+
+static float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;
+  return ms;
+}
+
+extern "C" int mc_align_movie(mc_context ctx, const mc_movie_desc* m, const mc_params* p,
+                              mc_outputs* out, mc_report* rep) {
+  if (!ctx || !m || !p) return set_err(MC_ERR_ARG, "NULL argument");
+  HIP_TRY(hipSetDevice(ctx->device));
+  mc_outputs none;
+  memset(&none, 0, sizeof none);
+  if (!out) out = &none;
+  hipEvent_t ev[8];
+  for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+  struct EvGuard {
+    hipEvent_t* e;
+    ~EvGuard() {
+      for (int i = 0; i < 8; i++) (void)hipEventDestroy(e[i]);
+    }
+  } guard{ev};
+  HIP_TRY(hipEventRecord(ev[0], ctx->st));
+
+  // 1. spectra (DFAlign.cu:705-824)
+  Geom g;
+  int F = 0;
+  MC_TRY(load_frames(ctx, m, p, &g, &F));
+  const int n = g.n;
+  const int box = p->CCPeakSearchDim;
+  if (box < 16 || box > n || box > 256)
+    return set_err(MC_ERR_ARG, "peak box %d must be in [16, min(n, 256)]", box);
+  const int wnoise = p->NoisePeakSize - 1;  // DFAlign.cu:900
+  HIP_TRY(ctx->means.ensure(sizeof(double) * F));
+  HIP_TRY(mc::launch_dc_gather(slot(ctx, 0), ctx->spec_stride, F, 1.0 / ((double)n * n),
+                               ctx->means.as<double>(), ctx->st));
+  if (out->frame_mean)
+    HIP_TRY(hipMemcpyAsync(out->frame_mean, ctx->means.p, sizeof(double) * F,
+                           hipMemcpyDeviceToHost, ctx->st));
+  HIP_TRY(ctx->image.ensure(sizeof(float) * (size_t)n * n));
+  int rawsum_written = 0;
+  if (out->rawsum_image) {
+    // DFAlign.cu:808-823,853-857: unaligned sum of frames [nStartSum, nEndSum]
+    std::vector<double> sh(2 * (size_t)F, NAN);
+    for (int f = 0; f < F; f++) {
+      const int j = f + p->nStart;
+      if (j >= p->nStartSum && j <= p->nEndSum) sh[2 * f] = sh[2 * f + 1] = 0.0;
+    }
+    MC_TRY(shift_sum(ctx, sh, slot(ctx, ctx->sum_slot), nullptr, -1));
+    MC_TRY(to_image(ctx, slot(ctx, ctx->sum_slot), ctx->image.as<float>()));
+    HIP_TRY(hipMemcpyAsync(out->rawsum_image, ctx->image.p, sizeof(float) * (size_t)n * n,
+                           hipMemcpyDeviceToHost, ctx->st));
+    rawsum_written = 1;
+  }
+  HIP_TRY(hipEventRecord(ev[1], ctx->st));
+
+  // 2. pairwise shifts (DFAlign.cu:875-905)
+  const int P = mc_pair_list(F, p->FrameDistOffset, nullptr, 0);
+  const int K = F - 1;
+  std::vector<int> pairs(2 * (size_t)P);
+  mc_pair_list(F, p->FrameDistOffset, pairs.data(), P);
+  std::vector<int> slots(pairs);
+  for (int q = 0; q < P; q++) slots[2 * q + 1] += ctx->off2;
+  std::vector<double> res;
+  MC_TRY(pair_stage(ctx, slots, p->bfactor, box, wnoise, res, out->boxmaps));
+  HIP_TRY(hipEventRecord(ev[2], ctx->st));
+  std::vector<double> b(2 * (size_t)P);
+  for (int q = 0; q < P; q++) {
+    b[2 * q] = res[3 * q];
+    b[2 * q + 1] = res[3 * q + 1];
+    if (out->pair_cc) out->pair_cc[q] = res[3 * q + 2];
+  }
+  if (out->pairs) memcpy(out->pairs, pairs.data(), sizeof(int) * 2 * P);
+  if (out->pair_b) memcpy(out->pair_b, b.data(), sizeof(double) * 2 * P);
+
+  // 3. least squares, outlier rejection, least squares (DFAlign.cu:914-948)
+  std::vector<double> s(2 * (size_t)K), ki(P);
+  MC_TRY(mc_lsq(pairs.data(), P, F, b.data(), s.data()));
+  residuals(pairs.data(), P, s.data(), b.data(), ki.data());
+  if (out->shift1) memcpy(out->shift1, s.data(), sizeof(double) * 2 * K);
+  if (out->ki1) memcpy(out->ki1, ki.data(), sizeof(double) * P);
+  std::vector<int> good(P);
+  const int ng = mc_threshold(pairs.data(), P, F, ki.data(), p->kiThresh, good.data());
+  std::vector<int> gp(2 * (size_t)ng);
+  std::vector<double> gb(2 * (size_t)ng);
+  for (int q = 0; q < ng; q++) {
+    gp[2 * q] = pairs[2 * good[q]];
+    gp[2 * q + 1] = pairs[2 * good[q] + 1];
+    gb[2 * q] = b[2 * good[q]];
+    gb[2 * q + 1] = b[2 * good[q] + 1];
+  }
+  MC_TRY(mc_lsq(gp.data(), ng, F, gb.data(), s.data()));
+  if (out->good) memcpy(out->good, good.data(), sizeof(int) * ng);
+  if (out->shift2) memcpy(out->shift2, s.data(), sizeof(double) * 2 * K);
+  if (out->ki2) residuals(gp.data(), ng, s.data(), gb.data(), out->ki2);
+  HIP_TRY(hipEventRecord(ev[3], ctx->st));
+
+  // 4. iterative alignment to the sum (DFAlign.cu:950-1071)
+  std::vector<double> oldx(2 * (size_t)F, 0.0), newx(2 * (size_t)F, 0.0);
+  {
+    double ax = 0, ay = 0;
+    for (int k = 0; k < K; k++) {
+      newx[2 * (k + 1)] = ax + s[2 * k];
+      newx[2 * (k + 1) + 1] = ay + s[2 * k + 1];
+      ax += s[2 * k];
+      ay += s[2 * k + 1];
+    }
+  }
+  oldx = newx;
+  double rmsd = 0;
+  int it = 0, converged = 0;
+  std::vector<int> islots(2 * (size_t)F);
+  for (int k = 0; k < F; k++) {
+    islots[2 * k] = ctx->sum_slot;  // GPUShiftCC(dfft=frame, dsum=sum): sum * conj(frame)
+    islots[2 * k + 1] = k;
+  }
+  for (it = 0; it < p->iter && !converged; it++) {
+    std::vector<double> sh(2 * (size_t)F);
+    for (int k = 0; k < 2 * F; k++) sh[k] = -oldx[k];
+    MC_TRY(shift_sum(ctx, sh, slot(ctx, ctx->sum_slot), nullptr, -1));
+    MC_TRY(pair_stage(ctx, islots, p->bfactor, box, wnoise, res, nullptr));
+    for (int k = 0; k < F; k++) {
+      newx[2 * k] = res[3 * k];
+      newx[2 * k + 1] = res[3 * k + 1];
+    }
+    if (out->iter_meas) memcpy(out->iter_meas + (size_t)it * 2 * F, newx.data(), sizeof(double) * 2 * F);
+    const double rx = newx[0], ry = newx[1];
+    for (int k = 0; k < F; k++) {
+      const double dx = oldx[2 * k] - newx[2 * k], dy = oldx[2 * k + 1] - newx[2 * k + 1];
+      oldx[2 * k] = newx[2 * k] - rx;
+      oldx[2 * k + 1] = newx[2 * k + 1] - ry;
+      rmsd += dx * dx + dy * dy;
+    }
+    rmsd /= F;  // accumulates across rounds as the reference does (DFAlign.cu:1036-1041)
+    if (out->rmsd) out->rmsd[it] = rmsd;
+    if (rmsd < (double)p->iterthres * (double)p->iterthres) converged = 1;
+  }
+  for (int k = 0; k < K; k++) {
+    s[2 * k] = oldx[2 * (k + 1)] - oldx[2 * k];
+    s[2 * k + 1] = oldx[2 * (k + 1) + 1] - oldx[2 * k + 1];
+  }
+  if (out->final_shift) memcpy(out->final_shift, s.data(), sizeof(double) * 2 * K);
+  HIP_TRY(hipEventRecord(ev[4], ctx->st));
+
+  // 5. final phase-shift-and-sum (DFAlign.cu:1114-1292)
+  int nss = p->nStartSum - p->nStart, nes = p->nEndSum - p->nStart, nes2 = p->nEndSum2 - p->nStart;
+  if (nss <= 0) nss = 0;
+  if (nes <= nss || nes >= F) nes = F - 1;
+  if (nes2 <= nss || nes2 >= F) nes2 = F - 1;
+  if (nes2 > nes) nes2 = nes;
+  double midx = 0, midy = 0;
+  if (p->bAlignToMid) {
+    const int nzf = m->nz_file > 0 ? m->nz_file : m->nz;
+    int ref = nzf / 2 + 1;
+    if (ref < p->nStart) ref = p->nStart;
+    const int lim = std::min(ref - p->nStart, K);  // guards DFAlign.cu:1138
+    for (int j = 0; j < lim; j++) {
+      midx += s[2 * j];
+      midy += s[2 * j + 1];
+    }
+  }
+  std::vector<double> fs(2 * (size_t)F, NAN);
+  {
+    double tx = 0, ty = 0;
+    for (int j = 1; j < nss + 1; j++) {
+      tx += s[2 * (j - 1)];
+      ty += s[2 * (j - 1) + 1];
+    }
+    // first summed frame: shifted only with -atm 1 (DFAlign.cu:1148)
+    fs[2 * nss] = p->bAlignToMid ? -tx + midx : 0.0;
+    fs[2 * nss + 1] = p->bAlignToMid ? -ty + midy : 0.0;
+    for (int j = nss + 1; j <= nes; j++) {
+      tx += s[2 * (j - 1)];
+      ty += s[2 * (j - 1) + 1];
+      fs[2 * j] = -tx + midx;
+      fs[2 * j + 1] = -ty + midy;
+    }
+  }
+  if (out->frame_shift) memcpy(out->frame_shift, fs.data(), sizeof(double) * 2 * F);
+  const bool want2 = out->sum2_image && p->nEndSum2 > 0 && p->nEndSum2 < p->nEndSum;
+  float2* s2 = nullptr;
+  if (want2) {
+    HIP_TRY(ctx->tmp.ensure(sizeof(float2) * ctx->spec_stride * 2));
+    s2 = ctx->tmp.as<float2>() + ctx->spec_stride;  // second half of tmp
+  }
+  MC_TRY(shift_sum(ctx, fs, slot(ctx, ctx->sum_slot), s2, nes2));
+  float* img = out->sum_image_dev ? static_cast<float*>(out->sum_image_dev) : ctx->image.as<float>();
+  MC_TRY(to_image(ctx, slot(ctx, ctx->sum_slot), img));
+  if (out->sum_image)
+    HIP_TRY(hipMemcpyAsync(out->sum_image, img, sizeof(float) * (size_t)n * n,
+                           hipMemcpyDeviceToHost, ctx->st));
+  int sum2_written = 0;
+  if (want2) {
+    // the second sum shares tmp: copy it into a spectrum slot first
+    HIP_TRY(hipMemcpyAsync(slot(ctx, ctx->sum_slot), s2, sizeof(float2) * ctx->spec_stride,
+                           hipMemcpyDeviceToDevice, ctx->st));
+    HIP_TRY(ctx->sum2img.ensure(sizeof(float) * (size_t)n * n));
+    MC_TRY(to_image(ctx, slot(ctx, ctx->sum_slot), ctx->sum2img.as<float>()));
+    HIP_TRY(hipMemcpyAsync(out->sum2_image, ctx->sum2img.p, sizeof(float) * (size_t)n * n,
+                           hipMemcpyDeviceToHost, ctx->st));
+    sum2_written = 1;
+  }
+  HIP_TRY(hipEventRecord(ev[5], ctx->st));
+  HIP_TRY(hipEventSynchronize(ev[5]));
+  if (rep) {
+    memset(rep, 0, sizeof *rep);
+    rep->n = n;
+    rep->nsamUnbin = g.nsu;
+    rep->nframe = F;
+    rep->npairs = P;
+    rep->ngood = ng;
+    rep->niter = it;
+    rep->converged = converged;
+    rep->rmsd = rmsd;
+    rep->sum2_written = sum2_written;
+    rep->rawsum_written = rawsum_written;
+    rep->ms_spectra = elapsed(ev[0], ev[1]);
+    rep->ms_pairs = elapsed(ev[1], ev[2]);
+    rep->ms_solve = elapsed(ev[2], ev[3]);
+    rep->ms_iter = elapsed(ev[3], ev[4]);
+    rep->ms_sum = elapsed(ev[4], ev[5]);
+    rep->ms_total = elapsed(ev[0], ev[5]);
+  }
+  return MC_OK;
+}

@@ -1,0 +1,213 @@
+// mc_device.h -- device-side building blocks shared by the HIP kernels:
+// complex helpers, register DFT butterflies, the LDS Stockham FFT engine and
+// XCD-aware block remapping for gfx950 (CDNA4, wave64, 160 KB LDS per WG).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MC_DEV __device__ __forceinline__
+
+namespace mc {
+
+// ---------------------------------------------------------------- complex
+MC_DEV float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+MC_DEV float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+MC_DEV float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+MC_DEV float2 cmulc(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+MC_DEV float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+MC_DEV float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// a * (S i), S = +1 or -1
+template <int S> MC_DEV float2 mul_si(float2 a) {
+  return S > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// twiddle e^{S * 2 pi i e / N} from a table t[e] = e^{+2 pi i e / N}
+template <int S> MC_DEV float2 twid(const float2* t, int e) {
+  float2 w = t[e];
+  return S > 0 ? w : make_float2(w.x, -w.y);
+}
+
+// ------------------------------------------------------ register butterflies
+// In-place DFT of R registers, X[k] = sum_r v[r] e^{S 2 pi i r k / R}.
+template <int S> MC_DEV void dft2(float2* v) {
+  float2 a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+
+template <int S> MC_DEV void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+  float2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
+  float2 t2 = cadd(a1, a3), t3 = mul_si<S>(csub(a1, a3));
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  a1 = cadd(t1, t3);
+  a3 = csub(t1, t3);
+}
+template <int S> MC_DEV void dft4(float2* v) { dft4<S>(v[0], v[1], v[2], v[3]); }
+
+template <int S> MC_DEV void dft8(float2* v) {
+  const float c = 0.70710678118654752440f;
+  float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  dft4<S>(e0, e1, e2, e3);
+  dft4<S>(o0, o1, o2, o3);
+  // W8^1 = (c, S c), W8^2 = S i, W8^3 = (-c, S c)
+  o1 = cmul(o1, make_float2(c, S * c));
+  o2 = mul_si<S>(o2);
+  o3 = cmul(o3, make_float2(-c, S * c));
+  v[0] = cadd(e0, o0);
+  v[4] = csub(e0, o0);
+  v[1] = cadd(e1, o1);
+  v[5] = csub(e1, o1);
+  v[2] = cadd(e2, o2);
+  v[6] = csub(e2, o2);
+  v[3] = cadd(e3, o3);
+  v[7] = csub(e3, o3);
+}
+
+template <int S> MC_DEV void dft16(float2* v) {
+  // 16 = 4 x 4: X[k1 + 4 k2] = sum_n2 W4^{n2 k2} W16^{n2 k1} sum_n1 v[n2 + 4 n1] W4^{n1 k1}
+  const float c1 = 0.92387953251128675613f;  // cos(pi/8)
+  const float s1 = 0.38268343236508977173f;  // sin(pi/8)
+  const float c2 = 0.70710678118654752440f;
+  float2 a[4][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; n2++) {
+    a[n2][0] = v[n2];
+    a[n2][1] = v[n2 + 4];
+    a[n2][2] = v[n2 + 8];
+    a[n2][3] = v[n2 + 12];
+    dft4<S>(a[n2][0], a[n2][1], a[n2][2], a[n2][3]);
+  }
+  // twiddles W16^{n2 k1}, exponent e = n2*k1 in {0..9}
+  // W16^1 = (c1, S s1), ^2 = (c2, S c2), ^3 = (s1, S c1), ^4 = S i,
+  // ^6 = (-c2, S c2), ^9 = (-c1, -S s1)
+  a[1][1] = cmul(a[1][1], make_float2(c1, S * s1));
+  a[1][2] = cmul(a[1][2], make_float2(c2, S * c2));
+  a[1][3] = cmul(a[1][3], make_float2(s1, S * c1));
+  a[2][1] = cmul(a[2][1], make_float2(c2, S * c2));
+  a[2][2] = mul_si<S>(a[2][2]);
+  a[2][3] = cmul(a[2][3], make_float2(-c2, S * c2));
+  a[3][1] = cmul(a[3][1], make_float2(s1, S * c1));
+  a[3][2] = cmul(a[3][2], make_float2(-c2, S * c2));
+  a[3][3] = cmul(a[3][3], make_float2(-c1, -S * s1));
+#pragma unroll
+  for (int k1 = 0; k1 < 4; k1++) {
+    float2 b0 = a[0][k1], b1 = a[1][k1], b2 = a[2][k1], b3 = a[3][k1];
+    dft4<S>(b0, b1, b2, b3);
+    v[k1] = b0;
+    v[k1 + 4] = b1;
+    v[k1 + 8] = b2;
+    v[k1 + 12] = b3;
+  }
+}
+
+template <int R, int S> MC_DEV void dft(float2* v) {
+  if constexpr (R == 2) dft2<S>(v);
+  else if constexpr (R == 4) dft4<S>(v);
+  else if constexpr (R == 8) dft8<S>(v);
+  else if constexpr (R == 16) dft16<S>(v);
+  else static_assert(R == 2 || R == 4 || R == 8 || R == 16, "radix");
+}
+
+// ---------------------------------------------------- Stockham stage plans
+// Radix of stage s for a power-of-two length N (balanced plans).
+__host__ __device__ constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
+__host__ __device__ constexpr int stage_count(int N) {
+  return N <= 16 ? 1 : (N == 32 || N == 64 || N == 128 || N == 256) ? 2 : (N <= 4096 ? 3 : 4);
+}
+__host__ __device__ constexpr int stage_radix(int N, int s) {
+  // 16:[16] 32:[8,4] 64:[8,8] 128:[16,8] 256:[16,16] 512:[8,8,8]
+  // 1024:[16,8,8] 2048:[16,16,8] 4096:[16,16,16] 8192:[16,16,8,4]
+  return N <= 16 ? N
+       : N == 32 ? (s == 0 ? 8 : 4)
+       : N == 64 ? 8
+       : N == 128 ? (s == 0 ? 16 : 8)
+       : N == 256 ? 16
+       : N == 512 ? 8
+       : N == 1024 ? (s == 0 ? 16 : 8)
+       : N == 2048 ? (s < 2 ? 16 : 8)
+       : N == 4096 ? 16
+       : (s < 2 ? 16 : (s == 2 ? 8 : 4));
+}
+__host__ __device__ constexpr int stage_ns(int N, int s) {
+  return s == 0 ? 1 : stage_ns(N, s - 1) * stage_radix(N, s - 1);
+}
+
+// One Stockham butterfly of stage (R, Ns) for transform length N at index j
+// (0 <= j < N/R): gather v[r] = x[j + r N/R], twiddle by W_{Ns R}^{(j%Ns) r},
+// DFT_R.  tw: table of e^{+2 pi i e/NTW}, NTW a multiple of N.
+template <int N, int R, int Ns, int S>
+MC_DEV void stockham_bfly(float2* v, int j, const float2* tw, int ntw) {
+  if constexpr (Ns > 1) {
+    const int k = j % Ns;
+    const int step = ntw / (Ns * R);
+#pragma unroll
+    for (int r = 1; r < R; r++) v[r] = cmul(v[r], twid<S>(tw, (k * r * step) & (ntw - 1)));
+  }
+  dft<R, S>(v);
+}
+template <int R, int Ns> MC_DEV int stockham_out(int j) {
+  return (j / Ns) * Ns * R + (j % Ns);
+}
+
+// Full in-LDS FFT of NB transforms of length N (buf[b*TS + i]), NT threads.
+// Data and result in natural order in LDS.  Caller synchronises before.
+template <int N, int NT, int S, int NB, int STAGE, int TS>
+MC_DEV void fft_lds_stage(float2* buf, const float2* tw, int ntw) {
+  constexpr int R = stage_radix(N, STAGE);
+  constexpr int Ns = stage_ns(N, STAGE);
+  constexpr int NBF = NB * (N / R);
+  constexpr int PER = (NBF + NT - 1) / NT;
+  float2 v[PER][R];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int p = 0; p < PER; p++) {
+    const int q = tid + p * NT;
+    if (NBF % NT == 0 || q < NBF) {
+      const int b = q / (N / R), j = q % (N / R);
+#pragma unroll
+      for (int r = 0; r < R; r++) v[p][r] = buf[b * TS + j + r * (N / R)];
+      stockham_bfly<N, R, Ns, S>(v[p], j, tw, ntw);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < PER; p++) {
+    const int q = tid + p * NT;
+    if (NBF % NT == 0 || q < NBF) {
+      const int b = q / (N / R), j = q % (N / R);
+      const int o = stockham_out<R, Ns>(j);
+#pragma unroll
+      for (int r = 0; r < R; r++) buf[b * TS + o + r * Ns] = v[p][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <int N, int NT, int S, int NB, int TS = N>
+MC_DEV void fft_lds(float2* buf, const float2* tw, int ntw) {
+  fft_lds_stage<N, NT, S, NB, 0, TS>(buf, tw, ntw);
+  if constexpr (stage_count(N) > 1) fft_lds_stage<N, NT, S, NB, 1, TS>(buf, tw, ntw);
+  if constexpr (stage_count(N) > 2) fft_lds_stage<N, NT, S, NB, 2, TS>(buf, tw, ntw);
+  if constexpr (stage_count(N) > 3) fft_lds_stage<N, NT, S, NB, 3, TS>(buf, tw, ntw);
+}
+
+// --------------------------------------------------------- XCD remapping
+// gfx950 dispatches consecutive workgroups round-robin over 8 XCDs, each
+// with its own L2.  Remap so that a contiguous range of logical block ids
+// runs on one XCD in order: neighbouring logical blocks then share L2.
+MC_DEV int xcd_remap(int b, int nblocks) {
+  const int nx = 8;
+  const int per = nblocks / nx;  // blocks per XCD (full rounds)
+  const int full = per * nx;
+  if (b >= full) return b;       // tail: identity
+  return (b % nx) * per + b / nx;
+}
+
+}  // namespace mc

@@ -1,0 +1,299 @@
+// mc_fft.hip -- stage 1 (frames -> spectra) and sum-image inverse transforms.
+//
+// Spectrum layout in HBM ("column-major half spectrum"): S[kx][ky], kx in
+// [0, n/2], ky in [0, n), complex float, ky contiguous.  It holds the same
+// values as the cuFFT in-place R2C result the reference keeps on the host
+// (reference cufunc.cu:272-276 called at DFAlign.cu:772,797), transposed so
+// that every consumer of the alignment path (pair correlation, phase shift
+// and sum) streams contiguous ky vectors.
+//
+// Forward 2-D R2C of an nu x nu canvas in two passes:
+//  k_r2c_rows : ROWS image rows per workgroup; each real row is packed into
+//               nu/2 complex values, transformed in LDS (Stockham), unpacked
+//               to nu/2+1 bins, and only the bins kept after Fourier-crop
+//               binning (kx <= n/2) are stored, transposed, to RT[kx][y].
+//               Crop offset, element decode (int8/uint8/int16/uint16/f32)
+//               and zero fill outside the data region are fused in the load.
+//  k_fill     : per-frame mean of the data region (pad geometry only).
+//  k_c2c_cols : one column kx per workgroup: adds fill * (spectrum of the
+//               fill mask), transforms along y in LDS, keeps the rows of
+//               the Fourier crop (reference cufunc.cu:211-225) and writes
+//               S[kx][ky] contiguously.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "mc_device.h"
+#include "mc_kernels.h"
+
+namespace mc {
+
+// ----------------------------------------------------------- twiddle cache
+const float2* twiddles(int n) {
+  static std::mutex mu;
+  static std::map<std::pair<int, int>, float2*> cache;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(mu);
+  auto key = std::make_pair(dev, n);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  std::vector<float2> h(n);
+  for (int e = 0; e < n; e++) {
+    double a = 2.0 * M_PI * (double)e / (double)n;
+    h[e] = make_float2((float)cos(a), (float)sin(a));
+  }
+  float2* d = nullptr;
+  if (hipMalloc(&d, sizeof(float2) * n) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    return nullptr;
+  }
+  cache[key] = d;
+  return d;
+}
+
+// ----------------------------------------------------------------- loads
+MC_DEV float load_elem(const char* row, int dtype, int x) {
+  switch (dtype) {
+    case DT_INT8: return (float)reinterpret_cast<const int8_t*>(row)[x];
+    case DT_UINT8: return (float)reinterpret_cast<const uint8_t*>(row)[x];
+    case DT_INT16: return (float)reinterpret_cast<const int16_t*>(row)[x];
+    case DT_UINT16: return (float)reinterpret_cast<const uint16_t*>(row)[x];
+    default: return reinterpret_cast<const float*>(row)[x];
+  }
+}
+
+// ------------------------------------------------------------- row pass
+template <int NU, int ROWS, int NT>
+__global__ __launch_bounds__(NT) void k_r2c_rows(R2CGeom g, int f0, int nkx, float2* rt,
+                                                 size_t rt_fstride, float* rowsum,
+                                                 const float2* __restrict__ tw) {
+  constexpr int N = NU / 2;
+  __shared__ float2 buf[ROWS * N];
+  float* rb = reinterpret_cast<float*>(buf);
+  const int f = blockIdx.y;
+  const int y0 = blockIdx.x * ROWS;
+  const char* fbase = g.src + (size_t)(f0 + f) * g.frame_pitch;
+  for (int i = threadIdx.x; i < ROWS * NU; i += NT) {
+    const int r = i / NU, x = i % NU;
+    const int y = y0 + r;
+    float v = 0.f;
+    if (y < g.h && x < g.w)
+      v = load_elem(fbase + (size_t)(y + g.offy) * g.row_pitch, g.dtype, x + g.offx);
+    rb[i] = v;
+  }
+  __syncthreads();
+  fft_lds<N, NT, -1, ROWS>(buf, tw, NU);
+  // unpack: X[k] = E[k] + e^{-2 pi i k/NU} O[k],
+  // E = (Z[k] + conj Z[N-k]) / 2, O = (Z[k] - conj Z[N-k]) / 2i
+  float2* out = rt + (size_t)f * rt_fstride;
+  for (int i = threadIdx.x; i < ROWS * nkx; i += NT) {
+    const int r = i % ROWS, k = i / ROWS;
+    const int y = y0 + r;
+    if (y >= g.h) continue;
+    const float2* Z = buf + r * N;
+    const float2 zk = Z[k & (N - 1)];
+    const float2 zc = cconj(Z[(N - k) & (N - 1)]);
+    const float2 E = cscale(cadd(zk, zc), 0.5f);
+    const float2 D = csub(zk, zc);
+    const float2 O = make_float2(0.5f * D.y, -0.5f * D.x);
+    const float2 X = cadd(E, cmul(twid<-1>(tw, k), O));
+    out[(size_t)k * NU + y] = X;
+    if (k == 0 && rowsum) rowsum[(size_t)f * NU + y] = X.x;
+  }
+}
+
+// ------------------------------------------------- per-frame fill value
+__global__ __launch_bounds__(256) void k_fill(const float* rowsum, int nu, int h, int w,
+                                              float* fill) {
+  __shared__ double red[256];
+  const int f = blockIdx.x;
+  double s = 0;
+  for (int y = threadIdx.x; y < h; y += 256) s += rowsum[(size_t)f * nu + y];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) fill[f] = (float)(red[0] / ((double)w * (double)h));
+}
+
+// ---------------------------------------------------------- column pass
+template <int NU, int NT>
+__global__ __launch_bounds__(NT) void k_c2c_cols(const float2* rt, size_t rt_fstride, int nkx,
+                                                 int h_data, const float* fill,
+                                                 const float2* __restrict__ padP, int n,
+                                                 float2* spec, size_t spec_stride,
+                                                 const float2* __restrict__ tw) {
+  __shared__ float2 buf[NU];
+  const int L = xcd_remap(blockIdx.x, gridDim.x);
+  const int kx = L % nkx, f = L / nkx;
+  const float2* src = rt + (size_t)f * rt_fstride + (size_t)kx * NU;
+  float fv = 0.f;
+  float2 P = make_float2(0.f, 0.f);
+  if (fill) {
+    fv = fill[f];
+    P = padP[kx];
+  }
+  for (int i = threadIdx.x; i < NU; i += NT) {
+    float2 v = make_float2(0.f, 0.f);
+    if (i < h_data) {
+      v = src[i];
+      if (fill) v = cadd(v, cscale(P, fv));
+    } else if (fill && kx == 0) {
+      v.x = fv * (float)NU;
+    }
+    buf[i] = v;
+  }
+  __syncthreads();
+  fft_lds<NU, NT, -1, 1>(buf, tw, NU);
+  float2* dst = spec + (size_t)f * spec_stride + (size_t)kx * n;
+  for (int i = threadIdx.x; i < n; i += NT) {
+    const int si = i < n / 2 ? i : NU - n + i;
+    dst[i] = buf[si];
+  }
+}
+
+template <int NU>
+static hipError_t frames_to_spectra_t(const R2CGeom& g, int f0, int nframes, float2* spectra,
+                                      size_t spec_stride, float2* rt, float* rowsum,
+                                      float* fill, const float2* padP, hipStream_t st) {
+  constexpr int ROWS = 4;
+  constexpr int N = NU / 2;
+  constexpr int NTR = (ROWS * N / 16) < 64 ? 64 : ((ROWS * N / 16) > 256 ? 256 : ROWS * N / 16);
+  constexpr int NTC = (NU / 16) < 64 ? 64 : ((NU / 16) > 256 ? 256 : NU / 16);
+  const float2* tw = twiddles(NU);
+  if (!tw) return hipErrorOutOfMemory;
+  const int nkx = g.n / 2 + 1;
+  const size_t rt_fstride = (size_t)nkx * NU;
+  dim3 gr((g.h + ROWS - 1) / ROWS, nframes);
+  hipLaunchKernelGGL((k_r2c_rows<NU, ROWS, NTR>), gr, dim3(NTR), 0, st, g, f0, nkx, rt,
+                     rt_fstride, g.pad_mode ? rowsum : nullptr, tw);
+  if (g.pad_mode) {
+    hipLaunchKernelGGL(k_fill, dim3(nframes), dim3(256), 0, st, rowsum, NU, g.h, g.w, fill);
+  }
+  hipLaunchKernelGGL((k_c2c_cols<NU, NTC>), dim3(nkx * nframes), dim3(NTC), 0, st, rt,
+                     rt_fstride, nkx, g.h, g.pad_mode ? fill : nullptr, padP, g.n, spectra,
+                     spec_stride, tw);
+  return hipGetLastError();
+}
+
+hipError_t launch_frames_to_spectra(const R2CGeom& g, int f0, int nframes, float2* spectra,
+                                    size_t spec_stride, float2* rt, float* rowsum, float* fill,
+                                    const float2* padP, hipStream_t st) {
+  switch (g.nu) {
+    case 64: return frames_to_spectra_t<64>(g, f0, nframes, spectra, spec_stride, rt, rowsum, fill, padP, st);
+    case 128: return frames_to_spectra_t<128>(g, f0, nframes, spectra, spec_stride, rt, rowsum, fill, padP, st);
+    case 256: return frames_to_spectra_t<256>(g, f0, nframes, spectra, spec_stride, rt, rowsum, fill, padP, st);
+    case 512: return frames_to_spectra_t<512>(g, f0, nframes, spectra, spec_stride, rt, rowsum, fill, padP, st);
+    case 1024: return frames_to_spectra_t<1024>(g, f0, nframes, spectra, spec_stride, rt, rowsum, fill, padP, st);
+    case 2048: return frames_to_spectra_t<2048>(g, f0, nframes, spectra, spec_stride, rt, rowsum, fill, padP, st);
+    case 4096: return frames_to_spectra_t<4096>(g, f0, nframes, spectra, spec_stride, rt, rowsum, fill, padP, st);
+    case 8192: return frames_to_spectra_t<8192>(g, f0, nframes, spectra, spec_stride, rt, rowsum, fill, padP, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// =================================================== inverse (sum images)
+// Column pass: inverse transform of S[kx][:] along ky, stored row-major
+// tmp[ky][kx] (pitch n/2+1).
+template <int N, int NT>
+__global__ __launch_bounds__(NT) void k_cols_inv(const float2* spec, float2* tmp,
+                                                 const float2* __restrict__ tw) {
+  __shared__ float2 buf[N];
+  constexpr int H = N / 2 + 1;
+  const int kx = blockIdx.x;
+  const float2* src = spec + (size_t)kx * N;
+  for (int i = threadIdx.x; i < N; i += NT) buf[i] = src[i];
+  __syncthreads();
+  fft_lds<N, NT, 1, 1>(buf, tw, N);
+  for (int i = threadIdx.x; i < N; i += NT) tmp[(size_t)i * H + kx] = buf[i];
+}
+
+// Row pass: C2R of ROWS rows of tmp with the DC/Nyquist projection
+// (imaginary parts of bins 0 and N/2 ignored), x *= scale.
+template <int N, int ROWS, int NT>
+__global__ __launch_bounds__(NT) void k_c2r_rows(const float2* tmp, float scale, float* img,
+                                                 size_t pitch, const float2* __restrict__ tw) {
+  constexpr int NH = N / 2;        // packed complex length
+  constexpr int H = N / 2 + 1;     // bins per row
+  constexpr int TS = NH + 1;       // LDS row stride
+  __shared__ float2 buf[ROWS * TS];
+  const int y0 = blockIdx.x * ROWS;
+  for (int i = threadIdx.x; i < ROWS * H; i += NT) {
+    const int r = i / H, k = i % H;
+    float2 v = tmp[(size_t)(y0 + r) * H + k];
+    if (k == 0 || k == NH) v.y = 0.f;
+    buf[r * TS + k] = v;
+  }
+  __syncthreads();
+  // Z[k] = (X[k] + conj X[NH-k]) + i e^{+2 pi i k/N} (X[k] - conj X[NH-k])
+  constexpr int NP = NH / 2 + 1;  // pairs (k, NH-k), k in [0, NH/2]
+  float2 za[(ROWS * NP + NT - 1) / NT], zb[(ROWS * NP + NT - 1) / NT];
+#pragma unroll
+  for (int p = 0; p < (ROWS * NP + NT - 1) / NT; p++) {
+    const int i = threadIdx.x + p * NT;
+    if (i < ROWS * NP) {
+      const int r = i / NP, k = i % NP;
+      const float2 a = buf[r * TS + k], b = buf[r * TS + NH - k];
+      const float2 s1 = cadd(a, cconj(b)), d1 = csub(a, cconj(b));
+      za[p] = cadd(s1, mul_si<1>(cmul(tw[k], d1)));
+      const float2 s2 = cadd(b, cconj(a)), d2 = csub(b, cconj(a));
+      zb[p] = cadd(s2, mul_si<1>(cmul(tw[NH - k], d2)));
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < (ROWS * NP + NT - 1) / NT; p++) {
+    const int i = threadIdx.x + p * NT;
+    if (i < ROWS * NP) {
+      const int r = i / NP, k = i % NP;
+      buf[r * TS + k] = za[p];
+      if (k != 0 && k != NH - k) buf[r * TS + NH - k] = zb[p];
+    }
+  }
+  __syncthreads();
+  fft_lds<NH, NT, 1, ROWS, TS>(buf, tw, N);
+  for (int i = threadIdx.x; i < ROWS * NH; i += NT) {
+    const int r = i / NH, m = i % NH;
+    const float2 z = buf[r * TS + m];
+    reinterpret_cast<float2*>(img + (size_t)(y0 + r) * pitch)[m] =
+        make_float2(z.x * scale, z.y * scale);
+  }
+}
+
+template <int N>
+static hipError_t spectrum_to_image_t(const float2* spec, float scale, float* img, size_t pitch,
+                                      float2* tmp, hipStream_t st) {
+  constexpr int NTC = (N / 16) < 64 ? 64 : ((N / 16) > 256 ? 256 : N / 16);
+  constexpr int ROWS = 4;
+  constexpr int NTR = (ROWS * N / 32) < 64 ? 64 : ((ROWS * N / 32) > 256 ? 256 : ROWS * N / 32);
+  const float2* tw = twiddles(N);
+  if (!tw) return hipErrorOutOfMemory;
+  hipLaunchKernelGGL((k_cols_inv<N, NTC>), dim3(N / 2 + 1), dim3(NTC), 0, st, spec, tmp, tw);
+  hipLaunchKernelGGL((k_c2r_rows<N, ROWS, NTR>), dim3(N / ROWS), dim3(NTR), 0, st, tmp, scale,
+                     img, pitch, tw);
+  return hipGetLastError();
+}
+
+hipError_t launch_spectrum_to_image(const float2* spec, int n, float scale, float* img,
+                                    size_t img_pitch, float2* tmp, hipStream_t st) {
+  switch (n) {
+    case 64: return spectrum_to_image_t<64>(spec, scale, img, img_pitch, tmp, st);
+    case 128: return spectrum_to_image_t<128>(spec, scale, img, img_pitch, tmp, st);
+    case 256: return spectrum_to_image_t<256>(spec, scale, img, img_pitch, tmp, st);
+    case 512: return spectrum_to_image_t<512>(spec, scale, img, img_pitch, tmp, st);
+    case 1024: return spectrum_to_image_t<1024>(spec, scale, img, img_pitch, tmp, st);
+    case 2048: return spectrum_to_image_t<2048>(spec, scale, img, img_pitch, tmp, st);
+    case 4096: return spectrum_to_image_t<4096>(spec, scale, img, img_pitch, tmp, st);
+    case 8192: return spectrum_to_image_t<8192>(spec, scale, img, img_pitch, tmp, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace mc

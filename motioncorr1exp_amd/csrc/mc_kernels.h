@@ -1,0 +1,88 @@
+// mc_kernels.h -- host-side launch wrappers of the HIP kernels (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace mc {
+
+// Element type codes of frame data (match MC_DTYPE_* in include/mc_abi.h).
+enum { DT_INT8 = 0, DT_INT16 = 1, DT_FLOAT32 = 2, DT_UINT16 = 6, DT_UINT8 = 100 };
+
+// Geometry of one stage-1 launch (frames -> spectra), see mc_fft.hip.
+struct R2CGeom {
+  const char* src;      // device frames
+  int dtype;
+  size_t row_pitch;     // bytes
+  size_t frame_pitch;   // bytes
+  int offx, offy;       // region origin in the frame
+  int w, h;             // data region (<= nu); the rest of the canvas is fill
+  int nu;               // canvas (unbinned) size, power of two
+  int n;                // binned size (<= nu), even
+  int pad_mode;         // 1: canvas outside the region holds the frame mean
+};
+
+// Twiddle table e^{+2 pi i e/n}, e in [0, n), computed in double on the host.
+const float2* twiddles(int n);
+
+// Stage 1: spectra[f] (layout [kx][ky], kx in [0, n/2], ky in [0, n)) of
+// nframes frames starting at frame index f0.  rt: scratch of
+// nframes*(n/2+1)*nu float2; rowsum/fill: scratch floats (nframes*nu, nframes).
+hipError_t launch_frames_to_spectra(const R2CGeom& g, int f0, int nframes, float2* spectra,
+                                    size_t spec_stride, float2* rt, float* rowsum,
+                                    float* fill, const float2* padP, hipStream_t st);
+
+// Inverse 2-D transform of one spectrum ([kx][ky] layout, size n) to an
+// n x n real image scaled by `scale` (C2R with the DC/Nyquist projection).
+// tmp: scratch of (n/2+1)*n float2.
+hipError_t launch_spectrum_to_image(const float2* spec, int n, float scale, float* img,
+                                    size_t img_pitch, float2* tmp, hipStream_t st);
+
+// Pair stage: for each pair p, T[p][kx][r] (column pass) then the box map
+// C[p][r][c] (row pass).  pairs: device int2 (a, b) indices into `spectra`
+// (frame slots of stride spec_stride); operand a is kept, b conjugated.
+hipError_t launch_pair_cols(const float2* spectra, size_t spec_stride, const int2* pairs,
+                            int npairs, int n, float bfactor, int box, float2* tbuf,
+                            hipStream_t st);
+hipError_t launch_pair_rows(const float2* tbuf, int npairs, int n, int box, float* boxmaps,
+                            hipStream_t st);
+
+// Peaks: integer peak + FP64 16x16 -> 512x512 sub-pixel search per box map.
+// out: 3 doubles per pair (sx, sy, cc).
+hipError_t launch_find_peaks(const float* boxmaps, int npairs, int box, int n, int wnoise,
+                             double* out, hipStream_t st);
+
+// Phase-shift-and-sum: sum[kx][ky] = sum_f include_f * S_f[kx][ky] *
+// exp(i 2 pi (sx_f kx + sy_f ky')/n).  shifts: device double2 per frame;
+// frames with a NaN x shift are skipped.  If sum2 != nullptr, frames
+// f <= f2_last also accumulate into sum2.
+hipError_t launch_shift_sum(const float2* spectra, size_t spec_stride, int nframes, int n,
+                            const double2* shifts, float2* sum, float2* sum2, int f2_last,
+                            hipStream_t st);
+
+// Layout conversion between the resident [kx][ky] spectrum and the padded
+// row-major [ky][kx] layout (n rows of n/2+1 complex).
+hipError_t launch_spec_to_padded(const float2* spec, int n, float2* padded, hipStream_t st);
+hipError_t launch_padded_to_spec(const float2* padded, int n, float2* spec, hipStream_t st);
+
+// cufunc.h-compatible kernels on the padded layout with a MASK table
+// (x, y, z) per element (reference cufunc.cu:392-443, 227-237).
+struct MaskElem {
+  int x, y;
+  float z;
+};
+hipError_t launch_mask_shift_cc(float2* dfft, const float2* dsum, const MaskElem* mask,
+                                float shx, float shy, int n, hipStream_t st);
+hipError_t launch_mask_shift(float2* dfft, const MaskElem* mask, float shx, float shy, int n,
+                             hipStream_t st);
+hipError_t launch_log_modulus(float* dmod, const float2* dfft, int n, float scale,
+                              hipStream_t st);
+// First-element gather: out[f] = spectra[f*stride].x * scale (frame means).
+hipError_t launch_dc_gather(const float2* spectra, size_t stride, int nframes, double scale,
+                            double* out, hipStream_t st);
+
+// Elementwise helpers for the cufunc.h-compatible layer.
+hipError_t launch_add(float* dst, const float* src, size_t n, hipStream_t st);
+hipError_t launch_scale(float* dst, float s, size_t n, hipStream_t st);
+
+}  // namespace mc

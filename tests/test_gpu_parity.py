@@ -1,0 +1,180 @@
+"""GPU parity of the HIP path (libmc.so through the C ABI) against the FP64
+oracle on the same seeded inputs.
+
+Tolerances: FFT outputs within 2e-6 of the spectrum maximum (FP32 Stockham
+vs FP64); box maps within 1e-5 of their maximum; per-pair and per-frame
+shifts within 0.01 px except 1/32-grid flips (counted, at most a few per
+movie and each exactly 1/32 px); aligned sum within 1e-4 relative L2.
+"""
+import numpy as np
+import pytest
+
+import motioncorr1exp_amd as mc
+from motioncorr1exp_amd.synth import make_movie
+from _oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = mc.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def o64():
+    return Oracle("f64")
+
+
+def _grid_flips(a, b):
+    d = np.abs(np.asarray(a) - np.asarray(b))
+    return int(np.sum((d > 0.01) & (np.abs(d - 1 / 32) < 1e-6)))
+
+
+# ----------------------------------------------------------------- level 0
+@pytest.mark.parametrize("n", [64, 512, 2048])
+def test_level0_fft_matches_numpy(n):
+    rng = np.random.default_rng(n)
+    img = rng.standard_normal((n, n)).astype(np.float32)
+    buf = np.zeros((n, n + 2), np.float32)
+    buf[:, :n] = img
+    d = mc.DeviceArray(buf)
+    plan = mc.ctypes.c_void_p()
+    L = mc.lib()
+    mc.check(L.mc_fft_plan(mc.ctypes.byref(plan), n))
+    mc.check(L.mc_fft2d(d.ptr, plan))
+    mc.check(L.mc_sync())
+    spec = d.to_host(np.float32, (n, n + 2)).view(np.complex64)
+    ref = np.fft.rfft2(img.astype(np.float64))
+    assert np.abs(spec - ref).max() <= 2e-6 * np.abs(ref).max() * np.log2(n)
+    iplan = mc.ctypes.c_void_p()
+    mc.check(L.mc_ifft_plan(mc.ctypes.byref(iplan), n))
+    mc.check(L.mc_ifft2d(d.ptr, iplan))
+    mc.check(L.mc_sync())
+    back = d.to_host(np.float32, (n, n + 2))[:, :n] / (n * n)
+    assert np.abs(back - img).max() < 1e-5 * np.log2(n)
+    L.mc_fft_destroy(plan)
+    L.mc_fft_destroy(iplan)
+
+
+def test_level0_shift_cc_and_find_shift(o64):
+    # integer circular roll t gives b = -t... content phi(x - t) in frame j
+    n, box = 256, 64
+    rng = np.random.default_rng(7)
+    a = rng.standard_normal((n, n)).astype(np.float32)
+    b = np.roll(a, shift=(3, -5), axis=(0, 1))  # frame b = a shifted by (+x -5, +y 3)
+    L = mc.lib()
+    bufs = []
+    for im in (a, b):
+        buf = np.zeros((n, n + 2), np.float32)
+        buf[:, :n] = im
+        bufs.append(mc.DeviceArray(buf))
+    plan = mc.ctypes.c_void_p()
+    mc.check(L.mc_fft_plan(mc.ctypes.byref(plan), n))
+    iplan = mc.ctypes.c_void_p()
+    mc.check(L.mc_ifft_plan(mc.ctypes.byref(iplan), n))
+    for d in bufs:
+        mc.check(L.mc_fft2d(d.ptr, plan))
+    mask = (mc.MaskElem * (n * (n // 2 + 1)))()
+    mc.check(L.mc_make_pos_list(mask, n, 150.0))
+    dmask = mc.DeviceArray(np.frombuffer(mask, np.uint8))
+    # GPUShiftCC(dfft = b, dsum = a): a * conj(b)
+    mc.check(L.mc_shift_cc(bufs[1].ptr, bufs[0].ptr, dmask.ptr, 0.0, 0.0, n))
+    mc.check(L.mc_ifft2d(bufs[1].ptr, iplan))
+    hbox = np.zeros((box, box), np.float32)
+    sx, sy, cc = mc.ctypes.c_float(), mc.ctypes.c_float(), mc.ctypes.c_float()
+    mc.check(L.mc_find_shift(bufs[1].ptr, n, hbox.ctypes.data, box, mc.ctypes.byref(sx),
+                             mc.ctypes.byref(sy), mc.ctypes.byref(cc), -1))
+    # content of b is a moved by (+(-5), +3): t_a - t_b = (5, -3)
+    assert abs(sx.value - 5.0) < 1e-6 and abs(sy.value + 3.0) < 1e-6
+    # the same map through the oracle
+    ox, oy, occ = o64.find_shift_box(hbox.astype(np.float64), n)
+    assert (ox, oy) == (sx.value, sy.value)
+    assert abs(occ - cc.value) < 1e-6 * abs(occ)
+    L.mc_fft_destroy(plan)
+    L.mc_fft_destroy(iplan)
+
+
+# ----------------------------------------------------------------- level 1
+def _cfg1_movie():
+    return make_movie(512, 512, 10, dose=1.0, step=0.7, seed=1001)
+
+
+@pytest.fixture(scope="module")
+def cfg1(o64):
+    frames, drift = _cfg1_movie()
+    o = o64.align_movie(frames, dict(bin=1.0, fod=4, bfactor=150.0, box=96, full_c2r=1, iter=2,
+                                     nthreads=8), want_spectra=True)
+    return frames, drift, o
+
+
+def test_spectra_match_oracle(ctx, cfg1):
+    frames, drift, o = cfg1
+    d = mc.DeviceArray(frames)
+    params = mc.default_params(10, FrameDistOffset=4)
+    ctx.load_frames(d, frames.shape, frames.dtype, params)
+    spec = ctx.get_spectra(10, 512).view(np.complex64)
+    ref = o["spectra"].view(np.complex128)
+    err = np.abs(spec - ref).max() / np.abs(ref).max()
+    assert err < 5e-7, err
+
+
+def test_boxmaps_and_pairs_match_oracle(ctx, cfg1):
+    frames, drift, o = cfg1
+    d = mc.DeviceArray(frames)
+    params = mc.default_params(10, FrameDistOffset=4, iter=2)
+    res = ctx.align_movie(d, frames.shape, frames.dtype, params,
+                          want=("pairs", "pair_b", "pair_cc", "boxmaps", "shift1", "final_shift",
+                                "frame_shift", "sum_image", "iter_meas", "rmsd"))
+    assert np.array_equal(res["pairs"], o["pairs"])
+    bm, obm = res["boxmaps"], o["boxmaps"]
+    scale = np.abs(obm).reshape(len(obm), -1).max(1)[:, None, None]
+    assert (np.abs(bm - obm) / scale).max() < 1e-5
+    db = np.abs(res["pair_b"] - o["pair_b"])
+    assert np.all((db < 1e-9) | (np.abs(db - 1 / 32) < 1e-9))
+    assert _grid_flips(res["pair_b"], o["pair_b"]) <= 2
+    assert np.allclose(res["pair_cc"], o["pair_cc"], rtol=1e-4)
+    dfs = np.abs(res["final_shift"] - o["final_shift"])
+    assert np.all((dfs <= 0.0101) | (np.abs(dfs - 1 / 32) < 1e-9) | (np.abs(dfs - 1 / 16) < 1e-9))
+    assert np.sum(dfs > 0.0101) <= 2
+    rel = np.linalg.norm(res["sum_image"] - o["sum_image"]) / np.linalg.norm(o["sum_image"])
+    assert rel < 1e-4, rel
+
+
+def test_peaks_stage_isolated(ctx, cfg1):
+    """Same FP32 box maps into the GPU FP64 peak search and the oracle."""
+    frames, drift, o = cfg1
+    boxes = o["boxmaps"].astype(np.float32)
+    b, cc = ctx.find_peaks(boxes, 512)
+    for q in range(len(boxes)):
+        ox, oy, occ = o64_find(boxes[q])
+        assert (b[q, 0], b[q, 1]) == (ox, oy)
+        assert abs(cc[q] - occ) <= 1e-9 * abs(occ)
+
+
+def o64_find(box):
+    return Oracle("f64").find_shift_box(box.astype(np.float64), 512)
+
+
+def test_sum_stage_isolated(ctx, cfg1):
+    """Oracle spectra and oracle shifts into the GPU shift-and-sum."""
+    frames, drift, o = cfg1
+    ctx.set_spectra(o["spectra"].astype(np.float32))
+    img = ctx.sum_image(o["frame_shift"], 512)
+    rel = np.linalg.norm(img - o["sum_image"]) / np.linalg.norm(o["sum_image"])
+    assert rel < 2e-6, rel
+
+
+def test_drift_recovered(ctx):
+    """Known synthetic drift: final frame shifts equal -(t_k - t_ref)."""
+    frames, drift = make_movie(512, 512, 10, dose=4.0, step=1.5, seed=77)
+    d = mc.DeviceArray(frames)
+    params = mc.default_params(10, FrameDistOffset=4, bAlignToMid=0)
+    res = ctx.align_movie(d, frames.shape, frames.dtype, params, want=("frame_shift",))
+    fs = res["frame_shift"]
+    # applied shift registers frame k onto frame 0: s_k = t_k - t_0
+    expect = drift - drift[0]
+    expect[0] = 0
+    assert np.abs(fs[1:] - expect[1:]).max() < 0.15
