@@ -1,0 +1,264 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X drift-correction path on its headline workload.
+
+Workload (one step = one movie aligned end to end, frames resident in HBM):
+3838 x 3710 x 40 float32 counts, mean 0.8 e/px/frame, seeded Gaussian random
+field phantom, cumulative random-walk drift N(0, 0.7^2) px per axis, Poisson
+noise; padded to 4096 x 4096 (region mean fill), bin 1, all pairs with frame
+distance >= 18 (-fod 2, 253 pairs), B-factor 150, peak box 96, -itr 0.
+A second line item times the same movie with -itr 3.
+
+Prints one JSON line.  Errors are stated against the FP64 CPU oracle run on
+the same movie: per-frame shift RMS / max (px) and 1/32-grid flips, aligned
+sum relative L2 error end to end and stage-isolated (oracle shifts fed to
+the GPU sum).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "movies aligned/sec at 3838x3710x40; shift RMS err vs FP64 oracle (px)"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def compulsory_bytes(F, n, nx, ny, in_elem, P, box, itr):
+    """Compulsory HBM bytes of one movie: every tensor read once per consuming
+    stage, every output written once (stage 1: frames in, spectra out; pairs:
+    spectra in, box maps out; per align-to-sum round: spectra twice + sum;
+    final sum: spectra in, sum out, image out)."""
+    S = n * (n // 2 + 1) * 8
+    b = F * nx * ny * in_elem + F * S          # stage 1
+    b += F * S + P * box * box * 4             # pairs
+    b += itr * (2 * F * S + 2 * S + F * box * box * 4)  # align-to-sum rounds
+    b += F * S + S + S + 4 * n * n             # final sum + C2R
+    return b
+
+
+def cpu_baseline(frames_shape, pad, P, F, nthreads, budget_s=20.0):
+    """Times the FP32 port of the reference algorithm (oracle/liboracle_f32.so)
+    on a bounded sample of the workload and scales it to movies/s:
+    T = F*t_frame + P*t_pair + F*t_shift + t_c2r, each term timed on 2 samples
+    (t_frame: crop/pad + 4096^2 R2C; t_pair: cross-power + full 4096^2 C2R +
+    FindShift; t_shift: phase shift + accumulate; t_c2r: final inverse)."""
+    import ctypes
+
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _oracle import Oracle
+
+    o = Oracle("f32")
+    L = o.lib
+    try:
+        L.omp_set_num_threads  # noqa: B018
+    except AttributeError:
+        pass
+    os.environ["OMP_NUM_THREADS"] = str(nthreads)
+    n = pad
+    rng = np.random.default_rng(5)
+    nz, ny, nx = frames_shape
+    frame = rng.poisson(0.8, size=(ny, nx)).astype(np.float32)
+    buf = np.zeros((n, n + 2), np.float32)
+    buf2 = np.zeros((n, n + 2), np.float32)
+    L.or_pad2fft.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                             ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    box = np.zeros((96, 96), np.float32)
+    sx, sy = ctypes.c_double(), ctypes.c_double()
+    reps = 2
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        L.or_pad2fft(frame.ctypes.data, nx, ny, buf.ctypes.data, 0, 0, n)
+        L.or_r2c2d(buf.ctypes.data, n)
+    t_frame = (time.perf_counter() - t0) / reps
+    spec = buf.copy()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        buf2[:] = spec
+        L.or_cross_power(buf2.ctypes.data, spec.ctypes.data, n, ctypes.c_float(150.0))
+        L.or_c2r2d(buf2.ctypes.data, n)
+        L.or_find_shift(buf2.ctypes.data, n, box.ctypes.data, 96, ctypes.byref(sx),
+                        ctypes.byref(sy), -1)
+    t_pair = (time.perf_counter() - t0) / reps
+    acc = np.zeros_like(spec)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        buf2[:] = spec
+        L.or_phase_shift(buf2.ctypes.data, n, 0.37, -1.21)
+        acc += buf2
+    t_shift = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    L.or_c2r2d(acc.ctypes.data, n)
+    t_c2r = time.perf_counter() - t0
+    T = F * t_frame + P * t_pair + F * t_shift + t_c2r
+    return {
+        "value": 1.0 / T, "unit": "movies/s", "cores": nthreads, "kind": "port",
+        "sample": ("FP32 port of the reference path (oracle/liboracle_f32.so, own FFT, OpenMP): "
+                   "2x frame pad+R2C 4096^2 (%.3fs), 2x pair cross-power+full C2R+FindShift "
+                   "(%.3fs), 2x phase-shift+add (%.3fs), 1x C2R (%.3fs); movie = %d frames + "
+                   "%d pairs + %d shifts + 1 C2R = %.1fs"
+                   % (t_frame, t_pair, t_shift, t_c2r, F, P, F, T)),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-oracle", action="store_true", help="skip the FP64 error check")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-itr3", action="store_true")
+    ap.add_argument("--frames", type=int, default=40)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU per box; --gpus %d is not supported "
+              "(movies are independent: multi-GPU is movie-level data parallelism, "
+              "see DESIGN.md)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import motioncorr1exp_amd as mc
+    from motioncorr1exp_amd.synth import make_movie
+
+    nx, ny, F, pad = 3838, 3710, args.frames, 4096
+    t0 = time.time()
+    frames, drift = make_movie(nx, ny, F, dose=0.8, step=0.7, seed=1003)
+    log("generated %dx%dx%d movie in %.1fs" % (nx, ny, F, time.time() - t0))
+    params = mc.default_params(F, pad=pad, bin=1.0, FrameDistOffset=2, bfactor=150.0,
+                               CCPeakSearchDim=96, iter=0)
+    ctx = mc.Context(0)
+    dev = mc.DeviceArray(frames)
+    sum_dev = mc.DeviceArray(nbytes=pad * pad * 4)
+    L = mc.lib()
+
+    def step(prm, want=("frame_shift",)):
+        return ctx.align_movie(dev, frames.shape, frames.dtype, prm, want=want, sum_dev=sum_dev)
+
+    for _ in range(args.warmup):
+        step(params)
+    mc.check(L.mc_sync())
+    ctx.kernel_timing(True)
+    mc.check(L.mc_sync())
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = step(params)
+    mc.check(L.mc_sync())
+    dt = time.perf_counter() - t0
+    kern = {}
+    for g in mc.KT:
+        c, t = ctx.kernel_time(g)
+        kern[g] = {"launches": c, "ms_per_launch": (t / c) if c else 0.0,
+                   "ms_per_movie": t / args.steps}
+    ctx.kernel_timing(False)
+    ms = dt * 1e3 / args.steps
+    rep = res["report"]
+    P = rep["npairs"]
+    n = rep["n"]
+    log("itr0: %.3f ms/movie (%s)" % (ms, {k: round(v["ms_per_movie"], 3) for k, v in kern.items()}))
+
+    # roofline of the dominant kernel group
+    dom = max(kern, key=lambda k: kern[k]["ms_per_movie"])
+    S = n * (n // 2 + 1) * 8
+    unit_bytes = {
+        # frames in + spectra out per frame (padded 4096^2 canvas)
+        "spectra": F * (nx * ny * 4 + S),
+        # every spectrum read once + the pair maps of one launch
+        "pair_cols": F * S + P * (n // 2 + 1) * 96 * 8,
+        "pair_rows": P * (n // 2 + 1) * 96 * 8 + P * 96 * 96 * 4,
+        "peaks": P * 96 * 96 * 4,
+        "sum": F * S + S,
+        "image": S + 4 * n * n,
+    }
+    alg = unit_bytes[dom]
+    per_launch_ms = kern[dom]["ms_per_launch"]
+    # launches of the dominant group per movie
+    lpm = kern[dom]["launches"] / args.steps
+    achieved = alg / lpm / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else 0.0
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "algorithmic_bytes_per_launch": alg / lpm}
+    comp = compulsory_bytes(F, n, nx, ny, 4, P, 96, 0)
+    movies_s = args.steps / dt
+    whole = {"compulsory_bytes_per_movie": comp,
+             "achieved_GBs": comp * movies_s / 1e9,
+             "frac_of_8TBs": comp * movies_s / 1e9 / HBM_PEAK_GBS}
+
+    out = {
+        "metric": METRIC, "value": movies_s, "unit": "movies/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded random-field phantom, Poisson counts, known drift)",
+        "config": {"workload": "3838x3710x40 f32 counts dose 0.8, pad 4096, bin 1, fod 2 "
+                               "(253 pairs), bft 150, pbx 96, itr 0",
+                   "movies": 1, "frames": F, "nx": nx, "ny": ny, "pad": pad, "pairs": P,
+                   "form": "plain FP32"},
+        "roofline": roofline, "whole_movie_roofline": whole, "kernels": kern,
+        "stage_ms": {k: rep[k] for k in ("ms_spectra", "ms_pairs", "ms_solve", "ms_iter",
+                                         "ms_sum", "ms_total")},
+    }
+
+    if not args.no_itr3:
+        p3 = dict(params, iter=3)
+        step(p3)
+        mc.check(L.mc_sync())
+        t0 = time.perf_counter()
+        k3 = max(2, args.steps // 2)
+        for _ in range(k3):
+            r3 = step(p3)
+        mc.check(L.mc_sync())
+        dt3 = time.perf_counter() - t0
+        out["itr3"] = {"movies_per_s": k3 / dt3, "ms_per_movie": dt3 * 1e3 / k3,
+                       "niter": r3["report"]["niter"]}
+        log("itr3: %.3f ms/movie" % (dt3 * 1e3 / k3))
+
+    if not args.no_oracle:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from _oracle import Oracle
+
+        t0 = time.time()
+        full = ctx.align_movie(dev, frames.shape, frames.dtype, params,
+                               want=("frame_shift", "final_shift", "pair_b", "sum_image"))
+        o = Oracle("f64").align_movie(
+            frames, dict(pad=pad, bin=1.0, fod=2, bfactor=150.0, box=96, iter=0, full_c2r=0,
+                         nthreads=min(os.cpu_count() or 1, 16)), want_boxes=False)
+        log("oracle FP64 in %.1fs" % (time.time() - t0))
+        d = full["frame_shift"] - o["frame_shift"]
+        d = d[np.isfinite(d).all(1)]
+        flips = int(np.sum(np.abs(np.abs(full["pair_b"] - o["pair_b"]) - 1 / 32) < 1e-9))
+        rel = float(np.linalg.norm(full["sum_image"] - o["sum_image"]) /
+                    np.linalg.norm(o["sum_image"]))
+        iso = ctx.sum_image(o["frame_shift"], n)
+        rel_iso = float(np.linalg.norm(iso - o["sum_image"]) / np.linalg.norm(o["sum_image"]))
+        truth = drift - drift[F // 2 + 1]  # RefFrame = nz/2+1 (DFAlign.cu:1133)
+        out["accuracy"] = {
+            "shift_rms_err_px": float(np.sqrt(np.mean(d ** 2))),
+            "shift_max_err_px": float(np.abs(d).max()),
+            "pair_grid_flips": flips,
+            "sum_rel_err": rel, "sum_rel_err_stage_isolated": rel_iso,
+            "vs_truth_rms_px": float(np.sqrt(np.mean((full["frame_shift"] - truth) ** 2))),
+        }
+        out["shift_rms_err_px"] = out["accuracy"]["shift_rms_err_px"]
+        log("accuracy %s" % out["accuracy"])
+
+    if not args.no_cpu:
+        nthreads = min(os.cpu_count() or 1, 16)
+        out["cpu_baseline"] = cpu_baseline(frames.shape, pad, P, F, nthreads)
+        out["gpu_over_cpu"] = movies_s / out["cpu_baseline"]["value"]
+        log("cpu baseline %s" % out["cpu_baseline"])
+
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
