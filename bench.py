@@ -118,6 +118,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-itr3", action="store_true")
     ap.add_argument("--frames", type=int, default=40)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="only copy 1 GiB device-to-device (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU per box; --gpus %d is not supported "
@@ -129,6 +131,17 @@ def main():
 
     import motioncorr1exp_amd as mc
     from motioncorr1exp_amd.synth import make_movie
+
+    if args.calib_copy:
+        nb = 1 << 30
+        a = mc.DeviceArray(nbytes=nb)
+        b = mc.DeviceArray(nbytes=nb)
+        mc.check(mc.lib().mc_mem_zero(a.ptr, nb))
+        mc.check(mc.lib().mc_sync())
+        mc.check(mc.lib().mc_mem_d2d(b.ptr, a.ptr, nb))
+        mc.check(mc.lib().mc_sync())
+        print(json.dumps({"calib_copy_bytes": nb}))
+        return
 
     nx, ny, F, pad = 3838, 3710, args.frames, 4096
     t0 = time.time()

@@ -139,16 +139,45 @@ __host__ __device__ constexpr int stage_ns(int N, int s) {
   return s == 0 ? 1 : stage_ns(N, s - 1) * stage_radix(N, s - 1);
 }
 
+// Twiddle sources: e^{+2 pi i e/ntw} for e in [0, ntw).
+struct TwTable {  // full table (global or LDS)
+  const float2* t;
+  int ntw;
+  MC_DEV float2 operator()(int e) const { return t[e & (ntw - 1)]; }
+};
+// Two-level table in LDS: w(e) = lo[e & (L-1)] * hi[e >> log2 L].  With
+// correctly rounded entries the product is within ~1.5 ulp.
+struct TwSplit {
+  const float2* lo;
+  const float2* hi;
+  int lbits, ntw;
+  MC_DEV float2 operator()(int e) const {
+    e &= ntw - 1;
+    return cmul(lo[e & ((1 << lbits) - 1)], hi[e >> lbits]);
+  }
+};
+// Fills lo (2^lbits) and hi (ntw >> lbits) from a global full table.
+MC_DEV void tw_split_load(float2* lo, float2* hi, const float2* __restrict__ g, int ntw,
+                          int lbits, int tid, int nt) {
+  const int L = 1 << lbits;
+  for (int i = tid; i < L; i += nt) lo[i] = g[i];
+  for (int i = tid; i < (ntw >> lbits); i += nt) hi[i] = g[i << lbits];
+}
+template <int S, class TW> MC_DEV float2 twv(const TW& tw, int e) {
+  float2 w = tw(e);
+  return S > 0 ? w : make_float2(w.x, -w.y);
+}
+
 // One Stockham butterfly of stage (R, Ns) for transform length N at index j
 // (0 <= j < N/R): gather v[r] = x[j + r N/R], twiddle by W_{Ns R}^{(j%Ns) r},
-// DFT_R.  tw: table of e^{+2 pi i e/NTW}, NTW a multiple of N.
-template <int N, int R, int Ns, int S>
-MC_DEV void stockham_bfly(float2* v, int j, const float2* tw, int ntw) {
+// DFT_R.  tw: e^{+2 pi i e/ntw}, ntw a multiple of N.
+template <int N, int R, int Ns, int S, class TW>
+MC_DEV void stockham_bfly(float2* v, int j, const TW& tw, int ntw) {
   if constexpr (Ns > 1) {
     const int k = j % Ns;
     const int step = ntw / (Ns * R);
 #pragma unroll
-    for (int r = 1; r < R; r++) v[r] = cmul(v[r], twid<S>(tw, (k * r * step) & (ntw - 1)));
+    for (int r = 1; r < R; r++) v[r] = cmul(v[r], twv<S>(tw, k * r * step));
   }
   dft<R, S>(v);
 }
@@ -156,10 +185,17 @@ template <int R, int Ns> MC_DEV int stockham_out(int j) {
   return (j / Ns) * Ns * R + (j % Ns);
 }
 
-// Full in-LDS FFT of NB transforms of length N (buf[b*TS + i]), NT threads.
-// Data and result in natural order in LDS.  Caller synchronises before.
-template <int N, int NT, int S, int NB, int STAGE, int TS>
-MC_DEV void fft_lds_stage(float2* buf, const float2* tw, int ntw) {
+// LDS index padding: one float2 of padding after every 2^PB elements, so
+// that the stride-R writes of Stockham stage 0 spread over all banks.
+template <int PB> MC_DEV int lix(int i) { return PB ? i + (i >> PB) : i; }
+template <int PB> __host__ __device__ constexpr int lds_len(int n) {
+  return PB ? n + ((n - 1) >> PB) + 1 : n;
+}
+
+// Full in-LDS FFT of NB transforms of length N (buf[b*TS + lix(i)]), NT
+// threads.  Data and result in natural order.  Caller synchronises before.
+template <int N, int NT, int S, int NB, int STAGE, int TS, class TW, int PB = 0>
+MC_DEV void fft_lds_stage(float2* buf, const TW& tw, int ntw) {
   constexpr int R = stage_radix(N, STAGE);
   constexpr int Ns = stage_ns(N, STAGE);
   constexpr int NBF = NB * (N / R);
@@ -172,7 +208,7 @@ MC_DEV void fft_lds_stage(float2* buf, const float2* tw, int ntw) {
     if (NBF % NT == 0 || q < NBF) {
       const int b = q / (N / R), j = q % (N / R);
 #pragma unroll
-      for (int r = 0; r < R; r++) v[p][r] = buf[b * TS + j + r * (N / R)];
+      for (int r = 0; r < R; r++) v[p][r] = buf[b * TS + lix<PB>(j + r * (N / R))];
       stockham_bfly<N, R, Ns, S>(v[p], j, tw, ntw);
     }
   }
@@ -184,18 +220,22 @@ MC_DEV void fft_lds_stage(float2* buf, const float2* tw, int ntw) {
       const int b = q / (N / R), j = q % (N / R);
       const int o = stockham_out<R, Ns>(j);
 #pragma unroll
-      for (int r = 0; r < R; r++) buf[b * TS + o + r * Ns] = v[p][r];
+      for (int r = 0; r < R; r++) buf[b * TS + lix<PB>(o + r * Ns)] = v[p][r];
     }
   }
   __syncthreads();
 }
 
+template <int N, int NT, int S, int NB, int TS, class TW, int PB = 0>
+MC_DEV void fft_lds_t(float2* buf, const TW& tw, int ntw) {
+  fft_lds_stage<N, NT, S, NB, 0, TS, TW, PB>(buf, tw, ntw);
+  if constexpr (stage_count(N) > 1) fft_lds_stage<N, NT, S, NB, 1, TS, TW, PB>(buf, tw, ntw);
+  if constexpr (stage_count(N) > 2) fft_lds_stage<N, NT, S, NB, 2, TS, TW, PB>(buf, tw, ntw);
+  if constexpr (stage_count(N) > 3) fft_lds_stage<N, NT, S, NB, 3, TS, TW, PB>(buf, tw, ntw);
+}
 template <int N, int NT, int S, int NB, int TS = N>
 MC_DEV void fft_lds(float2* buf, const float2* tw, int ntw) {
-  fft_lds_stage<N, NT, S, NB, 0, TS>(buf, tw, ntw);
-  if constexpr (stage_count(N) > 1) fft_lds_stage<N, NT, S, NB, 1, TS>(buf, tw, ntw);
-  if constexpr (stage_count(N) > 2) fft_lds_stage<N, NT, S, NB, 2, TS>(buf, tw, ntw);
-  if constexpr (stage_count(N) > 3) fft_lds_stage<N, NT, S, NB, 3, TS>(buf, tw, ntw);
+  fft_lds_t<N, NT, S, NB, TS>(buf, TwTable{tw, ntw}, ntw);
 }
 
 // --------------------------------------------------------- XCD remapping

@@ -57,51 +57,88 @@ const float2* twiddles(int n) {
 }
 
 // ----------------------------------------------------------------- loads
-MC_DEV float load_elem(const char* row, int dtype, int x) {
-  switch (dtype) {
-    case DT_INT8: return (float)reinterpret_cast<const int8_t*>(row)[x];
-    case DT_UINT8: return (float)reinterpret_cast<const uint8_t*>(row)[x];
-    case DT_INT16: return (float)reinterpret_cast<const int16_t*>(row)[x];
-    case DT_UINT16: return (float)reinterpret_cast<const uint16_t*>(row)[x];
-    default: return reinterpret_cast<const float*>(row)[x];
-  }
-}
+template <int DT> struct Elem;
+template <> struct Elem<DT_INT8> { typedef int8_t T; };
+template <> struct Elem<DT_UINT8> { typedef uint8_t T; };
+template <> struct Elem<DT_INT16> { typedef int16_t T; };
+template <> struct Elem<DT_UINT16> { typedef uint16_t T; };
+template <> struct Elem<DT_FLOAT32> { typedef float T; };
+
+// two-level twiddle split for a table of size ntw
+__host__ __device__ constexpr int tw_lbits(int ntw) { return ilog2c(ntw) / 2; }
 
 // ------------------------------------------------------------- row pass
-template <int NU, int ROWS, int NT>
+template <int NU, int ROWS, int NT, int DT>
 __global__ __launch_bounds__(NT) void k_r2c_rows(R2CGeom g, int f0, int nkx, float2* rt,
                                                  size_t rt_fstride, float* rowsum,
-                                                 const float2* __restrict__ tw) {
+                                                 const float2* __restrict__ gtw) {
+  typedef typename Elem<DT>::T T;
   constexpr int N = NU / 2;
-  __shared__ float2 buf[ROWS * N];
-  float* rb = reinterpret_cast<float*>(buf);
+  constexpr int PB = 4;
+  constexpr int TS = lds_len<PB>(N + 1) | 1;  // odd row stride (float2): rows in distinct banks
+  constexpr int LB = tw_lbits(NU);
+  __shared__ float2 buf[ROWS * TS];
+  __shared__ float2 tlo[1 << LB], thi[NU >> LB];
   const int f = blockIdx.y;
   const int y0 = blockIdx.x * ROWS;
   const char* fbase = g.src + (size_t)(f0 + f) * g.frame_pitch;
-  for (int i = threadIdx.x; i < ROWS * NU; i += NT) {
-    const int r = i / NU, x = i % NU;
+  tw_split_load(tlo, thi, gtw, NU, LB, threadIdx.x, NT);
+  // packed row z[m] = (x[2m], x[2m+1])
+#pragma unroll
+  for (int r = 0; r < ROWS; r++) {
     const int y = y0 + r;
-    float v = 0.f;
-    if (y < g.h && x < g.w)
-      v = load_elem(fbase + (size_t)(y + g.offy) * g.row_pitch, g.dtype, x + g.offx);
-    rb[i] = v;
+    const T* row = reinterpret_cast<const T*>(fbase + (size_t)(y + g.offy) * g.row_pitch) + g.offx;
+    const bool yok = y < g.h;
+#pragma unroll 4
+    for (int m = threadIdx.x; m < N; m += NT) {
+      const int x = 2 * m;
+      float2 z;
+      z.x = (yok && x < g.w) ? (float)row[x] : 0.f;
+      z.y = (yok && x + 1 < g.w) ? (float)row[x + 1] : 0.f;
+      buf[r * TS + lix<PB>(m)] = z;
+    }
   }
   __syncthreads();
-  fft_lds<N, NT, -1, ROWS>(buf, tw, NU);
-  // unpack: X[k] = E[k] + e^{-2 pi i k/NU} O[k],
-  // E = (Z[k] + conj Z[N-k]) / 2, O = (Z[k] - conj Z[N-k]) / 2i
+  const TwSplit tw{tlo, thi, LB, NU};
+  fft_lds_t<N, NT, -1, ROWS, TS, TwSplit, PB>(buf, tw, NU);
+  // unpack in place, one thread per bin pair (k, N-k), k in [0, N/2]:
+  // X[k] = E[k] + e^{-2 pi i k/NU} O[k], E = (Z[k] + conj Z[N-k]) / 2,
+  // O = (Z[k] - conj Z[N-k]) / 2i; X[N] from k = 0.
+  constexpr int NP = N / 2 + 1;
+  for (int i = threadIdx.x; i < ROWS * NP; i += NT) {
+    const int r = i / NP, k = i % NP;
+    float2* Z = buf + r * TS;
+    const float2 zk = Z[lix<PB>(k)], zm = Z[lix<PB>((N - k) & (N - 1))];
+    // bin k
+    {
+      const float2 zc = cconj(zm);
+      const float2 E = cscale(cadd(zk, zc), 0.5f);
+      const float2 D = csub(zk, zc);
+      const float2 O = make_float2(0.5f * D.y, -0.5f * D.x);
+      const float2 X = cadd(E, cmul(twv<-1>(tw, k), O));
+      if (k == 0) {
+        // X[N] = E[0] - O[0]
+        Z[lix<PB>(N)] = csub(E, O);
+      }
+      Z[lix<PB>(k)] = X;
+    }
+    // bin N-k (k in (0, N/2))
+    if (k > 0 && k < N - k) {
+      const float2 zc = cconj(zk);
+      const float2 E = cscale(cadd(zm, zc), 0.5f);
+      const float2 D = csub(zm, zc);
+      const float2 O = make_float2(0.5f * D.y, -0.5f * D.x);
+      Z[lix<PB>(N - k)] = cadd(E, cmul(twv<-1>(tw, N - k), O));
+    }
+  }
+  __syncthreads();
+  // transposed store RT[kx][y]: ROWS consecutive floats2 per kx
   float2* out = rt + (size_t)f * rt_fstride;
   for (int i = threadIdx.x; i < ROWS * nkx; i += NT) {
     const int r = i % ROWS, k = i / ROWS;
     const int y = y0 + r;
     if (y >= g.h) continue;
-    const float2* Z = buf + r * N;
-    const float2 zk = Z[k & (N - 1)];
-    const float2 zc = cconj(Z[(N - k) & (N - 1)]);
-    const float2 E = cscale(cadd(zk, zc), 0.5f);
-    const float2 D = csub(zk, zc);
-    const float2 O = make_float2(0.5f * D.y, -0.5f * D.x);
-    const float2 X = cadd(E, cmul(twid<-1>(tw, k), O));
+    const float2 X = buf[r * TS + lix<PB>(k)];
     out[(size_t)k * NU + y] = X;
     if (k == 0 && rowsum) rowsum[(size_t)f * NU + y] = X.x;
   }
@@ -129,8 +166,12 @@ __global__ __launch_bounds__(NT) void k_c2c_cols(const float2* rt, size_t rt_fst
                                                  int h_data, const float* fill,
                                                  const float2* __restrict__ padP, int n,
                                                  float2* spec, size_t spec_stride,
-                                                 const float2* __restrict__ tw) {
-  __shared__ float2 buf[NU];
+                                                 const float2* __restrict__ gtw) {
+  constexpr int LB = tw_lbits(NU);
+  constexpr int PB = 4;
+  __shared__ float2 buf[lds_len<PB>(NU)];
+  __shared__ float2 tlo[1 << LB], thi[NU >> LB];
+  tw_split_load(tlo, thi, gtw, NU, LB, threadIdx.x, NT);
   const int L = xcd_remap(blockIdx.x, gridDim.x);
   const int kx = L % nkx, f = L / nkx;
   const float2* src = rt + (size_t)f * rt_fstride + (size_t)kx * NU;
@@ -148,14 +189,14 @@ __global__ __launch_bounds__(NT) void k_c2c_cols(const float2* rt, size_t rt_fst
     } else if (fill && kx == 0) {
       v.x = fv * (float)NU;
     }
-    buf[i] = v;
+    buf[lix<PB>(i)] = v;
   }
   __syncthreads();
-  fft_lds<NU, NT, -1, 1>(buf, tw, NU);
+  fft_lds_t<NU, NT, -1, 1, NU, TwSplit, PB>(buf, TwSplit{tlo, thi, LB, NU}, NU);
   float2* dst = spec + (size_t)f * spec_stride + (size_t)kx * n;
   for (int i = threadIdx.x; i < n; i += NT) {
     const int si = i < n / 2 ? i : NU - n + i;
-    dst[i] = buf[si];
+    dst[i] = buf[lix<PB>(si)];
   }
 }
 
@@ -172,8 +213,15 @@ static hipError_t frames_to_spectra_t(const R2CGeom& g, int f0, int nframes, flo
   const int nkx = g.n / 2 + 1;
   const size_t rt_fstride = (size_t)nkx * NU;
   dim3 gr((g.h + ROWS - 1) / ROWS, nframes);
-  hipLaunchKernelGGL((k_r2c_rows<NU, ROWS, NTR>), gr, dim3(NTR), 0, st, g, f0, nkx, rt,
-                     rt_fstride, g.pad_mode ? rowsum : nullptr, tw);
+  float* rs = g.pad_mode ? rowsum : nullptr;
+  switch (g.dtype) {
+    case DT_INT8: hipLaunchKernelGGL((k_r2c_rows<NU, ROWS, NTR, DT_INT8>), gr, dim3(NTR), 0, st, g, f0, nkx, rt, rt_fstride, rs, tw); break;
+    case DT_UINT8: hipLaunchKernelGGL((k_r2c_rows<NU, ROWS, NTR, DT_UINT8>), gr, dim3(NTR), 0, st, g, f0, nkx, rt, rt_fstride, rs, tw); break;
+    case DT_INT16: hipLaunchKernelGGL((k_r2c_rows<NU, ROWS, NTR, DT_INT16>), gr, dim3(NTR), 0, st, g, f0, nkx, rt, rt_fstride, rs, tw); break;
+    case DT_UINT16: hipLaunchKernelGGL((k_r2c_rows<NU, ROWS, NTR, DT_UINT16>), gr, dim3(NTR), 0, st, g, f0, nkx, rt, rt_fstride, rs, tw); break;
+    case DT_FLOAT32: hipLaunchKernelGGL((k_r2c_rows<NU, ROWS, NTR, DT_FLOAT32>), gr, dim3(NTR), 0, st, g, f0, nkx, rt, rt_fstride, rs, tw); break;
+    default: return hipErrorInvalidValue;
+  }
   if (g.pad_mode) {
     hipLaunchKernelGGL(k_fill, dim3(nframes), dim3(256), 0, st, rowsum, NU, g.h, g.w, fill);
   }

@@ -19,48 +19,100 @@
 //    transform with the DC/Nyquist projection), register-tiled.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
 #include "mc_device.h"
 #include "mc_kernels.h"
 
 namespace mc {
 
-template <int N, int M, int NT>
+// Persistent: the grid is a multiple of 8 workgroups; workgroup b runs on
+// XCD b % 8 and walks a contiguous range of work items (kx major, pair
+// minor) owned by that XCD, so the frame columns of one kx are fetched from
+// HBM once into that XCD's L2 and reused by all pairs.  Per-workgroup
+// constants (M-point twiddles in LDS; ky weights, combine twiddles in
+// registers) are loaded once.
+template <int N, int M, int NT, int SP>
 __global__ __launch_bounds__(NT) void k_pair_cols(const float2* __restrict__ spectra,
                                                   size_t spec_stride,
                                                   const int2* __restrict__ pairs, int npairs,
-                                                  float mb, float inv_sizec, int box,
+                                                  const float* __restrict__ wxy, int box,
                                                   float2* __restrict__ tbuf,
                                                   const float2* __restrict__ tw) {
   constexpr int A = N / M;
   constexpr int H = N / 2 + 1;
-  constexpr int TS = M + 1;
+  constexpr int PB = 4;
+  constexpr int TS = lds_len<PB>(M) | 1;
+  constexpr int EPT = (N + NT - 1) / NT;  // elements per thread
+  constexpr int AS = A / SP;              // combine terms per thread
   __shared__ float2 buf[A * TS];
-  const int L = xcd_remap(blockIdx.x, gridDim.x);
-  const int p = L % npairs, kx = L / npairs;
-  const int2 pr = pairs[p];
-  const float2* a = spectra + (size_t)pr.x * spec_stride + (size_t)kx * N;
-  const float2* b = spectra + (size_t)pr.y * spec_stride + (size_t)kx * N;
-  const int kx2 = kx * kx;
-  for (int e = threadIdx.x; e < N; e += NT) {
-    const float2 va = a[e], vb = b[e];
-    const int yy = e < N / 2 ? e : e - N;
-    float z = expf(mb * (float)(kx2 + yy * yy));
-    if ((kx + yy) & 1) z = -z;
-    float2 x = cmulc(va, vb);
-    x = cscale(cscale(x, inv_sizec), z);
-    buf[(e % A) * TS + e / A] = x;
+  __shared__ float2 twm[M];
+  __shared__ float2 part[SP][256];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < M; i += NT) twm[i] = tw[i * (N / M)];
+  // ky weights (-1)^{y'} exp(m y'^2) / sizec for this thread's elements
+  float wy[EPT];
+#pragma unroll
+  for (int i = 0; i < EPT; i++) {
+    const int e = tid + i * NT;
+    wy[i] = e < N ? wxy[H + e] : 0.f;
   }
-  __syncthreads();
-  fft_lds<M, NT, 1, A, TS>(buf, tw, N);
+  // combine: thread -> (output r, term block h)
   const int ori = (N - box) / 2;
-  float2* out = tbuf + ((size_t)p * H + kx) * box;
-  for (int r = threadIdx.x; r < box; r += NT) {
-    const int R = ori + r;
-    const int Rm = R & (M - 1);
-    float2 acc = buf[Rm];
-#pragma unroll 4
-    for (int q = 1; q < A; q++) acc = cadd(acc, cmul(tw[(q * R) & (N - 1)], buf[q * TS + Rm]));
-    out[r] = acc;
+  const int cr = tid % box, ch = tid / box;
+  const bool cact = tid < box * SP;
+  const int R = ori + cr;
+  float2 ctw[AS];
+#pragma unroll
+  for (int i = 0; i < AS; i++) {
+    const int a = ch * AS + i;
+    ctw[i] = cact ? tw[(a * R) & (N - 1)] : make_float2(0.f, 0.f);
+  }
+  const int Rm = R & (M - 1);
+  const int xcd = blockIdx.x % 8, lid = blockIdx.x / 8, nper = gridDim.x / 8;
+  const long long items = (long long)npairs * H;
+  const long long i0 = items * xcd / 8, i1 = items * (xcd + 1) / 8;
+  __syncthreads();
+  for (long long it = i0 + lid; it < i1; it += nper) {
+    const int kx = (int)(it / npairs), p = (int)(it % npairs);
+    const int2 pr = pairs[p];
+    const float2* a = spectra + (size_t)pr.x * spec_stride + (size_t)kx * N;
+    const float2* b = spectra + (size_t)pr.y * spec_stride + (size_t)kx * N;
+    const float wx = wxy[kx];
+    float2 va[EPT], vb[EPT];
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+      const int e = tid + i * NT;
+      if (e < N) {
+        va[i] = a[e];
+        vb[i] = b[e];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+      const int e = tid + i * NT;
+      if (e < N) buf[(e % A) * TS + lix<PB>(e / A)] = cscale(cmulc(va[i], vb[i]), wx * wy[i]);
+    }
+    __syncthreads();
+    fft_lds_t<M, NT, 1, A, TS, TwTable, PB>(buf, TwTable{twm, M}, M);
+    if (cact) {
+      float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int i = 0; i < AS; i++) acc = cadd(acc, cmul(ctw[i], buf[(ch * AS + i) * TS + lix<PB>(Rm)]));
+      part[ch][cr] = acc;
+    }
+    __syncthreads();
+    if (tid < box) {
+      float2 acc = part[0][tid];
+#pragma unroll
+      for (int h = 1; h < SP; h++) acc = cadd(acc, part[h][tid]);
+      tbuf[((size_t)p * H + kx) * box + tid] = acc;
+    }
+    // buf and part are rewritten only after the next item's first barrier
   }
 }
 
@@ -121,35 +173,83 @@ __global__ __launch_bounds__(256) void k_pair_rows(const float2* __restrict__ tb
     }
 }
 
-template <int N, int M>
+// Weight table: wxy[kx] = (-1)^kx exp(m kx^2), kx in [0, n/2]; wxy[n/2+1+ky]
+// = (-1)^{y'} exp(m y'^2) / sizec, so z/sizec = wxy[kx] * wxy[n/2+1+ky] with
+// m = -0.5 B/n^2 as in MkPosList (cufunc.cu:358-390).
+static const float* weights(int n, float bfactor) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, float>, float*> cache;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(mu);
+  auto key = std::make_tuple(dev, n, bfactor);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  const int H = n / 2 + 1;
+  const double m = -0.5 * (double)bfactor / n / n;
+  const double sizec = (double)H * n;
+  std::vector<float> h(H + n);
+  for (int kx = 0; kx < H; kx++) h[kx] = (float)((kx & 1 ? -1.0 : 1.0) * std::exp(m * kx * kx));
+  for (int ky = 0; ky < n; ky++) {
+    const int yy = ky < n / 2 ? ky : ky - n;
+    h[H + ky] = (float)(((yy & 1) ? -1.0 : 1.0) * std::exp(m * (double)yy * yy) / sizec);
+  }
+  float* d = nullptr;
+  if (hipMalloc(&d, sizeof(float) * h.size()) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    return nullptr;
+  }
+  cache[key] = d;
+  return d;
+}
+
+static int pair_grid() {
+  static int g = 0;
+  if (!g) {
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    g = ((cus * 4 + 7) / 8) * 8;
+  }
+  return g;
+}
+
+template <int N, int M, int SP>
 static hipError_t pair_cols_t(const float2* spectra, size_t spec_stride, const int2* pairs,
-                              int npairs, float mb, float inv_sizec, int box, float2* tbuf,
+                              int npairs, const float* wxy, int box, float2* tbuf,
                               hipStream_t st) {
-  constexpr int NT = (N / 16) < 64 ? 64 : ((N / 16) > 256 ? 256 : N / 16);
+  constexpr int NT = 256;
   const float2* tw = twiddles(N);
   if (!tw) return hipErrorOutOfMemory;
-  const long long nb = (long long)npairs * (N / 2 + 1);
-  if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((k_pair_cols<N, M, NT>), dim3((unsigned)nb), dim3(NT), 0, st, spectra,
-                     spec_stride, pairs, npairs, mb, inv_sizec, box, tbuf, tw);
+  hipLaunchKernelGGL((k_pair_cols<N, M, NT, SP>), dim3(pair_grid()), dim3(NT), 0, st, spectra,
+                     spec_stride, pairs, npairs, wxy, box, tbuf, tw);
   return hipGetLastError();
 }
 
 template <int N>
 static hipError_t pair_cols_n(const float2* spectra, size_t spec_stride, const int2* pairs,
-                              int npairs, float mb, float inv_sizec, int box, float2* tbuf,
+                              int npairs, const float* wxy, int box, float2* tbuf,
                               hipStream_t st) {
+  if constexpr (N / 128 >= 4) {
+    if (box <= 64)
+      return pair_cols_t<N, 128, 4>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
+  }
+  if constexpr (N / 128 >= 2) {
+    if (box <= 128)
+      return pair_cols_t<N, 128, 2>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
+  }
   if constexpr (N >= 128) {
     if (box <= 128)
-      return pair_cols_t<N, 128>(spectra, spec_stride, pairs, npairs, mb, inv_sizec, box, tbuf, st);
+      return pair_cols_t<N, 128, 1>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
   }
   if constexpr (N >= 256) {
     if (box <= 256)
-      return pair_cols_t<N, 256>(spectra, spec_stride, pairs, npairs, mb, inv_sizec, box, tbuf, st);
+      return pair_cols_t<N, 256, 1>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
   }
   if constexpr (N == 64) {
     if (box <= 64)
-      return pair_cols_t<64, 64>(spectra, spec_stride, pairs, npairs, mb, inv_sizec, box, tbuf, st);
+      return pair_cols_t<64, 64, 1>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
   }
   return hipErrorInvalidValue;
 }
@@ -159,18 +259,17 @@ hipError_t launch_pair_cols(const float2* spectra, size_t spec_stride, const int
                             hipStream_t st) {
   if (npairs <= 0) return hipSuccess;
   if (box < 1 || box > n || box > 256) return hipErrorInvalidValue;
-  // reference MkPosList: float m = -0.5*bfactor/nsam/nsam (cufunc.cu:365)
-  const float mb = (float)(-0.5 * bfactor / n / n);
-  const float inv_sizec = 1.0f / (float)((n / 2 + 1) * n);
+  const float* wxy = weights(n, bfactor);
+  if (!wxy) return hipErrorOutOfMemory;
   switch (n) {
-    case 64: return pair_cols_n<64>(spectra, spec_stride, pairs, npairs, mb, inv_sizec, box, tbuf, st);
-    case 128: return pair_cols_n<128>(spectra, spec_stride, pairs, npairs, mb, inv_sizec, box, tbuf, st);
-    case 256: return pair_cols_n<256>(spectra, spec_stride, pairs, npairs, mb, inv_sizec, box, tbuf, st);
-    case 512: return pair_cols_n<512>(spectra, spec_stride, pairs, npairs, mb, inv_sizec, box, tbuf, st);
-    case 1024: return pair_cols_n<1024>(spectra, spec_stride, pairs, npairs, mb, inv_sizec, box, tbuf, st);
-    case 2048: return pair_cols_n<2048>(spectra, spec_stride, pairs, npairs, mb, inv_sizec, box, tbuf, st);
-    case 4096: return pair_cols_n<4096>(spectra, spec_stride, pairs, npairs, mb, inv_sizec, box, tbuf, st);
-    case 8192: return pair_cols_n<8192>(spectra, spec_stride, pairs, npairs, mb, inv_sizec, box, tbuf, st);
+    case 64: return pair_cols_n<64>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
+    case 128: return pair_cols_n<128>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
+    case 256: return pair_cols_n<256>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
+    case 512: return pair_cols_n<512>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
+    case 1024: return pair_cols_n<1024>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
+    case 2048: return pair_cols_n<2048>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
+    case 4096: return pair_cols_n<4096>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
+    case 8192: return pair_cols_n<8192>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
     default: return hipErrorInvalidValue;
   }
 }

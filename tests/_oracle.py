@@ -74,6 +74,8 @@ class Oracle:
         L.or_find_shift_box.restype = ctypes.c_double
         L.or_pruned_box.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
         L.or_bin_crop.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
+        L.or_pad2fft.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_int]
         L.or_pair_list.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_int]
         L.or_lsq.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
         L.or_residual.argtypes = [vp, ctypes.c_int, vp, vp, vp]

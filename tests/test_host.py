@@ -1,0 +1,86 @@
+"""CPU tests of libmc.so without a GPU: every symbol declared in
+include/mc_abi.h is exported, the host FP64 drift model matches the oracle
+and numpy, and device entry points fail with an error code (not a crash)
+when no GPU is present."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import motioncorr1exp_amd as mc
+from _oracle import Oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    hdr = open(os.path.join(ROOT, "include", "mc_abi.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    return sorted(set(re.findall(r"\b(mc_[a-z0-9_]+)\s*\(", hdr)))
+
+
+def test_header_and_binding_agree():
+    assert sorted(mc.ABI_SYMBOLS) == _declared()
+
+
+def test_library_exports_every_declared_symbol():
+    L = mc.lib()
+    missing = [s for s in _declared() if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_version_string():
+    assert b"gfx950" in mc.lib().mc_version()
+
+
+@pytest.mark.parametrize("F,fod", [(10, 4), (24, 11), (40, 2), (40, 19), (7, 0), (2, 0)])
+def test_pair_list_matches_oracle(F, fod):
+    assert np.array_equal(mc.pair_list(F, fod), Oracle("f64").pair_list(F, fod))
+
+
+def test_lsq_normal_equations_match_householder_and_numpy():
+    o = Oracle("f64")
+    rng = np.random.default_rng(1)
+    for F, fod in [(10, 4), (24, 11), (40, 2)]:
+        pairs = mc.pair_list(F, fod)
+        b = rng.standard_normal((len(pairs), 2)) * 3
+        s = mc.lsq(pairs, F, b)
+        assert np.abs(s - o.lsq(pairs, F, b)).max() < 1e-10
+        A = np.zeros((len(pairs), F - 1))
+        for q, (i, j) in enumerate(pairs):
+            A[q, i:j] = 1
+        assert np.abs(s - np.linalg.lstsq(A, b, rcond=None)[0]).max() < 1e-10
+
+
+def test_threshold_matches_oracle():
+    o = Oracle("f64")
+    rng = np.random.default_rng(2)
+    for F, fod, kit in [(10, 4, 1.0), (24, 11, 0.3), (40, 2, 0.5), (6, 0, 0.05)]:
+        pairs = mc.pair_list(F, fod)
+        ki = np.abs(rng.standard_normal(len(pairs))) * 0.6
+        ki[rng.integers(0, len(pairs), 3)] = ki[0]  # ties resolve by row index
+        assert np.array_equal(mc.threshold(pairs, F, ki, kit), o.threshold(pairs, F, ki, kit))
+
+
+def test_singular_system_reports_error():
+    pairs = np.array([[0, 1], [0, 1]], np.int32)  # frame 2 never constrained
+    with pytest.raises(mc.MCError):
+        mc.lsq(pairs, 3, np.zeros((2, 2)))
+
+
+def test_device_calls_fail_cleanly_without_gpu():
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    L = mc.lib()
+    h = mc.ctypes.c_void_p()
+    assert L.mc_context_create(0, mc.ctypes.byref(h)) != mc.MC_OK
+    assert L.mc_init_gpu(0) != mc.MC_OK
+    assert L.mc_device_count() == 0
+
+
+def test_default_params_follow_verify_para():
+    p = mc.default_params(40, nEndSum2=50)
+    assert p["nEnd"] == 39 and p["nEndSum"] == 39 and p["nEndSum2"] == 39
+    p = mc.default_params(40, nStartSum=5, nEndSum2=3)
+    assert p["nEndSum2"] == 0 and p["nStartSum"] == 5

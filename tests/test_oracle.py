@@ -1,0 +1,398 @@
+"""CPU tests that pin the oracle (the checker) against numpy/pocketfft golden
+vectors, closed forms, brute force on tiny inputs and invariants -- each
+chosen so that a dropped term, a wrong sign or index, or a transposed
+operand fails one of them.  No GPU needed.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from _oracle import Oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def o():
+    return Oracle("f64")
+
+
+@pytest.fixture(scope="module")
+def o32():
+    return Oracle("f32")
+
+
+@pytest.fixture(scope="module")
+def golden():
+    return np.load(os.path.join(HERE, "golden", "fft_golden.npz"))
+
+
+# ------------------------------------------------------------------- FFT
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 7, 12, 16, 26, 53, 100, 103, 512, 3710])
+def test_fft_c2c_matches_numpy(o, n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    f = o.fft_c2c(x, -1)
+    assert np.abs(f - np.fft.fft(x)).max() <= 1e-13 * max(1, np.abs(f).max())
+    b = o.fft_c2c(x, +1)
+    assert np.abs(b - np.fft.ifft(x) * n).max() <= 1e-13 * max(1, np.abs(b).max())
+
+
+@pytest.mark.parametrize("n", [16, 26, 32, 64])
+def test_r2c2d_golden(o, golden, n):
+    s = o.r2c2d(golden["img_%d" % n])
+    ref = golden["rfft2_%d" % n]
+    assert np.abs(s - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_reference_ramp_roundtrip(o, o32, golden):
+    """The 26 x 26 ramp of the reference's FFT self-check: R2C matches
+    pocketfft and C2R/n^2 gives the ramp back (FP64 and FP32 builds)."""
+    ramp = golden["ramp_26"]
+    s = o.r2c2d(ramp)
+    assert np.abs(s - golden["rfft2_ramp_26"]).max() < 1e-11
+    assert np.abs(o.c2r2d(s) / 26 ** 2 - ramp).max() < 1e-12
+    s32 = o32.r2c2d(ramp.astype(np.float32))
+    assert np.abs(o32.c2r2d(s32) / 26 ** 2 - ramp).max() < 1e-4
+
+
+def test_fft_closed_forms(o):
+    n = 32
+    d = np.zeros((n, n))
+    d[0, 0] = 1.0
+    assert np.allclose(o.r2c2d(d), 1.0)                         # delta -> constant
+    c = np.full((n, n), 2.5)
+    s = o.r2c2d(c)
+    assert abs(s[0, 0] - 2.5 * n * n) < 1e-9                    # constant -> DC delta
+    assert np.abs(s.ravel()[1:]).max() < 1e-9
+    y, x = np.mgrid[0:n, 0:n]
+    e = np.cos(2 * np.pi * (3 * x + 5 * y) / n)                 # one cosine -> two bins
+    s = o.r2c2d(e)
+    assert abs(s[5, 3] - n * n / 2) < 1e-9 and abs(s[n - 5, 3]) < 1e-9
+    mask = np.ones_like(s, bool)
+    mask[5, 3] = False
+    assert np.abs(s[mask]).max() < 1e-9
+
+
+def test_parseval(o):
+    n = 64
+    img = np.random.default_rng(3).standard_normal((n, n))
+    s = o.r2c2d(img)
+    w = np.full(s.shape[1], 2.0)
+    w[0] = w[-1] = 1.0
+    assert abs((np.abs(s) ** 2 * w).sum() / n ** 2 - (img ** 2).sum()) < 1e-8 * (img ** 2).sum()
+
+
+def test_c2r_semantics_match_numpy_irfft2(o):
+    """C2R of arbitrary (non-Hermitian) half spectra equals numpy's irfft2 * n^2:
+    complex column transforms, then the row C2R ignoring the imaginary parts
+    of the DC and Nyquist bins."""
+    n = 32
+    rng = np.random.default_rng(11)
+    X = rng.standard_normal((n, n // 2 + 1)) + 1j * rng.standard_normal((n, n // 2 + 1))
+    assert np.abs(o.c2r2d(X) - np.fft.irfft2(X, s=(n, n)) * n * n).max() < 1e-10
+
+
+# -------------------------------------------------------------- geometry
+def test_verify_crop_size(o):
+    v = o.lib.or_verify_crop_size
+    assert v(3838, 3710, 0, 0, 0, 1.0) == 3710
+    assert v(3838, 3710, 0, 0, 0, 2.0) == 3708          # multiple of 4 when binning 2x
+    assert v(3838, 3710, 0, 0, 0, 1.5) == 3710
+    assert v(4096, 4096, 0, 0, 1000, 1.0) == 1000
+    assert v(4096, 4096, 100, 10, 0, 1.0) == 3996
+    assert v(4096, 4096, 4096, 0, 0, 1.0) == 0           # offset out of range
+    assert v(4096, 4096, 0, 0, 0, 0.0) == 0              # bin <= 0
+    assert v(513, 600, 0, 0, 0, 1.0) == 512              # rounded to even
+
+
+def test_binned_size(o):
+    assert o.lib.or_binned_size(4096, 2.0) == 2048
+    assert o.lib.or_binned_size(3710, 1.5) == 2472
+    assert o.lib.or_binned_size(512, 1.0) == 512
+
+
+def test_bin_crop_is_slicing_and_scales_means(o):
+    nu, n = 64, 32
+    img = np.random.default_rng(4).standard_normal((nu, nu)) + 3.0
+    S = o.r2c2d(img)
+    B = np.zeros((n, n + 2))
+    src = o.padded(S)
+    o.lib.or_bin_crop(B.ctypes.data, src.ctypes.data, n, nu)
+    Bc = B.view(np.complex128)
+    ref = np.concatenate([S[: n // 2, : n // 2 + 1], S[nu - n // 2:, : n // 2 + 1]])
+    assert np.array_equal(Bc, ref)
+    # Fourier crop keeps the DC sum: the binned image mean is bin^2 x the mean
+    small = o.c2r2d(Bc) / (n * n)
+    assert abs(small.mean() - img.mean() * (nu / n) ** 2) < 1e-9
+
+
+def test_pad_fill_is_region_mean(o):
+    rng = np.random.default_rng(9)
+    frame = rng.random((40, 50)).astype(np.float32)
+    out = np.zeros((64, 66))
+    o.lib.or_pad2fft(frame.ctypes.data, 50, 40, out.ctypes.data, 5, 3, 64)
+    region = frame[3:, 5:].astype(np.float64)
+    assert np.array_equal(out[:37, :45], region)
+    assert np.allclose(out[37:, :64], region.mean()) and np.allclose(out[:, 45:64], region.mean())
+
+
+# ------------------------------------------------------ spectral kernels
+def test_weight_closed_form(o):
+    n, B = 64, 150.0
+    for kx, ky in [(0, 0), (1, 0), (0, 1), (3, 5), (32, 40), (10, 63)]:
+        yy = ky if ky < n // 2 else ky - n
+        z = (-1) ** ((kx + yy) % 2) * np.exp(-B * (kx * kx + yy * yy) / (2 * n * n))
+        assert abs(o.weight(n, B, kx, ky) - z) < 1e-15
+
+
+def _brute_xcorr(a, b):
+    """c(d) = sum_u a(u + d) b(u), circular, by direct summation."""
+    n = a.shape[0]
+    c = np.zeros((n, n))
+    for dy in range(n):
+        for dx in range(n):
+            c[dy, dx] = (np.roll(a, (-dy, -dx), axis=(0, 1)) * b).sum()
+    return c
+
+
+def test_cross_power_map_is_centred_cross_correlation(o):
+    """B = 0: C2R(F_a conj(F_b) z / sizec) = n^2/sizec * xcorr(a, b) moved by
+    n/2 (the (-1)^{x+y'} factor), checked against direct summation."""
+    n = 16
+    rng = np.random.default_rng(5)
+    a, b = rng.standard_normal((n, n)), rng.standard_normal((n, n))
+    X = o.cross_power(o.r2c2d(a), o.r2c2d(b), n, 0.0)
+    m = o.c2r2d(X)
+    c = _brute_xcorr(a, b)
+    sizec = n * (n // 2 + 1)
+    ref = np.roll(c, (n // 2, n // 2), axis=(0, 1)) * n * n / sizec
+    assert np.abs(m - ref).max() < 1e-10 * np.abs(ref).max()
+
+
+def test_integer_roll_gives_measured_shift(o):
+    """frame_j = frame_i moved by t: measured b_ij = t_i - t_j = -t (exact)."""
+    n, box = 128, 64
+    rng = np.random.default_rng(6)
+    a = rng.standard_normal((n, n))
+    for t in [(3, -5), (-7, 2), (0, 0), (11, 9)]:
+        b = np.roll(a, (t[1], t[0]), axis=(0, 1))       # content moved by (+tx, +ty)
+        X = o.cross_power(o.r2c2d(a), o.r2c2d(b), n, 150.0)
+        sx, sy, cc, _ = o.find_shift(o.c2r2d(X), box)
+        assert (sx, sy) == (-t[0], -t[1])
+
+
+def test_pruned_box_equals_full_inverse(o):
+    n = 128
+    rng = np.random.default_rng(8)
+    X = o.cross_power(o.r2c2d(rng.standard_normal((n, n))),
+                      o.r2c2d(rng.standard_normal((n, n))), n, 150.0)
+    full = o.c2r2d(X)
+    for box in (32, 96):
+        ori = (n - box) // 2
+        pr = o.pruned_box(X, box)
+        assert np.abs(pr - full[ori:ori + box, ori:ori + box]).max() < 1e-10 * np.abs(full).max()
+
+
+def test_phase_shift_integer_is_roll(o):
+    n = 64
+    img = np.random.default_rng(10).standard_normal((n, n))
+    s = o.phase_shift(o.r2c2d(img), 5.0, -3.0)
+    back = o.c2r2d(s) / (n * n)
+    # multiplying by e^{+i 2 pi k s/n} moves the content by -s
+    assert np.abs(back - np.roll(img, (3, -5), axis=(0, 1))).max() < 1e-10
+
+
+# ---------------------------------------------------------- peak search
+def test_find_shift_delta(o):
+    box, n = 96, 512
+    m = np.zeros((box, box))
+    m[40, 57] = 1.0
+    sx, sy, cc = o.find_shift_box(m, n)
+    assert (sx, sy) == (57 - 48, 40 - 48)
+
+
+def test_find_shift_on_grid_and_subpixel(o):
+    box, n = 96, 512
+    y, x = np.mgrid[0:box, 0:box]
+    for x0, y0 in [(50.3, 47.72), (44.1, 51.55), (48.0, 48.0), (30.9, 60.4)]:
+        m = np.exp(-((x - x0) ** 2 + (y - y0) ** 2) / (2 * 1.6 ** 2))
+        sx, sy, cc = o.find_shift_box(m, n)
+        assert abs(sx * 32 - round(sx * 32)) < 1e-9 and abs(sy * 32 - round(sy * 32)) < 1e-9
+        assert abs(sx - (x0 - 48)) <= 1 / 64 + 0.01 and abs(sy - (y0 - 48)) <= 1 / 64 + 0.01
+
+
+def test_find_shift_noise_exclusion(o):
+    """wNoise excludes the central (2w+1)^2 window: a stronger centre peak is
+    skipped and the off-centre one found (cufunc.cu:465,515)."""
+    box, n = 96, 512
+    y, x = np.mgrid[0:box, 0:box]
+    m = 2.0 * np.exp(-((x - 48) ** 2 + (y - 48) ** 2) / 4.0) + np.exp(-((x - 60) ** 2 + (y - 41) ** 2) / 4.0)
+    sx, sy, _ = o.find_shift_box(m, n, wnoise=-1)
+    assert (sx, sy) == (0, 0)
+    sx, sy, _ = o.find_shift_box(m, n, wnoise=2)
+    assert abs(sx - 12) < 0.1 and abs(sy + 7) < 0.1
+
+
+# ---------------------------------------------------------- drift model
+@pytest.mark.parametrize("F,fod,P", [(10, 4, 45), (24, 11, 276), (40, 2, 253), (40, 19, 780),
+                                     (2, 0, 1), (3, 0, 3), (4, 0, 3)])
+def test_pair_counts(o, F, fod, P):
+    p = o.pair_list(F, fod)
+    assert len(p) == P
+    d = max(1, min(F // 2 - fod, F // 2))
+    assert np.all(p[:, 1] - p[:, 0] >= d)
+
+
+def _design(pairs, F):
+    A = np.zeros((len(pairs), F - 1))
+    for q, (i, j) in enumerate(pairs):
+        A[q, i:j] = 1
+    return A
+
+
+def test_lsq_matches_numpy(o):
+    rng = np.random.default_rng(12)
+    F = 12
+    pairs = o.pair_list(F, 2)
+    b = rng.standard_normal((len(pairs), 2))
+    s = o.lsq(pairs, F, b)
+    ref = np.linalg.lstsq(_design(pairs, F), b, rcond=None)[0]
+    assert np.abs(s - ref).max() < 1e-12
+
+
+def test_lsq_exact_on_consistent_data(o):
+    F = 9
+    t = np.random.default_rng(13).standard_normal((F, 2))
+    pairs = o.pair_list(F, 4)
+    b = np.array([t[i] - t[j] for i, j in pairs])
+    s = o.lsq(pairs, F, b)
+    assert np.abs(s - (t[:-1] - t[1:])).max() < 1e-12
+
+
+def test_threshold_keeps_full_rank_and_drops_outlier(o):
+    F = 10
+    pairs = o.pair_list(F, 4)
+    t = np.random.default_rng(14).standard_normal((F, 2))
+    b = np.array([t[i] - t[j] for i, j in pairs])
+    b[7] += (5.0, -4.0)  # one corrupted pair
+    s = o.lsq(pairs, F, b)
+    ki = o.residual(pairs, s, b)
+    good = o.threshold(pairs, F, ki, 1.0)
+    A = _design(pairs[good], F)
+    assert np.linalg.matrix_rank(A) == F - 1
+    assert 7 not in good
+    assert np.all(np.diff(good) > 0)
+    # kept rows: all rows under threshold, and at least F-1 rows
+    assert len(good) >= F - 1
+    assert set(np.where(ki < 1.0)[0]) <= set(good) or len(good) == F - 1
+
+
+def test_threshold_adds_rows_until_connected(o):
+    """Only nframe-1 rows pass the threshold but they leave a frame isolated:
+    the next-smallest residual rows are added until the rank is full."""
+    F = 4
+    pairs = o.pair_list(F, 1)  # d = 1: all 6 pairs
+    ki = np.array([0.1, 0.2, 5.0, 0.3, 5.0, 6.0])  # rows (0,1),(0,2),(0,3)... see below
+    # pairs: (0,1) (0,2) (0,3) (1,2) (1,3) (2,3); smallest three: (0,1),(0,2),(1,2)
+    # -> frame 3 disconnected; the 4th smallest (0,3) or (1,3) must be added
+    good = o.threshold(pairs, F, ki, 1.0)
+    A = _design(pairs[good], F)
+    assert np.linalg.matrix_rank(A) == F - 1
+    assert list(good) == [0, 1, 2, 3]
+
+
+# ------------------------------------------------------------ whole movie
+def _movie(F, n, drift, dose=20.0, seed=3):
+    from motioncorr1exp_amd.synth import make_movie
+    return make_movie(n, n, F, dose=dose, seed=seed, drift=drift, poisson=False)
+
+
+def _rolled_movie(F, n, drift, seed):
+    """Circularly shifted copies of one smooth periodic image (exact integer
+    drift, so every correlation map is an exactly shifted autocorrelation)."""
+    rng = np.random.default_rng(seed)
+    k = np.fft.fftfreq(n)
+    w = 1.0 / (1.0 + (k[:, None] ** 2 + k[None, :] ** 2) / 0.05 ** 2)
+    img = np.fft.ifft2(np.fft.fft2(rng.standard_normal((n, n))) * np.sqrt(w)).real
+    img = 10.0 + img / img.std()
+    return np.stack([np.roll(img, (int(t[1]), int(t[0])), axis=(0, 1)) for t in drift]).astype(np.float32)
+
+
+def test_noise_free_integer_drift_is_exact(o):
+    F, n = 8, 256
+    rng = np.random.default_rng(15)
+    drift = np.cumsum(rng.integers(-3, 4, size=(F, 2)), axis=0).astype(float)
+    drift -= drift[0]
+    frames = _rolled_movie(F, n, drift, 15)
+    r = o.align_movie(frames, dict(fod=2, atm=0, iter=2, box=64, nthreads=4))
+    i, j = r["pairs"][:, 0], r["pairs"][:, 1]
+    assert np.abs(r["pair_b"] - (drift[i] - drift[j])).max() < 1e-9
+    assert np.abs(r["frame_shift"][1:] - (drift[1:] - drift[0])).max() < 1e-9
+
+
+def test_subpixel_drift_recovered(o):
+    F, n = 8, 256
+    rng = np.random.default_rng(16)
+    drift = np.cumsum(rng.normal(0, 1.2, size=(F, 2)), axis=0)
+    drift -= drift[0]
+    frames, _ = _movie(F, n, drift)
+    r = o.align_movie(frames, dict(fod=2, atm=1, box=64, nthreads=4), want_boxes=False)
+    ref = F // 2 + 1  # RefFrame = nz/2+1
+    assert np.abs(r["frame_shift"] - (drift - drift[ref])).max() < 0.05
+
+
+def test_rmsd_recursion_accumulates(o):
+    """rmsd is not reset between rounds: rmsd_k = (rmsd_{k-1} + sum|d|^2)/F."""
+    from motioncorr1exp_amd.synth import make_movie
+    F = 8
+    frames, _ = make_movie(256, 256, F, dose=2.0, step=1.0, seed=17)
+    r = o.align_movie(frames, dict(fod=2, iter=3, box=64, nthreads=4), want_boxes=False)
+    s2 = r["shift2"]
+    pos = np.vstack([[0, 0], np.cumsum(s2, axis=0)])
+    rm = 0.0
+    for k in range(r["niter"]):
+        meas = r["iter_meas"][k]
+        rm = (rm + ((pos - meas) ** 2).sum()) / F
+        assert abs(rm - r["rmsd"][k]) < 1e-12
+        pos = meas - meas[0]
+
+
+def test_first_summed_frame_unshifted_without_atm(o):
+    from motioncorr1exp_amd.synth import make_movie
+    frames, _ = make_movie(128, 128, 6, dose=3.0, step=1.0, seed=18)
+    r = o.align_movie(frames, dict(fod=1, atm=0, box=32, nStartSum=1, nthreads=4), want_boxes=False)
+    assert np.isnan(r["frame_shift"][0]).all()
+    assert (r["frame_shift"][1] == 0).all()
+
+
+def test_sum_of_integer_shifts_is_roll(o):
+    """Integer shifts: the aligned sum equals the sum of rolled frames."""
+    F, n = 6, 128
+    rng = np.random.default_rng(19)
+    drift = np.cumsum(rng.integers(-2, 3, size=(F, 2)), axis=0).astype(float)
+    drift -= drift[0]
+    frames = _rolled_movie(F, n, drift, 19)
+    r = o.align_movie(frames, dict(fod=1, atm=0, box=32, nthreads=4), want_boxes=False)
+    s = np.zeros((n, n))
+    for k in range(F):
+        sh = r["frame_shift"][k]
+        s += np.roll(frames[k].astype(np.float64), (-int(np.rint(sh[1])), -int(np.rint(sh[0]))), axis=(0, 1))
+    assert np.abs(r["sum_image"] - s).max() < 1e-8 * np.abs(s).max()
+
+
+def test_sharpness_gradient_central_differences(o):
+    F, n = 4, 64
+    rng = np.random.default_rng(20)
+    spectra = np.stack([o.padded(o.r2c2d(rng.standard_normal((n, n)))) for _ in range(F)])
+    s0 = rng.normal(0, 0.7, size=(F, 2))
+    J, g = o.sharpness(spectra, n, 150.0, s0)
+    h = 1e-3
+    for f in range(F):
+        for ax in range(2):
+            sp, sm = s0.copy(), s0.copy()
+            sp[f, ax] += h
+            sm[f, ax] -= h
+            fd = (o.sharpness(spectra, n, 150.0, sp, False)[0] - o.sharpness(spectra, n, 150.0, sm, False)[0]) / (2 * h)
+            assert abs(fd - g[f, ax]) <= 1e-6 * max(1.0, abs(J))
