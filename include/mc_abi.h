@@ -127,6 +127,8 @@ typedef struct {
   size_t frame_pitch;  /* bytes between frames (0: ny * row_pitch)        */
   int nz_file;         /* frames in the stack file; RefFrame = nz_file/2+1
                           (DFAlign.cu:1133); 0: nz                        */
+  int first_frame;     /* absolute index of frames[0]; frame indices in
+                          mc_params are absolute                          */
 } mc_movie_desc;
 
 /* Alignment parameters: APARA (DFAlign.h:16-63) and CDFAlign::m_iter /
@@ -169,6 +171,8 @@ typedef struct {
   double *shift2;      /* [2K] increments after outlier rejection           */
   double *ki2;         /* [P] residuals of the kept rows                     */
   double *iter_meas;   /* [iter*2F] per-round measured positions            */
+  double *iter_cc;     /* [iter*F] per-round CC of each frame vs the sum      */
+  float *iter_boxmaps; /* [iter*F*B*B] per-round box of each frame vs sum    */
   double *rmsd;        /* [iter] accumulated RMSD recursion                  */
   double *final_shift; /* [2K] increments after iteration                   */
   double *frame_shift; /* [2F] shift applied to each frame in the sum (NaN:

@@ -27,7 +27,8 @@ class MovieDesc(ctypes.Structure):
     _fields_ = [("frames", ctypes.c_void_p), ("frames2", ctypes.c_void_p),
                 ("dtype", ctypes.c_int), ("nx", ctypes.c_int), ("ny", ctypes.c_int),
                 ("nz", ctypes.c_int), ("row_pitch", ctypes.c_size_t),
-                ("frame_pitch", ctypes.c_size_t), ("nz_file", ctypes.c_int)]
+                ("frame_pitch", ctypes.c_size_t), ("nz_file", ctypes.c_int),
+                ("first_frame", ctypes.c_int)]
 
 
 class Params(ctypes.Structure):
@@ -53,6 +54,7 @@ class Outputs(ctypes.Structure):
                 ("shift1", _P(ctypes.c_double)), ("ki1", _P(ctypes.c_double)),
                 ("good", _P(ctypes.c_int)), ("shift2", _P(ctypes.c_double)),
                 ("ki2", _P(ctypes.c_double)), ("iter_meas", _P(ctypes.c_double)),
+                ("iter_cc", _P(ctypes.c_double)), ("iter_boxmaps", _P(ctypes.c_float)),
                 ("rmsd", _P(ctypes.c_double)), ("final_shift", _P(ctypes.c_double)),
                 ("frame_shift", _P(ctypes.c_double)), ("frame_mean", _P(ctypes.c_double)),
                 ("sum_image", _P(ctypes.c_float)), ("sum2_image", _P(ctypes.c_float)),
@@ -239,6 +241,7 @@ class Context:
         m.row_pitch = 0
         m.frame_pitch = 0
         m.nz_file = nz_file
+        m.first_frame = 0
         return m
 
     def align_movie(self, dev_frames, shape, dtype, params, dev_frames2=None, want=(),
@@ -258,6 +261,8 @@ class Context:
                      shift1=((K, 2), np.float64), ki1=((P,), np.float64),
                      good=((P,), np.int32), shift2=((K, 2), np.float64), ki2=((P,), np.float64),
                      iter_meas=((max(p.iter, 1), F, 2), np.float64),
+                     iter_cc=((max(p.iter, 1), F), np.float64),
+                     iter_boxmaps=((max(p.iter, 1), F, B, B), np.float32),
                      rmsd=((max(p.iter, 1),), np.float64), final_shift=((K, 2), np.float64),
                      frame_shift=((F, 2), np.float64), frame_mean=((F,), np.float64),
                      sum_image=((n, n), np.float32), sum2_image=((n, n), np.float32),
@@ -278,8 +283,9 @@ class Context:
             res["good"] = res["good"][: r.ngood]
         if "ki2" in res:
             res["ki2"] = res["ki2"][: r.ngood]
-        if "iter_meas" in res:
-            res["iter_meas"] = res["iter_meas"][: r.niter]
+        for k in ("iter_meas", "iter_cc", "iter_boxmaps"):
+            if k in res:
+                res[k] = res[k][: r.niter]
         if "rmsd" in res:
             res["rmsd"] = res["rmsd"][: r.niter]
         return res

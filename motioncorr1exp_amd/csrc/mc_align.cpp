@@ -443,15 +443,16 @@ static int load_frames(mc_context_s* c, const mc_movie_desc* m, const mc_params*
                        int* nframe_out) {
   MC_TRY(movie_geometry(m, p, g));
   const int nframe = p->nEnd - p->nStart + 1;
-  if (p->nStart < 0 || p->nEnd >= m->nz || nframe < 2)
-    return set_err(MC_ERR_ARG, "frame range [%d, %d] invalid for %d frames", p->nStart, p->nEnd,
-                   m->nz);
+  if (p->nStart < m->first_frame || p->nEnd >= m->first_frame + m->nz || nframe < 2)
+    return set_err(MC_ERR_ARG, "frame range [%d, %d] invalid for frames [%d, %d)", p->nStart,
+                   p->nEnd, m->first_frame, m->first_frame + m->nz);
   if (!m->frames) return set_err(MC_ERR_ARG, "frames is NULL");
   MC_TRY(ensure_spectra(c, g->n, nframe, m->frames2 != nullptr));
   {
     KScope ks(c, MC_KT_SPECTRA);
-    MC_TRY(run_stage1(c, m->frames, m, p, *g, p->nStart, nframe, 0));
-    if (m->frames2) MC_TRY(run_stage1(c, m->frames2, m, p, *g, p->nStart, nframe, c->off2));
+    const int first = p->nStart - m->first_frame;
+    MC_TRY(run_stage1(c, m->frames, m, p, *g, first, nframe, 0));
+    if (m->frames2) MC_TRY(run_stage1(c, m->frames2, m, p, *g, first, nframe, c->off2));
   }
   *nframe_out = nframe;
   return MC_OK;
@@ -751,10 +752,12 @@ extern "C" int mc_align_movie(mc_context ctx, const mc_movie_desc* m, const mc_p
     std::vector<double> sh(2 * (size_t)F);
     for (int k = 0; k < 2 * F; k++) sh[k] = -oldx[k];
     MC_TRY(shift_sum(ctx, sh, slot(ctx, ctx->sum_slot), nullptr, -1));
-    MC_TRY(pair_stage(ctx, islots, p->bfactor, box, wnoise, res, nullptr));
+    MC_TRY(pair_stage(ctx, islots, p->bfactor, box, wnoise, res,
+                      out->iter_boxmaps ? out->iter_boxmaps + (size_t)it * F * box * box : nullptr));
     for (int k = 0; k < F; k++) {
       newx[2 * k] = res[3 * k];
       newx[2 * k + 1] = res[3 * k + 1];
+      if (out->iter_cc) out->iter_cc[(size_t)it * F + k] = res[3 * k + 2];
     }
     if (out->iter_meas) memcpy(out->iter_meas + (size_t)it * 2 * F, newx.data(), sizeof(double) * 2 * F);
     const double rx = newx[0], ry = newx[1];
