@@ -189,6 +189,8 @@ typedef struct {
   int sum2_written, rawsum_written;
   double rmsd;
   float ms_total, ms_spectra, ms_pairs, ms_peaks, ms_solve, ms_iter, ms_sum;
+  double sharp_j0, sharp_j1; /* sharpness before / after refinement       */
+  int grad_accepted;         /* accepted refinement steps                 */
 } mc_report;
 
 /* Whole movie: DFAlign.cu:456-1355 numerics (without file I/O and logs). */
@@ -216,6 +218,17 @@ int mc_find_peaks(mc_context ctx, const float *boxmaps, int npairs, int box, int
  * (2F doubles; NaN x = frame skipped) and inverse to an n x n image scaled
  * by 1/n^2 (host). */
 int mc_sum_image(mc_context ctx, const double *frame_shift, float *image);
+
+/* Sharpness of the phase-shift-and-sum (no reference counterpart):
+ * J = sum_k w(k) |sum_f F_f(k) e^{i 2 pi k.s_f/n}|^2 over the half plane
+ * (w = exp(-B|k|^2/(2n^2)), doubled off the DC/Nyquist columns), and its
+ * adjoint dJ/ds_f (grad: 2F doubles) on the resident spectra.  Frames with a
+ * NaN x shift are left out of the sum and get a zero gradient.  With
+ * grad_steps > 0, mc_align_movie maximises J over the summed frames' shifts
+ * before the final sum (backtracking gradient ascent, first step
+ * grad_step_px, default 0.05 px); the peak-search shifts are the start. */
+int mc_sharpness(mc_context ctx, const double *frame_shift, float bfactor, double *J,
+                 double *grad);
 
 /* Host FP64 drift model (mfunc.h:202-298, DFAlign.cu:914-948). */
 int mc_pair_list(int nframe, int fod, int *pairs, int cap);

@@ -69,7 +69,8 @@ class Report(ctypes.Structure):
                 ("ms_total", ctypes.c_float), ("ms_spectra", ctypes.c_float),
                 ("ms_pairs", ctypes.c_float), ("ms_peaks", ctypes.c_float),
                 ("ms_solve", ctypes.c_float), ("ms_iter", ctypes.c_float),
-                ("ms_sum", ctypes.c_float)]
+                ("ms_sum", ctypes.c_float), ("sharp_j0", ctypes.c_double),
+                ("sharp_j1", ctypes.c_double), ("grad_accepted", ctypes.c_int)]
 
 
 class MaskElem(ctypes.Structure):
@@ -86,6 +87,7 @@ ABI_SYMBOLS = [
     "mc_context_create", "mc_context_destroy", "mc_align_movie", "mc_load_frames",
     "mc_set_spectra", "mc_get_spectra", "mc_pair_shifts", "mc_find_peaks", "mc_sum_image",
     "mc_pair_list", "mc_lsq", "mc_threshold", "mc_kernel_timing", "mc_kernel_timing_get",
+    "mc_sharpness",
 ]
 
 _lib = None
@@ -136,6 +138,7 @@ def lib():
         L.mc_threshold.argtypes = [vp, i, i, vp, d, vp]
         L.mc_kernel_timing.argtypes = [vp, i]
         L.mc_kernel_timing_get.argtypes = [vp, i, _P(i), _P(d)]
+        L.mc_sharpness.argtypes = [vp, vp, f, _P(d), vp]
         _lib = L
     return _lib
 
@@ -332,6 +335,13 @@ class Context:
         img = np.zeros((n, n), np.float32)
         check(self.L.mc_sum_image(self.h, fs.ctypes.data, img.ctypes.data))
         return img
+
+    def sharpness(self, frame_shift, bfactor):
+        fs = np.ascontiguousarray(frame_shift, np.float64)
+        J = ctypes.c_double()
+        g = np.zeros_like(fs)
+        check(self.L.mc_sharpness(self.h, fs.ctypes.data, bfactor, ctypes.byref(J), g.ctypes.data))
+        return J.value, g
 
     def kernel_timing(self, enable):
         check(self.L.mc_kernel_timing(self.h, 1 if enable else 0))

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -89,7 +90,7 @@ struct mc_context_s {
   int device = 0;
   hipStream_t st = nullptr;
   DevBuf spec, rt, rowsum, fill, padP, pairs_d, tbuf, box, peaks, shifts, image, tmp, means,
-      sum2img;
+      sum2img, wk, gpart, gout;
   // resident spectra
   int n = 0, nframe = 0, nslots = 0, off2 = 0, sum_slot = 0;
   size_t spec_stride = 0;
@@ -156,7 +157,8 @@ extern "C" int mc_context_destroy(mc_context ctx) {
   (void)hipStreamSynchronize(ctx->st);
   DevBuf* bufs[] = {&ctx->spec, &ctx->rt, &ctx->rowsum, &ctx->fill, &ctx->padP,
                     &ctx->pairs_d, &ctx->tbuf, &ctx->box, &ctx->peaks, &ctx->shifts,
-                    &ctx->image, &ctx->tmp, &ctx->means, &ctx->sum2img};
+                    &ctx->image, &ctx->tmp, &ctx->means, &ctx->sum2img, &ctx->wk,
+                    &ctx->gpart, &ctx->gout};
   for (DevBuf* b : bufs) b->release();
   for (hipEvent_t e : ctx->pool) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->st);
@@ -412,7 +414,10 @@ static int run_stage1(mc_context_s* c, const void* frames, const mc_movie_desc* 
                       const mc_params* p, const Geom& g, int first, int count, int s0) {
   const int nkx = g.n / 2 + 1;
   const size_t rt_per = (size_t)nkx * g.nsu;
-  int chunk = (int)std::max<size_t>(1, ((size_t)512 << 20) / (rt_per * sizeof(float2)));
+  // Frames per launch pair: the row-pass scratch of a chunk is sized to stay
+  // within the 256 MB Infinity Cache between the row and column passes.
+  static const size_t chunk_mb = getenv("MC_STAGE1_CHUNK_MB") ? atoi(getenv("MC_STAGE1_CHUNK_MB")) : 192;
+  int chunk = (int)std::max<size_t>(1, (chunk_mb << 20) / (rt_per * sizeof(float2)));
   chunk = std::min(chunk, count);
   HIP_TRY(c->rt.ensure(sizeof(float2) * rt_per * chunk));
   HIP_TRY(c->rowsum.ensure(sizeof(float) * (size_t)g.nsu * chunk));
@@ -504,6 +509,98 @@ static int shift_sum(mc_context_s* c, const std::vector<double>& sh, float2* dst
   KScope ks(c, MC_KT_SUM);
   HIP_TRY(mc::launch_shift_sum(c->spec.as<float2>(), c->spec_stride, c->nframe, c->n,
                                c->shifts.as<double2>(), dst, dst2, f2_last, c->st));
+  return MC_OK;
+}
+
+// Sharpness weights: wk[kx] = a_kx exp(-B kx^2/(2n^2)) (a = 1 at kx = 0 and
+// n/2, else 2), wk[n/2+1+ky] = exp(-B y'^2/(2n^2)).
+static int sharp_weights(mc_context_s* c, int n, float bfactor) {
+  static thread_local std::vector<float> h;
+  const int H = n / 2 + 1;
+  h.assign(H + n, 0.f);
+  const double m = -0.5 * (double)bfactor / ((double)n * n);
+  for (int kx = 0; kx < H; kx++)
+    h[kx] = (float)(((kx == 0 || 2 * kx == n) ? 1.0 : 2.0) * std::exp(m * kx * kx));
+  for (int ky = 0; ky < n; ky++) {
+    const int yy = ky < n / 2 ? ky : ky - n;
+    h[H + ky] = (float)std::exp(m * (double)yy * yy);
+  }
+  HIP_TRY(c->wk.ensure(sizeof(float) * h.size()));
+  HIP_TRY(hipMemcpyAsync(c->wk.p, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice, c->st));
+  HIP_TRY(hipStreamSynchronize(c->st));
+  return MC_OK;
+}
+
+// J(s) and dJ/ds on the resident spectra; fs: 2F shifts (NaN: frame out).
+static int sharpness_eval(mc_context_s* c, const std::vector<double>& fs, double* J,
+                          std::vector<double>& grad) {
+  const int F = c->nframe;
+  const int nchunks = 256;
+  MC_TRY(shift_sum(c, fs, slot(c, c->sum_slot), nullptr, -1));
+  HIP_TRY(c->gpart.ensure(sizeof(double2) * (size_t)nchunks * (F + 1)));
+  HIP_TRY(c->gout.ensure(sizeof(double2) * (F + 1)));
+  {
+    KScope ks(c, MC_KT_SUM);
+    HIP_TRY(mc::launch_sharpness(c->spec.as<float2>(), c->spec_stride, F, c->n,
+                                 c->shifts.as<double2>(), slot(c, c->sum_slot), c->wk.as<float>(),
+                                 c->gpart.as<double2>(), nchunks, c->gout.as<double2>(), c->st));
+  }
+  std::vector<double> o(2 * (size_t)(F + 1));
+  HIP_TRY(hipMemcpyAsync(o.data(), c->gout.p, sizeof(double) * o.size(), hipMemcpyDeviceToHost, c->st));
+  HIP_TRY(hipStreamSynchronize(c->st));
+  grad.assign(o.begin(), o.begin() + 2 * F);
+  *J = o[2 * F];
+  return MC_OK;
+}
+
+// Gradient ascent of J over the summed frames' shifts with a backtracking
+// trust region: s += step * g / max|g|; a step that lowers J is rejected and
+// the step halved.
+static int refine_shifts(mc_context_s* c, std::vector<double>& fs, float bfactor, int steps,
+                         float step_px, double* j0, double* j1, int* accepted) {
+  MC_TRY(sharp_weights(c, c->n, bfactor));
+  double J = 0;
+  std::vector<double> g;
+  MC_TRY(sharpness_eval(c, fs, &J, g));
+  *j0 = J;
+  double step = step_px > 0 ? step_px : 0.05;
+  int acc = 0;
+  for (int t = 0; t < steps; t++) {
+    double gmax = 0;
+    for (size_t k = 0; k < g.size(); k++)
+      if (!std::isnan(fs[k])) gmax = std::max(gmax, std::fabs(g[k]));
+    if (!(gmax > 0)) break;
+    std::vector<double> trial(fs);
+    for (size_t k = 0; k < g.size(); k++)
+      if (!std::isnan(fs[k])) trial[k] += step * g[k] / gmax;
+    double Jt = 0;
+    std::vector<double> gt;
+    MC_TRY(sharpness_eval(c, trial, &Jt, gt));
+    if (Jt > J) {
+      fs.swap(trial);
+      g.swap(gt);
+      J = Jt;
+      acc++;
+    } else {
+      step *= 0.5;
+    }
+  }
+  *j1 = J;
+  *accepted = acc;
+  return MC_OK;
+}
+
+extern "C" int mc_sharpness(mc_context ctx, const double* frame_shift, float bfactor, double* J,
+                            double* grad) {
+  if (!ctx || !frame_shift) return set_err(MC_ERR_ARG, "NULL argument");
+  if (!ctx->nframe) return set_err(MC_ERR_STATE, "no spectra loaded");
+  HIP_TRY(hipSetDevice(ctx->device));
+  MC_TRY(sharp_weights(ctx, ctx->n, bfactor));
+  std::vector<double> fs(frame_shift, frame_shift + 2 * (size_t)ctx->nframe), g;
+  double j = 0;
+  MC_TRY(sharpness_eval(ctx, fs, &j, g));
+  if (J) *J = j;
+  if (grad) memcpy(grad, g.data(), sizeof(double) * g.size());
   return MC_OK;
 }
 
@@ -813,6 +910,13 @@ extern "C" int mc_align_movie(mc_context ctx, const mc_movie_desc* m, const mc_p
     }
   }
   if (out->frame_shift) memcpy(out->frame_shift, fs.data(), sizeof(double) * 2 * F);
+  double sj0 = 0, sj1 = 0;
+  int gacc = 0;
+  if (p->grad_steps > 0) {
+    // adjoint sharpness refinement of the summed frames' shifts
+    MC_TRY(refine_shifts(ctx, fs, p->bfactor, p->grad_steps, p->grad_step_px, &sj0, &sj1, &gacc));
+    if (out->frame_shift) memcpy(out->frame_shift, fs.data(), sizeof(double) * 2 * F);
+  }
   const bool want2 = out->sum2_image && p->nEndSum2 > 0 && p->nEndSum2 < p->nEndSum;
   float2* s2 = nullptr;
   if (want2) {
@@ -850,6 +954,9 @@ extern "C" int mc_align_movie(mc_context ctx, const mc_movie_desc* m, const mc_p
     rep->rmsd = rmsd;
     rep->sum2_written = sum2_written;
     rep->rawsum_written = rawsum_written;
+    rep->sharp_j0 = sj0;
+    rep->sharp_j1 = sj1;
+    rep->grad_accepted = gacc;
     rep->ms_spectra = elapsed(ev[0], ev[1]);
     rep->ms_pairs = elapsed(ev[1], ev[2]);
     rep->ms_solve = elapsed(ev[2], ev[3]);

@@ -60,6 +60,14 @@ hipError_t launch_shift_sum(const float2* spectra, size_t spec_stride, int nfram
                             const double2* shifts, float2* sum, float2* sum2, int f2_last,
                             hipStream_t st);
 
+// Sharpness J of the shifted sum S (already formed by launch_shift_sum with
+// the same shifts) and dJ/ds_f: out[f] = (dJ/dsx_f, dJ/dsy_f), out[F] = (J, 0).
+// wk: [n/2+1] column weights (incl. the half-plane factor) then [n] row
+// weights, w(kx, ky) = wk[kx] * wk[n/2+1+ky].  part: nchunks*(F+1) double2.
+hipError_t launch_sharpness(const float2* spectra, size_t spec_stride, int nframes, int n,
+                            const double2* shifts, const float2* S, const float* wk,
+                            double2* part, int nchunks, double2* out, hipStream_t st);
+
 // Layout conversion between the resident [kx][ky] spectrum and the padded
 // row-major [ky][kx] layout (n rows of n/2+1 complex).
 hipError_t launch_spec_to_padded(const float2* spec, int n, float2* padded, hipStream_t st);

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -113,6 +114,119 @@ __global__ __launch_bounds__(NT) void k_pair_cols(const float2* __restrict__ spe
       tbuf[((size_t)p * H + kx) * box + tid] = acc;
     }
     // buf and part are rewritten only after the next item's first barrier
+  }
+}
+
+// Specialised column pass for M = 128 (box <= 128), n >= 512, NT = 256:
+//  stage 0 (radix 16 over b): the 16 inputs X[a + A(j + 8r)] of butterfly
+//    (a, j) are loaded straight from the two spectra (coalesced over a),
+//    multiplied and weighted in registers, transformed and written to LDS;
+//  stage 1 (radix 8, Ns = 16): reads and writes the same 8 LDS slots per
+//    butterfly, so it runs in place without a barrier; its 7 twiddles depend
+//    only on tid % 16 and stay in registers;
+//  combine: SP adjacent lanes share one output row and reduce with xor
+//    shuffles (no LDS round trip).
+template <int N, int SP>
+__global__ __launch_bounds__(256) void k_pair_cols_m128(const float2* __restrict__ spectra,
+                                                        size_t spec_stride,
+                                                        const int2* __restrict__ pairs,
+                                                        int npairs,
+                                                        const float* __restrict__ wxy, int box,
+                                                        float2* __restrict__ tbuf,
+                                                        const float2* __restrict__ tw) {
+  constexpr int NT = 256;
+  constexpr int M = 128;
+  constexpr int A = N / M;
+  constexpr int H = N / 2 + 1;
+  constexpr int PB = 4;
+  constexpr int TS = lds_len<PB>(M) | 1;
+  constexpr int NB0 = A * 8;                    // stage-0 butterflies
+  constexpr int P0 = (NB0 + NT - 1) / NT;
+  constexpr int NB1 = A * 16;                   // stage-1 butterflies
+  constexpr int P1 = (NB1 + NT - 1) / NT;
+  constexpr int AS = A / SP;
+  static_assert(NT % 16 == 0 && A % SP == 0, "shape");
+  __shared__ float2 buf[A * TS];
+  const int tid = threadIdx.x;
+  // stage-0 element weights (fixed per thread)
+  float wy[P0][16];
+#pragma unroll
+  for (int p = 0; p < P0; p++) {
+    const int q = tid + p * NT;
+    const int a = q % A, j = q / A;
+#pragma unroll
+    for (int r = 0; r < 16; r++) wy[p][r] = q < NB0 ? wxy[H + a + A * (j + 8 * r)] : 0.f;
+  }
+  // stage-1 twiddles W_128^{k r}, k = tid % 16 (inverse sign)
+  float2 t1[8];
+#pragma unroll
+  for (int r = 0; r < 8; r++) t1[r] = tw[((tid % 16) * r * (N / M)) & (N - 1)];
+  // combine: lane group (SP lanes) per output row
+  const int ori = (N - box) / 2;
+  const int cr = tid / SP, ch = tid % SP;
+  const bool cact = cr < box;
+  const int R = ori + (cact ? cr : 0);
+  const int Rm = R & (M - 1);
+  float2 ctw[AS];
+#pragma unroll
+  for (int i = 0; i < AS; i++) ctw[i] = tw[((ch * AS + i) * R) & (N - 1)];
+  const int xcd = blockIdx.x % 8, lid = blockIdx.x / 8, nper = gridDim.x / 8;
+  const long long items = (long long)npairs * H;
+  const long long i0 = items * xcd / 8, i1 = items * (xcd + 1) / 8;
+  for (long long it = i0 + lid; it < i1; it += nper) {
+    const int kx = (int)(it / npairs), p = (int)(it % npairs);
+    const int2 pr = pairs[p];
+    const float2* sa = spectra + (size_t)pr.x * spec_stride + (size_t)kx * N;
+    const float2* sb = spectra + (size_t)pr.y * spec_stride + (size_t)kx * N;
+    const float wx = wxy[kx];
+    // ---- stage 0 from global
+#pragma unroll
+    for (int pp = 0; pp < P0; pp++) {
+      const int q = tid + pp * NT;
+      if (NB0 % NT == 0 || q < NB0) {
+        const int a = q % A, j = q / A;
+        float2 v[16];
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+          const int e = a + A * (j + 8 * r);
+          v[r] = cmulc(sa[e], sb[e]);
+        }
+#pragma unroll
+        for (int r = 0; r < 16; r++) v[r] = cscale(v[r], wx * wy[pp][r]);
+        dft16<1>(v);
+#pragma unroll
+        for (int r = 0; r < 16; r++) buf[a * TS + lix<PB>(16 * j + r)] = v[r];
+      }
+    }
+    __syncthreads();
+    // ---- stage 1 in place (radix 8, Ns = 16)
+#pragma unroll
+    for (int pp = 0; pp < P1; pp++) {
+      const int q = tid + pp * NT;
+      if (NB1 % NT == 0 || q < NB1) {
+        const int a = q / 16, j = q % 16;
+        float2 v[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) v[r] = buf[a * TS + lix<PB>(j + 16 * r)];
+#pragma unroll
+        for (int r = 1; r < 8; r++) v[r] = cmul(v[r], t1[r]);
+        dft8<1>(v);
+#pragma unroll
+        for (int r = 0; r < 8; r++) buf[a * TS + lix<PB>(j + 16 * r)] = v[r];
+      }
+    }
+    __syncthreads();
+    // ---- combine T[R] = sum_a e^{2 pi i a R/N} Y_a[R mod M]
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < AS; i++) acc = cadd(acc, cmul(ctw[i], buf[(ch * AS + i) * TS + lix<PB>(Rm)]));
+#pragma unroll
+    for (int o = 1; o < SP; o <<= 1) {
+      acc.x += __shfl_xor(acc.x, o);
+      acc.y += __shfl_xor(acc.y, o);
+    }
+    if (cact && ch == 0) tbuf[((size_t)p * H + kx) * box + cr] = acc;
+    __syncthreads();  // buf is rewritten by the next item's stage 0
   }
 }
 
@@ -227,10 +341,30 @@ static hipError_t pair_cols_t(const float2* spectra, size_t spec_stride, const i
   return hipGetLastError();
 }
 
+template <int N, int SP>
+static hipError_t pair_cols_m128(const float2* spectra, size_t spec_stride, const int2* pairs,
+                                 int npairs, const float* wxy, int box, float2* tbuf,
+                                 hipStream_t st) {
+  const float2* tw = twiddles(N);
+  if (!tw) return hipErrorOutOfMemory;
+  hipLaunchKernelGGL((k_pair_cols_m128<N, SP>), dim3(pair_grid()), dim3(256), 0, st, spectra,
+                     spec_stride, pairs, npairs, wxy, box, tbuf, tw);
+  return hipGetLastError();
+}
+
 template <int N>
 static hipError_t pair_cols_n(const float2* spectra, size_t spec_stride, const int2* pairs,
                               int npairs, const float* wxy, int box, float2* tbuf,
                               hipStream_t st) {
+  if constexpr (N >= 512) {
+    static const bool legacy = getenv("MC_PAIR_COLS_V2") != nullptr;
+    if (!legacy) {
+      if (box <= 64)
+        return pair_cols_m128<N, 4>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
+      if (box <= 128)
+        return pair_cols_m128<N, 2>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
+    }
+  }
   if constexpr (N / 128 >= 4) {
     if (box <= 64)
       return pair_cols_t<N, 128, 4>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);

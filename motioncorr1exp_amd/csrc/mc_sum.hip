@@ -51,6 +51,90 @@ hipError_t launch_shift_sum(const float2* spectra, size_t spec_stride, int nfram
   return hipGetLastError();
 }
 
+// ------------------------------------------------ sharpness and its adjoint
+// J(s) = sum_k w(k) |S(k)|^2 with S = sum_f F_f e^{i phi_f}, phi_f(k) =
+// 2 pi (s_fx kx + s_fy ky')/n, w = exp(-B|k|^2/(2n^2)) * a_kx (a = 1 at DC and
+// Nyquist columns, 2 otherwise: the half-plane counts both halves).
+// dJ/ds_fx = -2 sum_k w (2 pi kx/n) Im(conj(S) F_f e^{i phi_f}); same for y.
+// Work item (chunk c, frame f): frame f < F accumulates its two gradient
+// components over the chunk's elements, f == F accumulates J.  Items of one
+// chunk are consecutive on one XCD, so the chunk of S is reused from L2.
+__global__ __launch_bounds__(256) void k_sharp_partial(const float2* __restrict__ spectra,
+                                                       size_t spec_stride, int nframes, int n,
+                                                       const double2* __restrict__ shifts,
+                                                       const float2* __restrict__ S,
+                                                       const float* __restrict__ wk,
+                                                       int nchunks, double2* __restrict__ part) {
+  __shared__ double rx[256], ry[256];
+  const int L = xcd_remap(blockIdx.x, gridDim.x);
+  const int c = L / (nframes + 1), f = L % (nframes + 1);
+  const int H = n / 2 + 1;
+  const size_t total = (size_t)H * n;
+  const size_t e0 = total * c / nchunks, e1 = total * (c + 1) / nchunks;
+  double gx = 0, gy = 0;
+  const double2 s = f < nframes ? shifts[f] : make_double2(0.0, 0.0);
+  const bool skip = f < nframes && s.x != s.x;
+  const float2* F = spectra + (size_t)(f < nframes ? f : 0) * spec_stride;
+  const double inv_n = 1.0 / n;
+  if (!skip) {
+    for (size_t e = e0 + threadIdx.x; e < e1; e += 256) {
+      const int kx = (int)(e / n), ky = (int)(e % n);
+      const int yy = ky < n / 2 ? ky : ky - n;
+      const float w = wk[kx] * wk[H + ky];
+      const float2 sv = S[e];
+      if (f == nframes) {
+        gx += (double)w * ((double)sv.x * sv.x + (double)sv.y * sv.y);
+      } else {
+        double u = (s.x * kx + s.y * yy) * inv_n;
+        u -= floor(u);
+        float sn, cs;
+        sincospif(2.0f * (float)u, &sn, &cs);
+        const float2 g = cmul(F[e], make_float2(cs, sn));
+        const double im = (double)sv.x * g.y - (double)sv.y * g.x;  // Im(conj(S) G)
+        const double t = -2.0 * w * im * (2.0 * M_PI * inv_n);
+        gx += t * kx;
+        gy += t * yy;
+      }
+    }
+  }
+  rx[threadIdx.x] = gx;
+  ry[threadIdx.x] = gy;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      rx[threadIdx.x] += rx[threadIdx.x + o];
+      ry[threadIdx.x] += ry[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(size_t)c * (nframes + 1) + f] = make_double2(rx[0], ry[0]);
+}
+
+// Sums the chunk partials in a fixed order: out[f] = (gx, gy), out[F] = (J, 0).
+__global__ void k_sharp_final(const double2* __restrict__ part, int nchunks, int nframes,
+                              double2* __restrict__ out) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f > nframes) return;
+  double x = 0, y = 0;
+  for (int c = 0; c < nchunks; c++) {
+    const double2 v = part[(size_t)c * (nframes + 1) + f];
+    x += v.x;
+    y += v.y;
+  }
+  out[f] = make_double2(x, y);
+}
+
+hipError_t launch_sharpness(const float2* spectra, size_t spec_stride, int nframes, int n,
+                            const double2* shifts, const float2* S, const float* wk,
+                            double2* part, int nchunks, double2* out, hipStream_t st) {
+  const int nb = nchunks * (nframes + 1);
+  hipLaunchKernelGGL(k_sharp_partial, dim3(nb), dim3(256), 0, st, spectra, spec_stride, nframes,
+                     n, shifts, S, wk, nchunks, part);
+  hipLaunchKernelGGL(k_sharp_final, dim3((nframes + 1 + 63) / 64), dim3(64), 0, st, part,
+                     nchunks, nframes, out);
+  return hipGetLastError();
+}
+
 // ------------------------------------------- elementwise (cufunc.cu:292-312)
 __global__ void k_add(float* __restrict__ dst, const float* __restrict__ src, size_t n) {
   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
