@@ -109,6 +109,76 @@ def cpu_baseline(frames_shape, pad, P, F, nthreads, budget_s=20.0):
     }
 
 
+def batch_leg(n_movies, steps, seed=1004):
+    """Batching form: n_movies movies of seeded uneven size (square sides
+    uniform over the even values 1024..4096, padded to the next power of two
+    with the region mean), 8..50 frames, dose uniform in [0.5, 1.5],
+    random-walk drift N(0, 0.7^2), uint8 counts generated on the device;
+    bin 1, fod 2, bft 150, pbx 96, 3 align-to-sum rounds.  Times
+    mc_align_batch (one call = one step over all movies) and the same movies
+    through mc_align_movie one by one, and checks that both give the same
+    shifts."""
+    import numpy as np
+
+    import motioncorr1exp_amd as mc
+    from motioncorr1exp_amd.synth import make_movie_gpu
+
+    rng = np.random.default_rng(seed)
+    t0 = time.time()
+    movies = []
+    for i in range(n_movies):
+        side = int(rng.integers(512, 2049)) * 2
+        F = int(rng.integers(8, 51))
+        dose = float(rng.uniform(0.5, 1.5))
+        pad = 1 << (side - 1).bit_length()
+        dev, drift = make_movie_gpu(side, side, F, dose=dose, step=0.7, seed=seed * 100 + i)
+        prm = mc.default_params(F, pad=pad, bin=1.0, FrameDistOffset=2, bfactor=150.0,
+                                CCPeakSearchDim=96, iter=3)
+        movies.append(dict(dev=dev, shape=(F, side, side), dtype=np.uint8, params=prm,
+                           want=("frame_shift",), drift=drift))
+    mc.check(mc.lib().mc_sync())
+    log("batch: %d movies generated on the device in %.1fs" % (n_movies, time.time() - t0))
+    ctx = mc.Context(0)
+    res, br = ctx.align_batch(movies)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res, br = ctx.align_batch(movies)
+    mc.check(mc.lib().mc_sync())
+    dt = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    seq = [ctx.align_movie(m["dev"], m["shape"], m["dtype"], m["params"], want=("frame_shift",))
+           for m in movies]
+    mc.check(mc.lib().mc_sync())
+    dts = time.perf_counter() - t0
+    same = max(float(np.nanmax(np.abs(a["frame_shift"] - b["frame_shift"])))
+               for a, b in zip(res, seq))
+    err = []
+    for m, r in zip(movies, res):
+        F = m["shape"][0]
+        truth = m["drift"] - m["drift"][F // 2 + 1]  # RefFrame = nz/2+1
+        err.append(r["frame_shift"] - truth)
+    err = np.concatenate(err)
+    sizes = {}
+    for m in movies:
+        n = m["params"]["pad"]
+        sizes[n] = sizes.get(n, 0) + 1
+    ctx.close()
+    out = {"movies": n_movies, "steps": steps, "movies_per_s": n_movies / dt,
+           "ms_per_batch": dt * 1e3, "sequential_movies_per_s": n_movies / dts,
+           "speedup_vs_sequential": dts / dt,
+           "frames_total": int(sum(m["shape"][0] for m in movies)),
+           "movies_per_transform_size": {str(k): v for k, v in sorted(sizes.items())},
+           "batch_report": br, "max_shift_diff_batch_vs_single_px": same,
+           "vs_truth_rms_px": float(np.sqrt(np.mean(err ** 2))),
+           "workload": "%d movies, sides U{1024..4096} even padded to 2^k, frames U{8..50}, "
+                       "dose U[0.5,1.5], uint8 counts, bin 1, fod 2, bft 150, pbx 96, itr 3"
+                       % n_movies}
+    for m in movies:
+        m["dev"].free()
+    log("batch: %s" % {k: v for k, v in out.items() if k != "workload"})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +188,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-itr3", action="store_true")
     ap.add_argument("--frames", type=int, default=40)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batching form")
+    ap.add_argument("--batch-movies", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--calib-copy", action="store_true",
                     help="only copy 1 GiB device-to-device (counter calibration)")
     args = ap.parse_args()
@@ -270,6 +343,10 @@ def main():
         log("cpu baseline %s" % out["cpu_baseline"])
 
     ctx.close()
+    dev.free()
+    sum_dev.free()
+    if not args.no_batch:
+        out["batch"] = batch_leg(args.batch_movies, args.batch_steps)
     print(json.dumps(out), flush=True)
 
 

@@ -197,6 +197,34 @@ typedef struct {
 int mc_align_movie(mc_context ctx, const mc_movie_desc *movie, const mc_params *p,
                    mc_outputs *out, mc_report *rep);
 
+/* Many movies (no reference counterpart: the reference aligns one stack per
+ * process run, DFAlign.cu:423-453).  Movies with the same binned transform
+ * size, peak box, B-factor and noise window form a group; a group is split
+ * into sub-batches whose resident spectra fit in device memory
+ * (MC_BATCH_MEM_MB, default 60% of the free memory).  In a sub-batch, stage 1
+ * runs per movie, the pair stage of all movies is one persistent launch, the
+ * drift model runs per movie on the host, the align-to-sum rounds of the
+ * movies still iterating are one batched sum launch plus one pair-stage
+ * launch, captured once as a HIP graph and replayed while that set of movies
+ * is unchanged, and the final sums run per movie.  Every movie gets the same
+ * results as mc_align_movie would give it; the ms_* fields of its report are
+ * those of its sub-batch.  The movies' frame buffers must all be resident. */
+typedef struct {
+  const mc_movie_desc *movie;
+  const mc_params *params;
+  mc_outputs *out; /* may be NULL */
+  mc_report *rep;  /* may be NULL */
+} mc_batch_item;
+
+typedef struct {
+  int groups, sub_batches;            /* launch groups and memory splits  */
+  int graph_captures, graph_replays;  /* align-to-sum round graphs        */
+  double ms_total;                    /* device time of all sub-batches   */
+} mc_batch_report;
+
+int mc_align_batch(mc_context ctx, const mc_batch_item *items, int count,
+                   mc_batch_report *brep);
+
 /* Stage entry points on the context's resident state ------------------- */
 /* Stage 1 only: frames -> resident spectra (sizes written to rep). */
 int mc_load_frames(mc_context ctx, const mc_movie_desc *movie, const mc_params *p,
@@ -244,7 +272,8 @@ int mc_threshold(const int *pairs, int npairs, int nframe, const double *ki, dou
 #define MC_KT_PEAKS 3
 #define MC_KT_SUM 4
 #define MC_KT_IMAGE 5
-#define MC_KT_COUNT 6
+#define MC_KT_ROUNDS 6 /* graph replays of whole align-to-sum rounds */
+#define MC_KT_COUNT 7
 int mc_kernel_timing(mc_context ctx, int enable);
 int mc_kernel_timing_get(mc_context ctx, int group, int *count, double *total_ms);
 

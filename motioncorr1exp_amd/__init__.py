@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libmc.so")
 MC_OK = 0
 MC_DTYPE = {np.dtype(np.int8): 0, np.dtype(np.int16): 1, np.dtype(np.float32): 2,
             np.dtype(np.uint16): 6, np.dtype(np.uint8): 100}
-KT = dict(spectra=0, pair_cols=1, pair_rows=2, peaks=3, sum=4, image=5)
+KT = dict(spectra=0, pair_cols=1, pair_rows=2, peaks=3, sum=4, image=5, rounds=6)
 
 
 class MCError(RuntimeError):
@@ -73,6 +73,17 @@ class Report(ctypes.Structure):
                 ("sharp_j1", ctypes.c_double), ("grad_accepted", ctypes.c_int)]
 
 
+class BatchItem(ctypes.Structure):
+    _fields_ = [("movie", _P(MovieDesc)), ("params", _P(Params)), ("out", _P(Outputs)),
+                ("rep", _P(Report))]
+
+
+class BatchReport(ctypes.Structure):
+    _fields_ = [("groups", ctypes.c_int), ("sub_batches", ctypes.c_int),
+                ("graph_captures", ctypes.c_int), ("graph_replays", ctypes.c_int),
+                ("ms_total", ctypes.c_double)]
+
+
 class MaskElem(ctypes.Structure):
     _fields_ = [("x", ctypes.c_int), ("y", ctypes.c_int), ("z", ctypes.c_float)]
 
@@ -87,7 +98,7 @@ ABI_SYMBOLS = [
     "mc_context_create", "mc_context_destroy", "mc_align_movie", "mc_load_frames",
     "mc_set_spectra", "mc_get_spectra", "mc_pair_shifts", "mc_find_peaks", "mc_sum_image",
     "mc_pair_list", "mc_lsq", "mc_threshold", "mc_kernel_timing", "mc_kernel_timing_get",
-    "mc_sharpness",
+    "mc_sharpness", "mc_align_batch",
 ]
 
 _lib = None
@@ -139,6 +150,7 @@ def lib():
         L.mc_kernel_timing.argtypes = [vp, i]
         L.mc_kernel_timing_get.argtypes = [vp, i, _P(i), _P(d)]
         L.mc_sharpness.argtypes = [vp, vp, f, _P(d), vp]
+        L.mc_align_batch.argtypes = [vp, _P(BatchItem), i, _P(BatchReport)]
         _lib = L
     return _lib
 
@@ -247,16 +259,11 @@ class Context:
         m.first_frame = 0
         return m
 
-    def align_movie(self, dev_frames, shape, dtype, params, dev_frames2=None, want=(),
-                    nz_file=0, sum_dev=None):
-        """Runs mc_align_movie.  `want` names optional outputs to return."""
-        p = to_params(params)
-        m = self.movie_desc(dev_frames, shape, dtype, dev_frames2, nz_file)
+    def _outputs(self, shape, p, want, sum_dev):
         F = p.nEnd - p.nStart + 1
         K = F - 1
         P = self.L.mc_pair_list(F, p.FrameDistOffset, None, 0)
         B = p.CCPeakSearchDim
-        # sizes need n: compute via mc_load_frames-free geometry on the host
         n = _binned_n(shape, p)
         res = {}
         sizes = dict(pairs=((P, 2), np.int32), pair_b=((P, 2), np.float64),
@@ -278,20 +285,54 @@ class Context:
             setattr(o, k, _ptr(res[k], ct))
         if sum_dev is not None:
             o.sum_image_dev = sum_dev.ptr
-        r = Report()
-        check(self.L.mc_align_movie(self.h, ctypes.byref(m), ctypes.byref(p), ctypes.byref(o),
-                                    ctypes.byref(r)))
+        return o, res
+
+    @staticmethod
+    def _finish(res, r):
         res["report"] = {k: getattr(r, k) for k, _ in Report._fields_}
         if "good" in res:
             res["good"] = res["good"][: r.ngood]
         if "ki2" in res:
             res["ki2"] = res["ki2"][: r.ngood]
-        for k in ("iter_meas", "iter_cc", "iter_boxmaps"):
+        for k in ("iter_meas", "iter_cc", "iter_boxmaps", "rmsd"):
             if k in res:
                 res[k] = res[k][: r.niter]
-        if "rmsd" in res:
-            res["rmsd"] = res["rmsd"][: r.niter]
         return res
+
+    def align_movie(self, dev_frames, shape, dtype, params, dev_frames2=None, want=(),
+                    nz_file=0, sum_dev=None):
+        """Runs mc_align_movie.  `want` names optional outputs to return."""
+        p = to_params(params)
+        m = self.movie_desc(dev_frames, shape, dtype, dev_frames2, nz_file)
+        o, res = self._outputs(shape, p, want, sum_dev)
+        r = Report()
+        check(self.L.mc_align_movie(self.h, ctypes.byref(m), ctypes.byref(p), ctypes.byref(o),
+                                    ctypes.byref(r)))
+        return self._finish(res, r)
+
+    def align_batch(self, movies):
+        """Runs mc_align_batch over a list of dicts with keys dev, shape,
+        dtype, params and optionally dev2, want, nz_file, sum_dev.  Returns
+        (per-movie results as align_movie returns them, batch report)."""
+        n = len(movies)
+        items = (BatchItem * n)()
+        keep, results = [], []
+        for i, mv in enumerate(movies):
+            p = to_params(mv["params"])
+            m = self.movie_desc(mv["dev"], mv["shape"], mv["dtype"], mv.get("dev2"),
+                                mv.get("nz_file", 0))
+            o, res = self._outputs(mv["shape"], p, mv.get("want", ()), mv.get("sum_dev"))
+            r = Report()
+            keep.append((p, m, o, r))
+            results.append(res)
+            items[i].movie = ctypes.pointer(m)
+            items[i].params = ctypes.pointer(p)
+            items[i].out = ctypes.pointer(o)
+            items[i].rep = ctypes.pointer(r)
+        br = BatchReport()
+        check(self.L.mc_align_batch(self.h, items, n, ctypes.byref(br)))
+        out = [self._finish(res, k[3]) for res, k in zip(results, keep)]
+        return out, {k: getattr(br, k) for k, _ in BatchReport._fields_}
 
     def load_frames(self, dev_frames, shape, dtype, params, dev_frames2=None):
         p = to_params(params)

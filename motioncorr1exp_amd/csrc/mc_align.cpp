@@ -90,7 +90,8 @@ struct mc_context_s {
   int device = 0;
   hipStream_t st = nullptr;
   DevBuf spec, rt, rowsum, fill, padP, pairs_d, tbuf, box, peaks, shifts, image, tmp, means,
-      sum2img, wk, gpart, gout;
+      sum2img, wk, gpart, gout, jobs;
+  int graph_captures = 0, graph_replays = 0;
   // resident spectra
   int n = 0, nframe = 0, nslots = 0, off2 = 0, sum_slot = 0;
   size_t spec_stride = 0;
@@ -158,7 +159,7 @@ extern "C" int mc_context_destroy(mc_context ctx) {
   DevBuf* bufs[] = {&ctx->spec, &ctx->rt, &ctx->rowsum, &ctx->fill, &ctx->padP,
                     &ctx->pairs_d, &ctx->tbuf, &ctx->box, &ctx->peaks, &ctx->shifts,
                     &ctx->image, &ctx->tmp, &ctx->means, &ctx->sum2img, &ctx->wk,
-                    &ctx->gpart, &ctx->gout};
+                    &ctx->gpart, &ctx->gout, &ctx->jobs};
   for (DevBuf* b : bufs) b->release();
   for (hipEvent_t e : ctx->pool) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->st);
@@ -463,18 +464,19 @@ static int load_frames(mc_context_s* c, const mc_movie_desc* m, const mc_params*
   return MC_OK;
 }
 
-// Pair stage on resident slots: pairs (a, b) are slot indices.
-static int pair_stage(mc_context_s* c, const std::vector<int>& slots, float bfactor, int box,
-                      int wnoise, std::vector<double>& res, float* boxmaps_host) {
-  const int np = (int)slots.size() / 2;
-  const int n = c->n;
+// Pair-stage buffers for np pairs (before any launch, outside graph capture).
+static int pair_reserve(mc_context_s* c, int np, int n, int box) {
   const int H = n / 2 + 1;
-  HIP_TRY(c->pairs_d.ensure(sizeof(int) * slots.size()));
+  HIP_TRY(c->pairs_d.ensure(sizeof(int) * 2 * (size_t)np));
   HIP_TRY(c->tbuf.ensure(sizeof(float2) * (size_t)np * H * box));
   HIP_TRY(c->box.ensure(sizeof(float) * (size_t)np * box * box));
-  HIP_TRY(c->peaks.ensure(sizeof(double) * 3 * np));
-  HIP_TRY(hipMemcpyAsync(c->pairs_d.p, slots.data(), sizeof(int) * slots.size(),
-                         hipMemcpyHostToDevice, c->st));
+  HIP_TRY(c->peaks.ensure(sizeof(double) * 3 * (size_t)np));
+  return MC_OK;
+}
+
+// Column pass, row pass and peak search over the np slot pairs in pairs_d;
+// results stay on the device (peaks, box maps).  Capturable.
+static int pair_launch(mc_context_s* c, int np, int n, float bfactor, int box, int wnoise) {
   {
     KScope ks(c, MC_KT_PAIR_COLS);
     HIP_TRY(mc::launch_pair_cols(c->spec.as<float2>(), c->spec_stride, c->pairs_d.as<int2>(), np,
@@ -489,6 +491,18 @@ static int pair_stage(mc_context_s* c, const std::vector<int>& slots, float bfac
     HIP_TRY(mc::launch_find_peaks(c->box.as<float>(), np, box, n, wnoise, c->peaks.as<double>(),
                                   c->st));
   }
+  return MC_OK;
+}
+
+// Pair stage on resident slots: pairs (a, b) are slot indices.
+static int pair_stage(mc_context_s* c, const std::vector<int>& slots, float bfactor, int box,
+                      int wnoise, std::vector<double>& res, float* boxmaps_host) {
+  const int np = (int)slots.size() / 2;
+  const int n = c->n;
+  MC_TRY(pair_reserve(c, np, n, box));
+  HIP_TRY(hipMemcpyAsync(c->pairs_d.p, slots.data(), sizeof(int) * slots.size(),
+                         hipMemcpyHostToDevice, c->st));
+  MC_TRY(pair_launch(c, np, n, bfactor, box, wnoise));
   res.resize(3 * (size_t)np);
   HIP_TRY(hipMemcpyAsync(res.data(), c->peaks.p, sizeof(double) * 3 * np, hipMemcpyDeviceToHost,
                          c->st));
@@ -499,15 +513,17 @@ static int pair_stage(mc_context_s* c, const std::vector<int>& slots, float bfac
   return MC_OK;
 }
 
-// Shift-and-sum of slots [0, nframe) with per-frame shifts into `dst` slot
-// (and optionally sum2 for frames <= f2_last), all on device.
-static int shift_sum(mc_context_s* c, const std::vector<double>& sh, float2* dst, float2* dst2,
-                     int f2_last) {
+// Shift-and-sum of slots [base, base+count) with per-frame shifts into `dst`
+// (and optionally dst2 for frames <= f2_last), all on device.  The shifts
+// are staged in the context's shift buffer; `sh` must stay alive until the
+// stream is synchronised.
+static int shift_sum(mc_context_s* c, int base, int count, const std::vector<double>& sh,
+                     float2* dst, float2* dst2, int f2_last) {
   HIP_TRY(c->shifts.ensure(sizeof(double) * sh.size()));
   HIP_TRY(hipMemcpyAsync(c->shifts.p, sh.data(), sizeof(double) * sh.size(),
                          hipMemcpyHostToDevice, c->st));
   KScope ks(c, MC_KT_SUM);
-  HIP_TRY(mc::launch_shift_sum(c->spec.as<float2>(), c->spec_stride, c->nframe, c->n,
+  HIP_TRY(mc::launch_shift_sum(slot(c, base), c->spec_stride, count, c->n,
                                c->shifts.as<double2>(), dst, dst2, f2_last, c->st));
   return MC_OK;
 }
@@ -531,19 +547,19 @@ static int sharp_weights(mc_context_s* c, int n, float bfactor) {
   return MC_OK;
 }
 
-// J(s) and dJ/ds on the resident spectra; fs: 2F shifts (NaN: frame out).
-static int sharpness_eval(mc_context_s* c, const std::vector<double>& fs, double* J,
-                          std::vector<double>& grad) {
-  const int F = c->nframe;
+// J(s) and dJ/ds of frames in slots [base, base+F), sum formed in slot
+// `sum`; fs: 2F shifts (NaN: frame out).
+static int sharpness_eval(mc_context_s* c, int base, int F, int sum, const std::vector<double>& fs,
+                          double* J, std::vector<double>& grad) {
   const int nchunks = 256;
-  MC_TRY(shift_sum(c, fs, slot(c, c->sum_slot), nullptr, -1));
+  MC_TRY(shift_sum(c, base, F, fs, slot(c, sum), nullptr, -1));
   HIP_TRY(c->gpart.ensure(sizeof(double2) * (size_t)nchunks * (F + 1)));
   HIP_TRY(c->gout.ensure(sizeof(double2) * (F + 1)));
   {
     KScope ks(c, MC_KT_SUM);
-    HIP_TRY(mc::launch_sharpness(c->spec.as<float2>(), c->spec_stride, F, c->n,
-                                 c->shifts.as<double2>(), slot(c, c->sum_slot), c->wk.as<float>(),
-                                 c->gpart.as<double2>(), nchunks, c->gout.as<double2>(), c->st));
+    HIP_TRY(mc::launch_sharpness(slot(c, base), c->spec_stride, F, c->n, c->shifts.as<double2>(),
+                                 slot(c, sum), c->wk.as<float>(), c->gpart.as<double2>(), nchunks,
+                                 c->gout.as<double2>(), c->st));
   }
   std::vector<double> o(2 * (size_t)(F + 1));
   HIP_TRY(hipMemcpyAsync(o.data(), c->gout.p, sizeof(double) * o.size(), hipMemcpyDeviceToHost, c->st));
@@ -556,12 +572,13 @@ static int sharpness_eval(mc_context_s* c, const std::vector<double>& fs, double
 // Gradient ascent of J over the summed frames' shifts with a backtracking
 // trust region: s += step * g / max|g|; a step that lowers J is rejected and
 // the step halved.
-static int refine_shifts(mc_context_s* c, std::vector<double>& fs, float bfactor, int steps,
-                         float step_px, double* j0, double* j1, int* accepted) {
+static int refine_shifts(mc_context_s* c, int base, int F, int sum, std::vector<double>& fs,
+                         float bfactor, int steps, float step_px, double* j0, double* j1,
+                         int* accepted) {
   MC_TRY(sharp_weights(c, c->n, bfactor));
   double J = 0;
   std::vector<double> g;
-  MC_TRY(sharpness_eval(c, fs, &J, g));
+  MC_TRY(sharpness_eval(c, base, F, sum, fs, &J, g));
   *j0 = J;
   double step = step_px > 0 ? step_px : 0.05;
   int acc = 0;
@@ -575,7 +592,7 @@ static int refine_shifts(mc_context_s* c, std::vector<double>& fs, float bfactor
       if (!std::isnan(fs[k])) trial[k] += step * g[k] / gmax;
     double Jt = 0;
     std::vector<double> gt;
-    MC_TRY(sharpness_eval(c, trial, &Jt, gt));
+    MC_TRY(sharpness_eval(c, base, F, sum, trial, &Jt, gt));
     if (Jt > J) {
       fs.swap(trial);
       g.swap(gt);
@@ -598,7 +615,7 @@ extern "C" int mc_sharpness(mc_context ctx, const double* frame_shift, float bfa
   MC_TRY(sharp_weights(ctx, ctx->n, bfactor));
   std::vector<double> fs(frame_shift, frame_shift + 2 * (size_t)ctx->nframe), g;
   double j = 0;
-  MC_TRY(sharpness_eval(ctx, fs, &j, g));
+  MC_TRY(sharpness_eval(ctx, 0, ctx->nframe, ctx->sum_slot, fs, &j, g));
   if (J) *J = j;
   if (grad) memcpy(grad, g.data(), sizeof(double) * g.size());
   return MC_OK;
@@ -721,7 +738,7 @@ extern "C" int mc_sum_image(mc_context ctx, const double* frame_shift, float* im
   HIP_TRY(hipSetDevice(ctx->device));
   const int n = ctx->n;
   std::vector<double> sh(frame_shift, frame_shift + 2 * (size_t)ctx->nframe);
-  MC_TRY(shift_sum(ctx, sh, slot(ctx, ctx->sum_slot), nullptr, -1));
+  MC_TRY(shift_sum(ctx, 0, ctx->nframe, sh, slot(ctx, ctx->sum_slot), nullptr, -1));
   HIP_TRY(ctx->image.ensure(sizeof(float) * (size_t)n * n));
   MC_TRY(to_image(ctx, slot(ctx, ctx->sum_slot), ctx->image.as<float>()));
   HIP_TRY(hipMemcpyAsync(image, ctx->image.p, sizeof(float) * (size_t)n * n,
@@ -736,146 +753,132 @@ static float elapsed(hipEvent_t a, hipEvent_t b) {
   return ms;
 }
 
-extern "C" int mc_align_movie(mc_context ctx, const mc_movie_desc* m, const mc_params* p,
-                              mc_outputs* out, mc_report* rep) {
-  if (!ctx || !m || !p) return set_err(MC_ERR_ARG, "NULL argument");
-  HIP_TRY(hipSetDevice(ctx->device));
-  mc_outputs none;
-  memset(&none, 0, sizeof none);
-  if (!out) out = &none;
-  hipEvent_t ev[8];
-  for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
-  struct EvGuard {
-    hipEvent_t* e;
-    ~EvGuard() {
-      for (int i = 0; i < 8; i++) (void)hipEventDestroy(e[i]);
-    }
-  } guard{ev};
-  HIP_TRY(hipEventRecord(ev[0], ctx->st));
+// ====================================================== whole movies
+// One movie of a call to mc_align_movie / mc_align_batch: its geometry, its
+// slot range in the shared spectrum array and the host-side drift state.
+struct MovieRun {
+  const mc_movie_desc* m = nullptr;
+  const mc_params* p = nullptr;
+  mc_outputs* out = nullptr;
+  mc_report* rep = nullptr;
+  Geom g{};
+  int F = 0, P = 0, K = 0, box = 0, wnoise = 0;
+  int base = 0, off2 = 0, sum = 0, nslot = 0;  // slots [base, base+nslot)
+  std::vector<int> pairs;
+  std::vector<double> b, s, oldx, newx, fs, sh, sh2;
+  int ng = 0, it = 0, converged = 0, rawsum_written = 0, sum2_written = 0, gacc = 0;
+  double rmsd = 0, sj0 = 0, sj1 = 0;
+  bool iterating() const { return it < p->iter && !converged; }
+};
 
-  // 1. spectra (DFAlign.cu:705-824)
-  Geom g;
-  int F = 0;
-  MC_TRY(load_frames(ctx, m, p, &g, &F));
-  const int n = g.n;
-  const int box = p->CCPeakSearchDim;
-  if (box < 16 || box > n || box > 256)
-    return set_err(MC_ERR_ARG, "peak box %d must be in [16, min(n, 256)]", box);
-  const int wnoise = p->NoisePeakSize - 1;  // DFAlign.cu:900
-  HIP_TRY(ctx->means.ensure(sizeof(double) * F));
-  HIP_TRY(mc::launch_dc_gather(slot(ctx, 0), ctx->spec_stride, F, 1.0 / ((double)n * n),
-                               ctx->means.as<double>(), ctx->st));
-  if (out->frame_mean)
-    HIP_TRY(hipMemcpyAsync(out->frame_mean, ctx->means.p, sizeof(double) * F,
-                           hipMemcpyDeviceToHost, ctx->st));
-  HIP_TRY(ctx->image.ensure(sizeof(float) * (size_t)n * n));
-  int rawsum_written = 0;
-  if (out->rawsum_image) {
-    // DFAlign.cu:808-823,853-857: unaligned sum of frames [nStartSum, nEndSum]
-    std::vector<double> sh(2 * (size_t)F, NAN);
-    for (int f = 0; f < F; f++) {
-      const int j = f + p->nStart;
-      if (j >= p->nStartSum && j <= p->nEndSum) sh[2 * f] = sh[2 * f + 1] = 0.0;
-    }
-    MC_TRY(shift_sum(ctx, sh, slot(ctx, ctx->sum_slot), nullptr, -1));
-    MC_TRY(to_image(ctx, slot(ctx, ctx->sum_slot), ctx->image.as<float>()));
-    HIP_TRY(hipMemcpyAsync(out->rawsum_image, ctx->image.p, sizeof(float) * (size_t)n * n,
-                           hipMemcpyDeviceToHost, ctx->st));
-    rawsum_written = 1;
-  }
-  HIP_TRY(hipEventRecord(ev[1], ctx->st));
+static int prepare_run(MovieRun& r) {
+  const mc_movie_desc* m = r.m;
+  const mc_params* p = r.p;
+  MC_TRY(movie_geometry(m, p, &r.g));
+  r.F = p->nEnd - p->nStart + 1;
+  if (p->nStart < m->first_frame || p->nEnd >= m->first_frame + m->nz || r.F < 2)
+    return set_err(MC_ERR_ARG, "frame range [%d, %d] invalid for frames [%d, %d)", p->nStart,
+                   p->nEnd, m->first_frame, m->first_frame + m->nz);
+  if (!m->frames) return set_err(MC_ERR_ARG, "frames is NULL");
+  r.box = p->CCPeakSearchDim;
+  if (r.box < 16 || r.box > r.g.n || r.box > 256)
+    return set_err(MC_ERR_ARG, "peak box %d must be in [16, min(n, 256)]", r.box);
+  r.wnoise = p->NoisePeakSize - 1;  // DFAlign.cu:900
+  r.K = r.F - 1;
+  r.P = mc_pair_list(r.F, p->FrameDistOffset, nullptr, 0);
+  r.pairs.assign(2 * (size_t)r.P, 0);
+  mc_pair_list(r.F, p->FrameDistOffset, r.pairs.data(), r.P);
+  r.nslot = r.F * (m->frames2 ? 2 : 1) + 1;
+  return MC_OK;
+}
 
-  // 2. pairwise shifts (DFAlign.cu:875-905)
-  const int P = mc_pair_list(F, p->FrameDistOffset, nullptr, 0);
-  const int K = F - 1;
-  std::vector<int> pairs(2 * (size_t)P);
-  mc_pair_list(F, p->FrameDistOffset, pairs.data(), P);
-  std::vector<int> slots(pairs);
-  for (int q = 0; q < P; q++) slots[2 * q + 1] += ctx->off2;
-  std::vector<double> res;
-  MC_TRY(pair_stage(ctx, slots, p->bfactor, box, wnoise, res, out->boxmaps));
-  HIP_TRY(hipEventRecord(ev[2], ctx->st));
-  std::vector<double> b(2 * (size_t)P);
+// Least squares, outlier rejection, least squares (DFAlign.cu:914-948) from
+// the pair peaks res (3 doubles per pair), then the start of the
+// align-to-sum recursion.
+static int solve_drift(MovieRun& r, const double* res) {
+  mc_outputs* out = r.out;
+  const int P = r.P, K = r.K, F = r.F;
+  r.b.assign(2 * (size_t)P, 0.0);
   for (int q = 0; q < P; q++) {
-    b[2 * q] = res[3 * q];
-    b[2 * q + 1] = res[3 * q + 1];
+    r.b[2 * q] = res[3 * q];
+    r.b[2 * q + 1] = res[3 * q + 1];
     if (out->pair_cc) out->pair_cc[q] = res[3 * q + 2];
   }
-  if (out->pairs) memcpy(out->pairs, pairs.data(), sizeof(int) * 2 * P);
-  if (out->pair_b) memcpy(out->pair_b, b.data(), sizeof(double) * 2 * P);
-
-  // 3. least squares, outlier rejection, least squares (DFAlign.cu:914-948)
-  std::vector<double> s(2 * (size_t)K), ki(P);
-  MC_TRY(mc_lsq(pairs.data(), P, F, b.data(), s.data()));
-  residuals(pairs.data(), P, s.data(), b.data(), ki.data());
-  if (out->shift1) memcpy(out->shift1, s.data(), sizeof(double) * 2 * K);
+  if (out->pairs) memcpy(out->pairs, r.pairs.data(), sizeof(int) * 2 * P);
+  if (out->pair_b) memcpy(out->pair_b, r.b.data(), sizeof(double) * 2 * P);
+  std::vector<double> ki(P);
+  r.s.assign(2 * (size_t)K, 0.0);
+  MC_TRY(mc_lsq(r.pairs.data(), P, F, r.b.data(), r.s.data()));
+  residuals(r.pairs.data(), P, r.s.data(), r.b.data(), ki.data());
+  if (out->shift1) memcpy(out->shift1, r.s.data(), sizeof(double) * 2 * K);
   if (out->ki1) memcpy(out->ki1, ki.data(), sizeof(double) * P);
   std::vector<int> good(P);
-  const int ng = mc_threshold(pairs.data(), P, F, ki.data(), p->kiThresh, good.data());
-  std::vector<int> gp(2 * (size_t)ng);
-  std::vector<double> gb(2 * (size_t)ng);
-  for (int q = 0; q < ng; q++) {
-    gp[2 * q] = pairs[2 * good[q]];
-    gp[2 * q + 1] = pairs[2 * good[q] + 1];
-    gb[2 * q] = b[2 * good[q]];
-    gb[2 * q + 1] = b[2 * good[q] + 1];
+  r.ng = mc_threshold(r.pairs.data(), P, F, ki.data(), r.p->kiThresh, good.data());
+  std::vector<int> gp(2 * (size_t)r.ng);
+  std::vector<double> gb(2 * (size_t)r.ng);
+  for (int q = 0; q < r.ng; q++) {
+    gp[2 * q] = r.pairs[2 * good[q]];
+    gp[2 * q + 1] = r.pairs[2 * good[q] + 1];
+    gb[2 * q] = r.b[2 * good[q]];
+    gb[2 * q + 1] = r.b[2 * good[q] + 1];
   }
-  MC_TRY(mc_lsq(gp.data(), ng, F, gb.data(), s.data()));
-  if (out->good) memcpy(out->good, good.data(), sizeof(int) * ng);
-  if (out->shift2) memcpy(out->shift2, s.data(), sizeof(double) * 2 * K);
-  if (out->ki2) residuals(gp.data(), ng, s.data(), gb.data(), out->ki2);
-  HIP_TRY(hipEventRecord(ev[3], ctx->st));
-
-  // 4. iterative alignment to the sum (DFAlign.cu:950-1071)
-  std::vector<double> oldx(2 * (size_t)F, 0.0), newx(2 * (size_t)F, 0.0);
-  {
-    double ax = 0, ay = 0;
-    for (int k = 0; k < K; k++) {
-      newx[2 * (k + 1)] = ax + s[2 * k];
-      newx[2 * (k + 1) + 1] = ay + s[2 * k + 1];
-      ax += s[2 * k];
-      ay += s[2 * k + 1];
-    }
-  }
-  oldx = newx;
-  double rmsd = 0;
-  int it = 0, converged = 0;
-  std::vector<int> islots(2 * (size_t)F);
-  for (int k = 0; k < F; k++) {
-    islots[2 * k] = ctx->sum_slot;  // GPUShiftCC(dfft=frame, dsum=sum): sum * conj(frame)
-    islots[2 * k + 1] = k;
-  }
-  for (it = 0; it < p->iter && !converged; it++) {
-    std::vector<double> sh(2 * (size_t)F);
-    for (int k = 0; k < 2 * F; k++) sh[k] = -oldx[k];
-    MC_TRY(shift_sum(ctx, sh, slot(ctx, ctx->sum_slot), nullptr, -1));
-    MC_TRY(pair_stage(ctx, islots, p->bfactor, box, wnoise, res,
-                      out->iter_boxmaps ? out->iter_boxmaps + (size_t)it * F * box * box : nullptr));
-    for (int k = 0; k < F; k++) {
-      newx[2 * k] = res[3 * k];
-      newx[2 * k + 1] = res[3 * k + 1];
-      if (out->iter_cc) out->iter_cc[(size_t)it * F + k] = res[3 * k + 2];
-    }
-    if (out->iter_meas) memcpy(out->iter_meas + (size_t)it * 2 * F, newx.data(), sizeof(double) * 2 * F);
-    const double rx = newx[0], ry = newx[1];
-    for (int k = 0; k < F; k++) {
-      const double dx = oldx[2 * k] - newx[2 * k], dy = oldx[2 * k + 1] - newx[2 * k + 1];
-      oldx[2 * k] = newx[2 * k] - rx;
-      oldx[2 * k + 1] = newx[2 * k + 1] - ry;
-      rmsd += dx * dx + dy * dy;
-    }
-    rmsd /= F;  // accumulates across rounds as the reference does (DFAlign.cu:1036-1041)
-    if (out->rmsd) out->rmsd[it] = rmsd;
-    if (rmsd < (double)p->iterthres * (double)p->iterthres) converged = 1;
-  }
+  MC_TRY(mc_lsq(gp.data(), r.ng, F, gb.data(), r.s.data()));
+  if (out->good) memcpy(out->good, good.data(), sizeof(int) * r.ng);
+  if (out->shift2) memcpy(out->shift2, r.s.data(), sizeof(double) * 2 * K);
+  if (out->ki2) residuals(gp.data(), r.ng, r.s.data(), gb.data(), out->ki2);
+  // positions of the frames from the increments (DFAlign.cu:955-966)
+  r.newx.assign(2 * (size_t)F, 0.0);
+  double ax = 0, ay = 0;
   for (int k = 0; k < K; k++) {
-    s[2 * k] = oldx[2 * (k + 1)] - oldx[2 * k];
-    s[2 * k + 1] = oldx[2 * (k + 1) + 1] - oldx[2 * k + 1];
+    ax += r.s[2 * k];
+    ay += r.s[2 * k + 1];
+    r.newx[2 * (k + 1)] = ax;
+    r.newx[2 * (k + 1) + 1] = ay;
   }
-  if (out->final_shift) memcpy(out->final_shift, s.data(), sizeof(double) * 2 * K);
-  HIP_TRY(hipEventRecord(ev[4], ctx->st));
+  r.oldx = r.newx;
+  return MC_OK;
+}
 
-  // 5. final phase-shift-and-sum (DFAlign.cu:1114-1292)
+// Shifts of one align-to-sum round: the sum of every frame moved by -oldx
+// (DFAlign.cu:974-989).
+static void iter_shifts(MovieRun& r) {
+  r.sh.assign(2 * (size_t)r.F, 0.0);
+  for (int k = 0; k < 2 * r.F; k++) r.sh[k] = -r.oldx[k];
+}
+
+// One align-to-sum round from the peaks of (sum, frame k) (DFAlign.cu:990-1071).
+static void iter_update(MovieRun& r, const double* res) {
+  mc_outputs* out = r.out;
+  const int F = r.F, it = r.it;
+  for (int k = 0; k < F; k++) {
+    r.newx[2 * k] = res[3 * k];
+    r.newx[2 * k + 1] = res[3 * k + 1];
+    if (out->iter_cc) out->iter_cc[(size_t)it * F + k] = res[3 * k + 2];
+  }
+  if (out->iter_meas) memcpy(out->iter_meas + (size_t)it * 2 * F, r.newx.data(), sizeof(double) * 2 * F);
+  const double rx = r.newx[0], ry = r.newx[1];
+  for (int k = 0; k < F; k++) {
+    const double dx = r.oldx[2 * k] - r.newx[2 * k], dy = r.oldx[2 * k + 1] - r.newx[2 * k + 1];
+    r.oldx[2 * k] = r.newx[2 * k] - rx;
+    r.oldx[2 * k + 1] = r.newx[2 * k + 1] - ry;
+    r.rmsd += dx * dx + dy * dy;
+  }
+  r.rmsd /= F;  // accumulates across rounds as the reference does (DFAlign.cu:1036-1041)
+  if (out->rmsd) out->rmsd[it] = r.rmsd;
+  if (r.rmsd < (double)r.p->iterthres * (double)r.p->iterthres) r.converged = 1;
+  r.it++;
+}
+
+// Increments after the iteration and the shift of every summed frame
+// (DFAlign.cu:1114-1148); frames outside [nss, nes] get NaN (not summed).
+static void final_shifts(MovieRun& r, int* nes2_out) {
+  const mc_params* p = r.p;
+  const int F = r.F, K = r.K;
+  for (int k = 0; k < K; k++) {
+    r.s[2 * k] = r.oldx[2 * (k + 1)] - r.oldx[2 * k];
+    r.s[2 * k + 1] = r.oldx[2 * (k + 1) + 1] - r.oldx[2 * k + 1];
+  }
+  if (r.out->final_shift) memcpy(r.out->final_shift, r.s.data(), sizeof(double) * 2 * K);
   int nss = p->nStartSum - p->nStart, nes = p->nEndSum - p->nStart, nes2 = p->nEndSum2 - p->nStart;
   if (nss <= 0) nss = 0;
   if (nes <= nss || nes >= F) nes = F - 1;
@@ -883,86 +886,434 @@ extern "C" int mc_align_movie(mc_context ctx, const mc_movie_desc* m, const mc_p
   if (nes2 > nes) nes2 = nes;
   double midx = 0, midy = 0;
   if (p->bAlignToMid) {
-    const int nzf = m->nz_file > 0 ? m->nz_file : m->nz;
+    const int nzf = r.m->nz_file > 0 ? r.m->nz_file : r.m->nz;
     int ref = nzf / 2 + 1;
     if (ref < p->nStart) ref = p->nStart;
     const int lim = std::min(ref - p->nStart, K);  // guards DFAlign.cu:1138
     for (int j = 0; j < lim; j++) {
-      midx += s[2 * j];
-      midy += s[2 * j + 1];
+      midx += r.s[2 * j];
+      midy += r.s[2 * j + 1];
     }
   }
-  std::vector<double> fs(2 * (size_t)F, NAN);
-  {
-    double tx = 0, ty = 0;
-    for (int j = 1; j < nss + 1; j++) {
-      tx += s[2 * (j - 1)];
-      ty += s[2 * (j - 1) + 1];
-    }
-    // first summed frame: shifted only with -atm 1 (DFAlign.cu:1148)
-    fs[2 * nss] = p->bAlignToMid ? -tx + midx : 0.0;
-    fs[2 * nss + 1] = p->bAlignToMid ? -ty + midy : 0.0;
-    for (int j = nss + 1; j <= nes; j++) {
-      tx += s[2 * (j - 1)];
-      ty += s[2 * (j - 1) + 1];
-      fs[2 * j] = -tx + midx;
-      fs[2 * j + 1] = -ty + midy;
-    }
+  r.fs.assign(2 * (size_t)F, NAN);
+  double tx = 0, ty = 0;
+  for (int j = 1; j < nss + 1; j++) {
+    tx += r.s[2 * (j - 1)];
+    ty += r.s[2 * (j - 1) + 1];
   }
-  if (out->frame_shift) memcpy(out->frame_shift, fs.data(), sizeof(double) * 2 * F);
-  double sj0 = 0, sj1 = 0;
-  int gacc = 0;
+  // first summed frame: shifted only with -atm 1 (DFAlign.cu:1148)
+  r.fs[2 * nss] = p->bAlignToMid ? -tx + midx : 0.0;
+  r.fs[2 * nss + 1] = p->bAlignToMid ? -ty + midy : 0.0;
+  for (int j = nss + 1; j <= nes; j++) {
+    tx += r.s[2 * (j - 1)];
+    ty += r.s[2 * (j - 1) + 1];
+    r.fs[2 * j] = -tx + midx;
+    r.fs[2 * j + 1] = -ty + midy;
+  }
+  *nes2_out = nes2;
+}
+
+// Final phase-shift-and-sum and images of one movie (DFAlign.cu:1143-1292).
+static int final_sum(mc_context_s* c, MovieRun& r) {
+  mc_outputs* out = r.out;
+  const mc_params* p = r.p;
+  const int n = c->n;
+  int nes2 = 0;
+  final_shifts(r, &nes2);
+  if (out->frame_shift) memcpy(out->frame_shift, r.fs.data(), sizeof(double) * 2 * r.F);
   if (p->grad_steps > 0) {
     // adjoint sharpness refinement of the summed frames' shifts
-    MC_TRY(refine_shifts(ctx, fs, p->bfactor, p->grad_steps, p->grad_step_px, &sj0, &sj1, &gacc));
-    if (out->frame_shift) memcpy(out->frame_shift, fs.data(), sizeof(double) * 2 * F);
+    MC_TRY(refine_shifts(c, r.base, r.F, r.sum, r.fs, p->bfactor, p->grad_steps, p->grad_step_px,
+                         &r.sj0, &r.sj1, &r.gacc));
+    if (out->frame_shift) memcpy(out->frame_shift, r.fs.data(), sizeof(double) * 2 * r.F);
   }
   const bool want2 = out->sum2_image && p->nEndSum2 > 0 && p->nEndSum2 < p->nEndSum;
   float2* s2 = nullptr;
   if (want2) {
-    HIP_TRY(ctx->tmp.ensure(sizeof(float2) * ctx->spec_stride * 2));
-    s2 = ctx->tmp.as<float2>() + ctx->spec_stride;  // second half of tmp
+    HIP_TRY(c->tmp.ensure(sizeof(float2) * c->spec_stride * 2));
+    s2 = c->tmp.as<float2>() + c->spec_stride;  // second half of tmp
   }
-  MC_TRY(shift_sum(ctx, fs, slot(ctx, ctx->sum_slot), s2, nes2));
-  float* img = out->sum_image_dev ? static_cast<float*>(out->sum_image_dev) : ctx->image.as<float>();
-  MC_TRY(to_image(ctx, slot(ctx, ctx->sum_slot), img));
+  MC_TRY(shift_sum(c, r.base, r.F, r.fs, slot(c, r.sum), s2, nes2));
+  HIP_TRY(c->image.ensure(sizeof(float) * (size_t)n * n));
+  float* img = out->sum_image_dev ? static_cast<float*>(out->sum_image_dev) : c->image.as<float>();
+  MC_TRY(to_image(c, slot(c, r.sum), img));
   if (out->sum_image)
     HIP_TRY(hipMemcpyAsync(out->sum_image, img, sizeof(float) * (size_t)n * n,
-                           hipMemcpyDeviceToHost, ctx->st));
-  int sum2_written = 0;
+                           hipMemcpyDeviceToHost, c->st));
   if (want2) {
     // the second sum shares tmp: copy it into a spectrum slot first
-    HIP_TRY(hipMemcpyAsync(slot(ctx, ctx->sum_slot), s2, sizeof(float2) * ctx->spec_stride,
-                           hipMemcpyDeviceToDevice, ctx->st));
-    HIP_TRY(ctx->sum2img.ensure(sizeof(float) * (size_t)n * n));
-    MC_TRY(to_image(ctx, slot(ctx, ctx->sum_slot), ctx->sum2img.as<float>()));
-    HIP_TRY(hipMemcpyAsync(out->sum2_image, ctx->sum2img.p, sizeof(float) * (size_t)n * n,
-                           hipMemcpyDeviceToHost, ctx->st));
-    sum2_written = 1;
+    HIP_TRY(hipMemcpyAsync(slot(c, r.sum), s2, sizeof(float2) * c->spec_stride,
+                           hipMemcpyDeviceToDevice, c->st));
+    HIP_TRY(c->sum2img.ensure(sizeof(float) * (size_t)n * n));
+    MC_TRY(to_image(c, slot(c, r.sum), c->sum2img.as<float>()));
+    HIP_TRY(hipMemcpyAsync(out->sum2_image, c->sum2img.p, sizeof(float) * (size_t)n * n,
+                           hipMemcpyDeviceToHost, c->st));
+    r.sum2_written = 1;
   }
-  HIP_TRY(hipEventRecord(ev[5], ctx->st));
+  // the host copies above read c->image / sum2img before the next movie's
+  // launches on the same stream overwrite them
+  HIP_TRY(hipStreamSynchronize(c->st));
+  return MC_OK;
+}
+
+// Graph of one align-to-sum round of a set of movies: the batched sums, the
+// pair column/row passes and the peak search, replayed while the set of
+// iterating movies is unchanged.  Only kernel launches are captured: the
+// shifts are uploaded and the peaks read back around each replay.
+struct RoundGraph {
+  hipGraphExec_t exec = nullptr;
+  std::vector<int> key;
+  ~RoundGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+  }
+};
+
+// Align-to-sum rounds run as graph replays unless MC_NO_GRAPH=1 (eager
+// launches of the same kernels; read at every call).
+static bool graphs_enabled() {
+  const char* e = getenv("MC_NO_GRAPH");
+  return !(e && atoi(e));
+}
+
+static int round_launch(mc_context_s* c, int njobs, int np, int n, float bfactor, int box,
+                        int wnoise) {
+  {
+    KScope ks(c, MC_KT_SUM);
+    HIP_TRY(mc::launch_shift_sum_multi(c->spec.as<float2>(), c->spec_stride, n,
+                                       c->jobs.as<int4>(), njobs, c->shifts.as<double2>(), c->st));
+  }
+  return pair_launch(c, np, n, bfactor, box, wnoise);
+}
+
+static int round_graph(mc_context_s* c, RoundGraph& gr, const std::vector<int>& key, int njobs,
+                       int np, int n, float bfactor, int box, int wnoise) {
+  if (gr.exec && gr.key == key) return MC_OK;
+  if (gr.exec) {
+    (void)hipGraphExecDestroy(gr.exec);
+    gr.exec = nullptr;
+  }
+  const bool kt = c->kt_on;
+  c->kt_on = false;  // no event records inside the graph
+  HIP_TRY(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+  const int rc = round_launch(c, njobs, np, n, bfactor, box, wnoise);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(c->st, &graph);
+  c->kt_on = kt;
+  if (rc != MC_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  HIP_TRY(ec);
+  const hipError_t ei = hipGraphInstantiate(&gr.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HIP_TRY(ei);
+  gr.key = key;
+  c->graph_captures++;
+  return MC_OK;
+}
+
+struct GroupTimes {
+  float spectra = 0, pairs = 0, solve = 0, iter = 0, sum = 0, total = 0;
+};
+
+// Aligns movies of one transform size, peak box, B-factor and noise window
+// whose spectra fit together in device memory: stage 1 per movie, then one
+// pair-stage launch over every pair of every movie, the host drift model per
+// movie, align-to-sum rounds batched over the movies still iterating (one
+// graph replay per round), and the final sum per movie.
+static int align_group(mc_context_s* c, std::vector<MovieRun*>& rs, bool use_graph,
+                       GroupTimes* tm) {
+  const int n = rs[0]->g.n;
+  const int box = rs[0]->box, wnoise = rs[0]->wnoise;
+  const float bfactor = rs[0]->p->bfactor;
+  const size_t stride = (size_t)(n / 2 + 1) * n;
+  int nslots = 0, ftot = 0;
+  for (MovieRun* r : rs) {
+    r->base = nslots;
+    r->off2 = r->m->frames2 ? r->F : 0;
+    r->sum = r->base + r->nslot - 1;
+    nslots += r->nslot;
+    ftot += r->F;
+  }
+  HIP_TRY(c->spec.ensure(sizeof(float2) * stride * nslots));
+  c->n = n;
+  c->spec_stride = stride;
+  c->nframe = 0;  // resident single-movie state is rebuilt below
+  hipEvent_t ev[6];
+  for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+  struct EvGuard {
+    hipEvent_t* e;
+    ~EvGuard() {
+      for (int i = 0; i < 6; i++) (void)hipEventDestroy(e[i]);
+    }
+  } guard{ev};
+  HIP_TRY(hipEventRecord(ev[0], c->st));
+
+  // 1. spectra (DFAlign.cu:705-824), frame means and unaligned sums
+  HIP_TRY(c->means.ensure(sizeof(double) * ftot));
+  {
+    KScope ks(c, MC_KT_SPECTRA);
+    for (MovieRun* r : rs) {
+      const int first = r->p->nStart - r->m->first_frame;
+      MC_TRY(run_stage1(c, r->m->frames, r->m, r->p, r->g, first, r->F, r->base));
+      if (r->m->frames2)
+        MC_TRY(run_stage1(c, r->m->frames2, r->m, r->p, r->g, first, r->F, r->base + r->off2));
+    }
+  }
+  {
+    int mo = 0;
+    for (MovieRun* r : rs) {
+      double* md = c->means.as<double>() + mo;
+      HIP_TRY(mc::launch_dc_gather(slot(c, r->base), stride, r->F, 1.0 / ((double)n * n), md, c->st));
+      if (r->out->frame_mean)
+        HIP_TRY(hipMemcpyAsync(r->out->frame_mean, md, sizeof(double) * r->F, hipMemcpyDeviceToHost,
+                               c->st));
+      mo += r->F;
+    }
+  }
+  HIP_TRY(c->image.ensure(sizeof(float) * (size_t)n * n));
+  for (MovieRun* r : rs) {
+    if (!r->out->rawsum_image) continue;
+    // DFAlign.cu:808-823,853-857: unaligned sum of frames [nStartSum, nEndSum]
+    r->sh.assign(2 * (size_t)r->F, NAN);
+    for (int f = 0; f < r->F; f++) {
+      const int j = f + r->p->nStart;
+      if (j >= r->p->nStartSum && j <= r->p->nEndSum) r->sh[2 * f] = r->sh[2 * f + 1] = 0.0;
+    }
+    MC_TRY(shift_sum(c, r->base, r->F, r->sh, slot(c, r->sum), nullptr, -1));
+    MC_TRY(to_image(c, slot(c, r->sum), c->image.as<float>()));
+    HIP_TRY(hipMemcpyAsync(r->out->rawsum_image, c->image.p, sizeof(float) * (size_t)n * n,
+                           hipMemcpyDeviceToHost, c->st));
+    HIP_TRY(hipStreamSynchronize(c->st));
+    r->rawsum_written = 1;
+  }
+  HIP_TRY(hipEventRecord(ev[1], c->st));
+
+  // 2. pairwise shifts of every movie in one launch (DFAlign.cu:875-905)
+  std::vector<int> slots;
+  for (MovieRun* r : rs)
+    for (int q = 0; q < r->P; q++) {
+      slots.push_back(r->base + r->pairs[2 * q]);
+      slots.push_back(r->base + r->off2 + r->pairs[2 * q + 1]);
+    }
+  std::vector<double> res;
+  MC_TRY(pair_stage(c, slots, bfactor, box, wnoise, res, nullptr));
+  HIP_TRY(hipEventRecord(ev[2], c->st));
+  {
+    size_t po = 0;
+    for (MovieRun* r : rs) {
+      if (r->out->boxmaps)
+        HIP_TRY(hipMemcpyAsync(r->out->boxmaps, c->box.as<float>() + po * box * box,
+                               sizeof(float) * (size_t)r->P * box * box, hipMemcpyDeviceToHost, c->st));
+      // 3. least squares, outlier rejection, least squares (DFAlign.cu:914-948)
+      MC_TRY(solve_drift(*r, res.data() + 3 * po));
+      po += r->P;
+    }
+    HIP_TRY(hipStreamSynchronize(c->st));
+  }
+  HIP_TRY(hipEventRecord(ev[3], c->st));
+
+  // 4. iterative alignment to the sum (DFAlign.cu:950-1071), batched
+  RoundGraph gr;
+  for (;;) {
+    std::vector<MovieRun*> act;
+    for (MovieRun* r : rs)
+      if (r->iterating()) act.push_back(r);
+    if (act.empty()) break;
+    std::vector<int> key, jobs, islots;
+    std::vector<double> shifts;
+    for (MovieRun* r : act) {
+      key.push_back(r->base);
+      iter_shifts(*r);
+      jobs.insert(jobs.end(), {r->base, r->F, (int)(shifts.size() / 2), r->sum});
+      shifts.insert(shifts.end(), r->sh.begin(), r->sh.end());
+      for (int k = 0; k < r->F; k++) {
+        islots.push_back(r->sum);  // GPUShiftCC(dfft=frame, dsum=sum): sum * conj(frame)
+        islots.push_back(r->base + k);
+      }
+    }
+    const int njobs = (int)act.size(), np = (int)islots.size() / 2;
+    const bool fresh = !(gr.exec && gr.key == key);
+    if (fresh) {
+      MC_TRY(pair_reserve(c, np, n, box));
+      HIP_TRY(c->jobs.ensure(sizeof(int) * jobs.size()));
+      HIP_TRY(hipMemcpyAsync(c->jobs.p, jobs.data(), sizeof(int) * jobs.size(),
+                             hipMemcpyHostToDevice, c->st));
+      HIP_TRY(hipMemcpyAsync(c->pairs_d.p, islots.data(), sizeof(int) * islots.size(),
+                             hipMemcpyHostToDevice, c->st));
+      HIP_TRY(c->shifts.ensure(sizeof(double) * shifts.size()));
+    }
+    HIP_TRY(hipMemcpyAsync(c->shifts.p, shifts.data(), sizeof(double) * shifts.size(),
+                           hipMemcpyHostToDevice, c->st));
+    if (use_graph) {
+      MC_TRY(round_graph(c, gr, key, njobs, np, n, bfactor, box, wnoise));
+      KScope ks(c, MC_KT_ROUNDS);
+      HIP_TRY(hipGraphLaunch(gr.exec, c->st));
+      c->graph_replays++;
+    } else {
+      gr.key = key;  // buffers uploaded for this set
+      MC_TRY(round_launch(c, njobs, np, n, bfactor, box, wnoise));
+    }
+    res.resize(3 * (size_t)np);
+    HIP_TRY(hipMemcpyAsync(res.data(), c->peaks.p, sizeof(double) * 3 * np, hipMemcpyDeviceToHost,
+                           c->st));
+    size_t po = 0;
+    for (MovieRun* r : act) {
+      if (r->out->iter_boxmaps)
+        HIP_TRY(hipMemcpyAsync(r->out->iter_boxmaps + (size_t)r->it * r->F * box * box,
+                               c->box.as<float>() + po * box * box,
+                               sizeof(float) * (size_t)r->F * box * box, hipMemcpyDeviceToHost, c->st));
+      po += r->F;
+    }
+    HIP_TRY(hipStreamSynchronize(c->st));
+    po = 0;
+    for (MovieRun* r : act) {
+      iter_update(*r, res.data() + 3 * po);
+      po += r->F;
+    }
+  }
+  HIP_TRY(hipEventRecord(ev[4], c->st));
+
+  // 5. final phase-shift-and-sum per movie (DFAlign.cu:1114-1292)
+  for (MovieRun* r : rs) MC_TRY(final_sum(c, *r));
+  HIP_TRY(hipEventRecord(ev[5], c->st));
   HIP_TRY(hipEventSynchronize(ev[5]));
-  if (rep) {
-    memset(rep, 0, sizeof *rep);
-    rep->n = n;
-    rep->nsamUnbin = g.nsu;
-    rep->nframe = F;
-    rep->npairs = P;
-    rep->ngood = ng;
-    rep->niter = it;
-    rep->converged = converged;
-    rep->rmsd = rmsd;
-    rep->sum2_written = sum2_written;
-    rep->rawsum_written = rawsum_written;
-    rep->sharp_j0 = sj0;
-    rep->sharp_j1 = sj1;
-    rep->grad_accepted = gacc;
-    rep->ms_spectra = elapsed(ev[0], ev[1]);
-    rep->ms_pairs = elapsed(ev[1], ev[2]);
-    rep->ms_solve = elapsed(ev[2], ev[3]);
-    rep->ms_iter = elapsed(ev[3], ev[4]);
-    rep->ms_sum = elapsed(ev[4], ev[5]);
-    rep->ms_total = elapsed(ev[0], ev[5]);
+  tm->spectra = elapsed(ev[0], ev[1]);
+  tm->pairs = elapsed(ev[1], ev[2]);
+  tm->solve = elapsed(ev[2], ev[3]);
+  tm->iter = elapsed(ev[3], ev[4]);
+  tm->sum = elapsed(ev[4], ev[5]);
+  tm->total = elapsed(ev[0], ev[5]);
+  if (rs.size() == 1) {
+    // the movie stays resident for the stage entry points
+    c->nframe = rs[0]->F;
+    c->nslots = rs[0]->nslot;
+    c->off2 = rs[0]->off2;
+    c->sum_slot = rs[0]->sum;
+  }
+  return MC_OK;
+}
+
+static void fill_report(const MovieRun& r, const GroupTimes& tm) {
+  mc_report* rep = r.rep;
+  if (!rep) return;
+  memset(rep, 0, sizeof *rep);
+  rep->n = r.g.n;
+  rep->nsamUnbin = r.g.nsu;
+  rep->nframe = r.F;
+  rep->npairs = r.P;
+  rep->ngood = r.ng;
+  rep->niter = r.it;
+  rep->converged = r.converged;
+  rep->rmsd = r.rmsd;
+  rep->sum2_written = r.sum2_written;
+  rep->rawsum_written = r.rawsum_written;
+  rep->sharp_j0 = r.sj0;
+  rep->sharp_j1 = r.sj1;
+  rep->grad_accepted = r.gacc;
+  rep->ms_spectra = tm.spectra;
+  rep->ms_pairs = tm.pairs;
+  rep->ms_solve = tm.solve;
+  rep->ms_iter = tm.iter;
+  rep->ms_sum = tm.sum;
+  rep->ms_total = tm.total;
+}
+
+extern "C" int mc_align_movie(mc_context ctx, const mc_movie_desc* m, const mc_params* p,
+                              mc_outputs* out, mc_report* rep) {
+  if (!ctx || !m || !p) return set_err(MC_ERR_ARG, "NULL argument");
+  HIP_TRY(hipSetDevice(ctx->device));
+  mc_outputs none;
+  memset(&none, 0, sizeof none);
+  MovieRun r;
+  r.m = m;
+  r.p = p;
+  r.out = out ? out : &none;
+  r.rep = rep;
+  MC_TRY(prepare_run(r));
+  std::vector<MovieRun*> rs{&r};
+  GroupTimes tm;
+  MC_TRY(align_group(ctx, rs, graphs_enabled(), &tm));
+  fill_report(r, tm);
+  return MC_OK;
+}
+
+// Device memory the batch may hold for resident spectra: MC_BATCH_MEM_MB, or
+// 60% of the free memory at the call.
+static size_t batch_budget(mc_context_s* c) {
+  if (getenv("MC_BATCH_MEM_MB")) return (size_t)atoll(getenv("MC_BATCH_MEM_MB")) << 20;
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) return (size_t)8 << 30;
+  return fr / 10 * 6 + c->spec.cap;
+}
+
+extern "C" int mc_align_batch(mc_context ctx, const mc_batch_item* items, int count,
+                              mc_batch_report* brep) {
+  if (!ctx || !items || count < 1) return set_err(MC_ERR_ARG, "bad batch arguments");
+  HIP_TRY(hipSetDevice(ctx->device));
+  std::vector<mc_outputs> none(count);
+  memset(none.data(), 0, sizeof(mc_outputs) * count);
+  std::vector<MovieRun> runs(count);
+  for (int i = 0; i < count; i++) {
+    if (!items[i].movie || !items[i].params) return set_err(MC_ERR_ARG, "batch item %d: NULL", i);
+    runs[i].m = items[i].movie;
+    runs[i].p = items[i].params;
+    runs[i].out = items[i].out ? items[i].out : &none[i];
+    runs[i].rep = items[i].rep;
+    const int rc = prepare_run(runs[i]);
+    if (rc != MC_OK) {
+      const std::string msg = g_err;
+      return set_err(rc, "batch item %d: %s", i, msg.c_str());
+    }
+  }
+  // groups: same transform size, box, B-factor and noise window, larger
+  // transforms first; each group split into sub-batches that fit the budget
+  std::vector<int> order(count);
+  for (int i = 0; i < count; i++) order[i] = i;
+  auto key_less = [&](int a, int b) {
+    const MovieRun &x = runs[a], &y = runs[b];
+    if (x.g.n != y.g.n) return x.g.n > y.g.n;
+    if (x.box != y.box) return x.box < y.box;
+    if (x.p->bfactor != y.p->bfactor) return x.p->bfactor < y.p->bfactor;
+    return x.wnoise < y.wnoise;
+  };
+  std::stable_sort(order.begin(), order.end(), key_less);
+  const size_t budget = batch_budget(ctx);
+  const int cap0 = ctx->graph_captures, rep0 = ctx->graph_replays;
+  int groups = 0, subs = 0;
+  double ms = 0;
+  size_t i = 0;
+  while (i < order.size()) {
+    size_t j = i + 1;
+    while (j < order.size() && !key_less(order[i], order[j]) && !key_less(order[j], order[i])) j++;
+    groups++;
+    const size_t slot_bytes = sizeof(float2) * (size_t)(runs[order[i]].g.n / 2 + 1) * runs[order[i]].g.n;
+    size_t k = i;
+    while (k < j) {
+      std::vector<MovieRun*> rs;
+      size_t bytes = 0;
+      while (k < j) {
+        const size_t need = slot_bytes * runs[order[k]].nslot;
+        if (!rs.empty() && bytes + need > budget) break;
+        rs.push_back(&runs[order[k]]);
+        bytes += need;
+        k++;
+      }
+      GroupTimes tm;
+      MC_TRY(align_group(ctx, rs, graphs_enabled(), &tm));
+      for (MovieRun* r : rs) fill_report(*r, tm);
+      ms += tm.total;
+      subs++;
+    }
+    i = j;
+  }
+  if (count > 1) ctx->nframe = 0;  // no single resident movie after a batch
+  if (brep) {
+    brep->groups = groups;
+    brep->sub_batches = subs;
+    brep->graph_captures = ctx->graph_captures - cap0;
+    brep->graph_replays = ctx->graph_replays - rep0;
+    brep->ms_total = ms;
   }
   return MC_OK;
 }

@@ -60,6 +60,11 @@ hipError_t launch_shift_sum(const float2* spectra, size_t spec_stride, int nfram
                             const double2* shifts, float2* sum, float2* sum2, int f2_last,
                             hipStream_t st);
 
+// Batched sums: job j = int4(first slot, frame count, first shift index,
+// destination slot) in slot units of `spectra`; one launch for all jobs.
+hipError_t launch_shift_sum_multi(float2* spectra, size_t spec_stride, int n, const int4* jobs,
+                                  int njobs, const double2* shifts, hipStream_t st);
+
 // Sharpness J of the shifted sum S (already formed by launch_shift_sum with
 // the same shifts) and dJ/ds_f: out[f] = (dJ/dsx_f, dJ/dsy_f), out[F] = (J, 0).
 // wk: [n/2+1] column weights (incl. the half-plane factor) then [n] row

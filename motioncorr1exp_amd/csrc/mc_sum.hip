@@ -51,6 +51,44 @@ hipError_t launch_shift_sum(const float2* spectra, size_t spec_stride, int nfram
   return hipGetLastError();
 }
 
+// Several sums in one launch (one per blockIdx.y): job j = (first slot,
+// frame count, first shift, destination slot) sums its frames into its
+// destination slot of the same spectrum array.  Destination slots are
+// disjoint from every job's frame slots.
+__global__ __launch_bounds__(256) void k_shift_sum_multi(float2* spectra, size_t spec_stride,
+                                                         int n, const int4* __restrict__ jobs,
+                                                         const double2* __restrict__ shifts) {
+  const size_t total = (size_t)(n / 2 + 1) * n;
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= total) return;
+  const int4 jb = jobs[blockIdx.y];
+  const int kx = (int)(e / n), ky = (int)(e % n);
+  const int yy = ky < n / 2 ? ky : ky - n;
+  const double inv_n = 1.0 / n;
+  float2 acc = make_float2(0.f, 0.f);
+  for (int f = 0; f < jb.y; f++) {
+    const double2 s = shifts[jb.z + f];
+    if (s.x != s.x) continue;
+    const float2 v = spectra[(size_t)(jb.x + f) * spec_stride + e];
+    double u = (s.x * kx + s.y * yy) * inv_n;
+    u -= floor(u);
+    float sn, cs;
+    sincospif(2.0f * (float)u, &sn, &cs);
+    acc = cadd(acc, cmul(v, make_float2(cs, sn)));
+  }
+  spectra[(size_t)jb.w * spec_stride + e] = acc;
+}
+
+hipError_t launch_shift_sum_multi(float2* spectra, size_t spec_stride, int n, const int4* jobs,
+                                  int njobs, const double2* shifts, hipStream_t st) {
+  if (njobs <= 0) return hipSuccess;
+  if (njobs > 65535) return hipErrorInvalidValue;
+  const size_t total = (size_t)(n / 2 + 1) * n;
+  hipLaunchKernelGGL(k_shift_sum_multi, dim3((unsigned)((total + 255) / 256), (unsigned)njobs),
+                     dim3(256), 0, st, spectra, spec_stride, n, jobs, shifts);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------ sharpness and its adjoint
 // J(s) = sum_k w(k) |S(k)|^2 with S = sum_f F_f e^{i phi_f}, phi_f(k) =
 // 2 pi (s_fx kx + s_fy ky')/n, w = exp(-B|k|^2/(2n^2)) * a_kx (a = 1 at DC and

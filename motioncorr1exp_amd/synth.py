@@ -124,3 +124,55 @@ def make_movie(nx, ny, nframe, dose=1.0, step=0.7, seed=1001, quad_sigma=0.0,
     with ThreadPoolExecutor(max_workers=w) as ex:
         list(ex.map(one, range(nframe)))
     return frames, drift
+
+
+# ------------------------------------------------------ on-device movies
+_gpu_lib = None
+
+
+def _synth_gpu_lib():
+    global _gpu_lib
+    if _gpu_lib is None:
+        path = os.path.join(_HERE, "libmc_synth_gpu.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(path + " (run __graft_entry__.build())")
+        lib = ctypes.CDLL(path)
+        vp = ctypes.c_void_p
+        i = ctypes.c_int
+        lib.mcs_gpu_movie.argtypes = [vp, i, i, i, i, vp, vp, i, ctypes.c_float,
+                                      ctypes.c_ulonglong]
+        _gpu_lib = lib
+    return _gpu_lib
+
+
+def phantom_waves(nw, seed, kc=0.05, kmax=0.5):
+    """nw plane waves (k_x, k_y in cycles/px, phase in cycles) whose radial
+    frequency density is proportional to k P(k), P(k) = 1/(1+(k/kc)^2), on
+    (0, kmax]: inverse CDF k = kc sqrt(exp(u ln(1 + kmax^2/kc^2)) - 1)."""
+    rng = np.random.default_rng(seed + 104729)
+    u = rng.random(nw)
+    k = kc * np.sqrt(np.expm1(u * np.log1p((kmax / kc) ** 2)))
+    th = rng.random(nw) * 2 * np.pi
+    ph = rng.random(nw)
+    return np.ascontiguousarray(np.stack([k * np.cos(th), k * np.sin(th), ph], 1), np.float64)
+
+
+def make_movie_gpu(nx, ny, nframe, dose=1.0, step=0.7, seed=1004, dtype=np.uint8, nwaves=256,
+                   drift=None):
+    """Movie generated in device memory (libmc_synth_gpu.so): a random-wave
+    phantom moved exactly by the drift and Poisson-sampled.  Returns
+    (motioncorr1exp_amd.DeviceArray of [nframe, ny, nx] dtype, drift)."""
+    import motioncorr1exp_amd as mc
+
+    if drift is None:
+        drift = make_drift(nframe, step, seed)
+    drift = np.ascontiguousarray(drift, np.float64)
+    waves = phantom_waves(nwaves, seed)
+    code = {np.dtype(np.uint8): 0, np.dtype(np.float32): 2}[np.dtype(dtype)]
+    d = mc.DeviceArray(nbytes=nframe * ny * nx * np.dtype(dtype).itemsize)
+    rc = _synth_gpu_lib().mcs_gpu_movie(d.ptr, code, nx, ny, nframe, drift.ctypes.data,
+                                        waves.ctypes.data, nwaves, dose,
+                                        (seed * 1000003 + 29) & 0xFFFFFFFFFFFFFFFF)
+    if rc != 0:
+        raise RuntimeError("mcs_gpu_movie failed (HIP error %d)" % rc)
+    return d, drift
