@@ -414,7 +414,7 @@ static float2* slot(mc_context_s* c, int s) { return c->spec.as<float2>() + c->s
 static int run_stage1(mc_context_s* c, const void* frames, const mc_movie_desc* m,
                       const mc_params* p, const Geom& g, int first, int count, int s0) {
   const int nkx = g.n / 2 + 1;
-  const size_t rt_per = (size_t)nkx * g.nsu;
+  const size_t rt_per = mc::rt_frame_elems(g.n, g.nsu);
   // Frames per launch pair: the row-pass scratch of a chunk is sized to stay
   // within the 256 MB Infinity Cache between the row and column passes.
   static const size_t chunk_mb = getenv("MC_STAGE1_CHUNK_MB") ? atoi(getenv("MC_STAGE1_CHUNK_MB")) : 192;

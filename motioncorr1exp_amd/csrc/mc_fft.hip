@@ -84,18 +84,36 @@ __global__ __launch_bounds__(NT) void k_r2c_rows(R2CGeom g, int f0, int nkx, flo
   const char* fbase = g.src + (size_t)(f0 + f) * g.frame_pitch;
   tw_split_load(tlo, thi, gtw, NU, LB, threadIdx.x, NT);
   // packed row z[m] = (x[2m], x[2m+1])
+  struct alignas(2 * sizeof(T)) Pair {
+    T a, b;
+  };
 #pragma unroll
   for (int r = 0; r < ROWS; r++) {
     const int y = y0 + r;
     const T* row = reinterpret_cast<const T*>(fbase + (size_t)(y + g.offy) * g.row_pitch) + g.offx;
     const bool yok = y < g.h;
+    if (g.vec && yok) {
 #pragma unroll 4
-    for (int m = threadIdx.x; m < N; m += NT) {
-      const int x = 2 * m;
-      float2 z;
-      z.x = (yok && x < g.w) ? (float)row[x] : 0.f;
-      z.y = (yok && x + 1 < g.w) ? (float)row[x + 1] : 0.f;
-      buf[r * TS + lix<PB>(m)] = z;
+      for (int m = threadIdx.x; m < N; m += NT) {
+        const int x = 2 * m;
+        float2 z = make_float2(0.f, 0.f);
+        if (x + 1 < g.w) {
+          const Pair v = *reinterpret_cast<const Pair*>(row + x);
+          z = make_float2((float)v.a, (float)v.b);
+        } else if (x < g.w) {
+          z.x = (float)row[x];
+        }
+        buf[r * TS + lix<PB>(m)] = z;
+      }
+    } else {
+#pragma unroll 4
+      for (int m = threadIdx.x; m < N; m += NT) {
+        const int x = 2 * m;
+        float2 z;
+        z.x = (yok && x < g.w) ? (float)row[x] : 0.f;
+        z.y = (yok && x + 1 < g.w) ? (float)row[x + 1] : 0.f;
+        buf[r * TS + lix<PB>(m)] = z;
+      }
     }
   }
   __syncthreads();
@@ -132,14 +150,17 @@ __global__ __launch_bounds__(NT) void k_r2c_rows(R2CGeom g, int f0, int nkx, flo
     }
   }
   __syncthreads();
-  // transposed store RT[kx][y]: ROWS consecutive floats2 per kx
+  // transposed store, bins in groups of four: RT[kx/4][y][kx%4]; 16
+  // consecutive lanes write 4 bins x 4 rows = one 128-byte line
   float2* out = rt + (size_t)f * rt_fstride;
-  for (int i = threadIdx.x; i < ROWS * nkx; i += NT) {
-    const int r = i % ROWS, k = i / ROWS;
+  const int nkg = (nkx + 3) / 4;
+  for (int i = threadIdx.x; i < ROWS * 4 * nkg; i += NT) {
+    const int kk = i & 3, r = (i >> 2) % ROWS, kg = (i >> 2) / ROWS;
+    const int k = 4 * kg + kk;
     const int y = y0 + r;
-    if (y >= g.h) continue;
+    if (y >= g.h || k >= nkx) continue;
     const float2 X = buf[r * TS + lix<PB>(k)];
-    out[(size_t)k * NU + y] = X;
+    out[((size_t)kg * NU + y) * 4 + kk] = X;
     if (k == 0 && rowsum) rowsum[(size_t)f * NU + y] = X.x;
   }
 }
@@ -174,7 +195,7 @@ __global__ __launch_bounds__(NT) void k_c2c_cols(const float2* rt, size_t rt_fst
   tw_split_load(tlo, thi, gtw, NU, LB, threadIdx.x, NT);
   const int L = xcd_remap(blockIdx.x, gridDim.x);
   const int kx = L % nkx, f = L / nkx;
-  const float2* src = rt + (size_t)f * rt_fstride + (size_t)kx * NU;
+  const float2* src = rt + (size_t)f * rt_fstride + (size_t)(kx >> 2) * NU * 4 + (kx & 3);
   float fv = 0.f;
   float2 P = make_float2(0.f, 0.f);
   if (fill) {
@@ -184,7 +205,7 @@ __global__ __launch_bounds__(NT) void k_c2c_cols(const float2* rt, size_t rt_fst
   for (int i = threadIdx.x; i < NU; i += NT) {
     float2 v = make_float2(0.f, 0.f);
     if (i < h_data) {
-      v = src[i];
+      v = src[(size_t)i * 4];
       if (fill) v = cadd(v, cscale(P, fv));
     } else if (fill && kx == 0) {
       v.x = fv * (float)NU;
@@ -211,15 +232,28 @@ static hipError_t frames_to_spectra_t(const R2CGeom& g, int f0, int nframes, flo
   const float2* tw = twiddles(NU);
   if (!tw) return hipErrorOutOfMemory;
   const int nkx = g.n / 2 + 1;
-  const size_t rt_fstride = (size_t)nkx * NU;
+  const size_t rt_fstride = rt_frame_elems(g.n, NU);
   dim3 gr((g.h + ROWS - 1) / ROWS, nframes);
   float* rs = g.pad_mode ? rowsum : nullptr;
+  R2CGeom gv = g;
+  {
+    // pair loads need every (x, x+1) element pair aligned to two elements
+    size_t es = 1;
+    switch (g.dtype) {
+      case DT_INT16: case DT_UINT16: es = 2; break;
+      case DT_FLOAT32: es = 4; break;
+      default: es = 1;
+    }
+    const size_t a = 2 * es;
+    gv.vec = ((uintptr_t)g.src % a == 0 && g.row_pitch % a == 0 && g.frame_pitch % a == 0 &&
+              (g.offx & 1) == 0) ? 1 : 0;
+  }
   switch (g.dtype) {
-    case DT_INT8: hipLaunchKernelGGL((k_r2c_rows<NU, ROWS, NTR, DT_INT8>), gr, dim3(NTR), 0, st, g, f0, nkx, rt, rt_fstride, rs, tw); break;
-    case DT_UINT8: hipLaunchKernelGGL((k_r2c_rows<NU, ROWS, NTR, DT_UINT8>), gr, dim3(NTR), 0, st, g, f0, nkx, rt, rt_fstride, rs, tw); break;
-    case DT_INT16: hipLaunchKernelGGL((k_r2c_rows<NU, ROWS, NTR, DT_INT16>), gr, dim3(NTR), 0, st, g, f0, nkx, rt, rt_fstride, rs, tw); break;
-    case DT_UINT16: hipLaunchKernelGGL((k_r2c_rows<NU, ROWS, NTR, DT_UINT16>), gr, dim3(NTR), 0, st, g, f0, nkx, rt, rt_fstride, rs, tw); break;
-    case DT_FLOAT32: hipLaunchKernelGGL((k_r2c_rows<NU, ROWS, NTR, DT_FLOAT32>), gr, dim3(NTR), 0, st, g, f0, nkx, rt, rt_fstride, rs, tw); break;
+    case DT_INT8: hipLaunchKernelGGL((k_r2c_rows<NU, ROWS, NTR, DT_INT8>), gr, dim3(NTR), 0, st, gv, f0, nkx, rt, rt_fstride, rs, tw); break;
+    case DT_UINT8: hipLaunchKernelGGL((k_r2c_rows<NU, ROWS, NTR, DT_UINT8>), gr, dim3(NTR), 0, st, gv, f0, nkx, rt, rt_fstride, rs, tw); break;
+    case DT_INT16: hipLaunchKernelGGL((k_r2c_rows<NU, ROWS, NTR, DT_INT16>), gr, dim3(NTR), 0, st, gv, f0, nkx, rt, rt_fstride, rs, tw); break;
+    case DT_UINT16: hipLaunchKernelGGL((k_r2c_rows<NU, ROWS, NTR, DT_UINT16>), gr, dim3(NTR), 0, st, gv, f0, nkx, rt, rt_fstride, rs, tw); break;
+    case DT_FLOAT32: hipLaunchKernelGGL((k_r2c_rows<NU, ROWS, NTR, DT_FLOAT32>), gr, dim3(NTR), 0, st, gv, f0, nkx, rt, rt_fstride, rs, tw); break;
     default: return hipErrorInvalidValue;
   }
   if (g.pad_mode) {

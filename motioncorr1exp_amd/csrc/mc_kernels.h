@@ -20,14 +20,21 @@ struct R2CGeom {
   int nu;               // canvas (unbinned) size, power of two
   int n;                // binned size (<= nu), even
   int pad_mode;         // 1: canvas outside the region holds the frame mean
+  int vec = 0;          // set by the launcher: element pairs are 2-element aligned
 };
+
+// Row-pass scratch of one frame (float2 elements): the kept bins kx in
+// [0, n/2] are stored in groups of four, RT[kx/4][y][kx%4], so that four
+// bins of four consecutive rows fill one 128-byte line.
+inline size_t rt_frame_elems(int n, int nu) { return (size_t)((n / 2 + 1 + 3) / 4) * 4 * nu; }
 
 // Twiddle table e^{+2 pi i e/n}, e in [0, n), computed in double on the host.
 const float2* twiddles(int n);
 
 // Stage 1: spectra[f] (layout [kx][ky], kx in [0, n/2], ky in [0, n)) of
 // nframes frames starting at frame index f0.  rt: scratch of
-// nframes*(n/2+1)*nu float2; rowsum/fill: scratch floats (nframes*nu, nframes).
+// nframes*rt_frame_elems(n, nu) float2; rowsum/fill: scratch floats
+// (nframes*nu, nframes).
 hipError_t launch_frames_to_spectra(const R2CGeom& g, int f0, int nframes, float2* spectra,
                                     size_t spec_stride, float2* rt, float* rowsum,
                                     float* fill, const float2* padP, hipStream_t st);

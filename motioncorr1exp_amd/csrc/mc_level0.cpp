@@ -128,7 +128,7 @@ static int make_plan(mc_plan* plan, int n, int inverse) {
   p->inverse = inverse;
   const size_t per = (size_t)(n / 2 + 1) * n;
   hipError_t e = hipMalloc(&p->spec, sizeof(float2) * per);
-  if (e == hipSuccess) e = hipMalloc(&p->rt, sizeof(float2) * per);
+  if (e == hipSuccess) e = hipMalloc(&p->rt, sizeof(float2) * std::max(per, mc::rt_frame_elems(n, n)));
   if (e == hipSuccess) e = hipMalloc(&p->rowsum, sizeof(float) * n);
   if (e == hipSuccess) e = hipMalloc(&p->fill, sizeof(float));
   if (e != hipSuccess) {

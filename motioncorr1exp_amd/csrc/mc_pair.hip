@@ -16,7 +16,9 @@
 //    which equals the column inverse at the box rows (M >= box).
 //  k_pair_rows : one pair per workgroup: C[r][c] = sum_kx a_kx Re(T[kx][r]
 //    e^{2 pi i kx R_c/n}) with a_0 = a_{n/2} = 1, a_kx = 2 (the C2R row
-//    transform with the DC/Nyquist projection), register-tiled.
+//    transform with the DC/Nyquist projection): a real GEMM with K =
+//    2(n/2+1), on the matrix cores (k_pair_rows_mfma, box <= 96) or
+//    register-tiled on the vector ALUs.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -287,6 +289,115 @@ __global__ __launch_bounds__(256) void k_pair_rows(const float2* __restrict__ tb
     }
 }
 
+// The same row pass on the matrix cores: per pair the box map is the real
+// product C = A B with A[r][(kx, 0|1)] = a_kx (Re, Im) T[kx][r] and
+// B[(kx, 0|1)][c] = (cos, -sin)(2 pi kx R_c/n), K = 2(n/2+1).  One
+// v_mfma_f32_32x32x2f32 consumes one kx for a 32 x 32 block (FP32 operands
+// and accumulation).  Chunks of KC bins of T and of the twiddles are staged
+// in LDS (double-buffered: the next chunk is fetched into registers while
+// the current one is multiplied); each of the four waves takes a quarter of
+// every chunk for all NB x NB blocks, and the four partial maps are summed
+// in LDS in wave order.  Lane layout (32x32x2 f32): lane l feeds
+// A[l % 32][l / 32] and B[l / 32][l % 32]; accumulator v of lane l is
+// C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_pair_rows_mfma(const float2* __restrict__ tbuf, int n,
+                                                        int box, float* __restrict__ boxmaps,
+                                                        const float2* __restrict__ tw) {
+  constexpr int W = 32 * NB;
+  constexpr int KC = 32;
+  constexpr int PER = (KC * W + 255) / 256;
+  __shared__ float2 Ts[2][KC][W];
+  __shared__ float2 Es[2][KC][W];
+  const int p = blockIdx.x;
+  const int H = n / 2 + 1;
+  const int ori = (n - box) / 2;
+  const float2* T = tbuf + (size_t)p * H * box;
+  const int wv = threadIdx.x / 64, l = threadIdx.x % 64;
+  const int li = l % 32, lh = l / 32;
+  v16f acc[NB][NB];
+#pragma unroll
+  for (int i = 0; i < NB; i++)
+#pragma unroll
+    for (int j = 0; j < NB; j++)
+#pragma unroll
+      for (int v = 0; v < 16; v++) acc[i][j][v] = 0.f;
+  float2 pt[PER], pe[PER];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int i = threadIdx.x + 256 * q;
+      const int kk = i / W, c = i % W, kx = k0 + kk;
+      float2 t = make_float2(0.f, 0.f), e = make_float2(0.f, 0.f);
+      if (i < KC * W && kx < H && c < box) {
+        const float al = (kx == 0 || 2 * kx == n) ? 1.f : 2.f;
+        t = cscale(T[(size_t)kx * box + c], al);
+        e = tw[(int)(((long long)kx * (ori + c)) & (n - 1))];
+      }
+      pt[q] = t;
+      pe[q] = e;
+    }
+  };
+  auto stash = [&](int b) {
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int i = threadIdx.x + 256 * q;
+      if (i < KC * W) {
+        Ts[b][i / W][i % W] = pt[q];
+        Es[b][i / W][i % W] = pe[q];
+      }
+    }
+  };
+  const int nch = (H + KC - 1) / KC;
+  fetch(0);
+  stash(0);
+  __syncthreads();
+  for (int ch = 0; ch < nch; ch++) {
+    const int b = ch & 1;
+    if (ch + 1 < nch) fetch((ch + 1) * KC);
+#pragma unroll 2
+    for (int kk = wv * (KC / 4); kk < (wv + 1) * (KC / 4); kk++) {
+      float a[NB], bb[NB];
+#pragma unroll
+      for (int i = 0; i < NB; i++) {
+        const float2 tv = Ts[b][kk][32 * i + li];
+        const float2 ev = Es[b][kk][32 * i + li];
+        a[i] = lh ? tv.y : tv.x;
+        bb[i] = lh ? -ev.y : ev.x;
+      }
+#pragma unroll
+      for (int i = 0; i < NB; i++)
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], bb[j], acc[i][j], 0, 0, 0);
+    }
+    if (ch + 1 < nch) stash(b ^ 1);
+    __syncthreads();
+  }
+  // ordered sum of the four partial maps in LDS (reuses the T buffers)
+  float* red = reinterpret_cast<float*>(&Ts[0][0][0]);
+  static_assert(sizeof(Ts) >= sizeof(float) * W * W, "reduction buffer");
+  for (int w = 0; w < 4; w++) {
+    if (wv == w) {
+#pragma unroll
+      for (int i = 0; i < NB; i++)
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+#pragma unroll
+          for (int v = 0; v < 16; v++) {
+            const int r = 32 * i + 8 * (v / 4) + 4 * lh + (v % 4), c = 32 * j + li;
+            if (w == 0) red[r * W + c] = acc[i][j][v];
+            else red[r * W + c] += acc[i][j][v];
+          }
+    }
+    __syncthreads();
+  }
+  float* out = boxmaps + (size_t)p * box * box;
+  for (int e = threadIdx.x; e < box * box; e += 256) out[e] = red[(e / box) * W + e % box];
+}
+
 // Weight table: wxy[kx] = (-1)^kx exp(m kx^2), kx in [0, n/2]; wxy[n/2+1+ky]
 // = (-1)^{y'} exp(m y'^2) / sizec, so z/sizec = wxy[kx] * wxy[n/2+1+ky] with
 // m = -0.5 B/n^2 as in MkPosList (cufunc.cu:358-390).
@@ -414,6 +525,16 @@ hipError_t launch_pair_rows(const float2* tbuf, int npairs, int n, int box, floa
   if (box < 1 || box > n || box > 256) return hipErrorInvalidValue;
   const float2* tw = twiddles(n);
   if (!tw) return hipErrorOutOfMemory;
+  static const bool valu = getenv("MC_PAIR_ROWS_VALU") != nullptr;
+  if (!valu) {
+    switch ((box + 31) / 32) {
+#define MC_ROWS_MFMA(B) \
+  case B: hipLaunchKernelGGL(k_pair_rows_mfma<B>, dim3(npairs), dim3(256), 0, st, tbuf, n, box, boxmaps, tw); return hipGetLastError();
+      MC_ROWS_MFMA(1) MC_ROWS_MFMA(2) MC_ROWS_MFMA(3)
+#undef MC_ROWS_MFMA
+      default: break;
+    }
+  }
   const int bt = (box + 15) / 16;
   switch (bt) {
 #define MC_ROWS_CASE(B) \
