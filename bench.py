@@ -109,7 +109,29 @@ def cpu_baseline(frames_shape, pad, P, F, nthreads, budget_s=20.0):
     }
 
 
-def batch_leg(n_movies, steps, seed=1004):
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_3838x3710x40_pad4096.txt")
+
+
+def committed_traffic(group):
+    """HBM bytes per launch of a kernel group from the committed rocprofv3
+    counter summary (FETCH_SIZE and WRITE_SIZE runs of this workload, scaled
+    by the copy calibration): the sum over the group's kernels of their bytes
+    per dispatch (one dispatch of each per launch of the group)."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    tot, seen = 0.0, False
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            if not line.startswith("traffic "):
+                continue
+            kv = dict(tok.split("=", 1) for tok in line.split()[1:])
+            if kv.get("group") == group:
+                tot += float(kv["bytes"])
+                seen = True
+    return tot if seen else None
+
+
+def batch_leg(n_movies, steps, seed=1004, tune=True):
     """Batching form: n_movies movies of seeded uneven size (square sides
     uniform over the even values 1024..4096, padded to the next power of two
     with the region mean), 8..50 frames, dose uniform in [0.5, 1.5],
@@ -139,6 +161,13 @@ def batch_leg(n_movies, steps, seed=1004):
     mc.check(mc.lib().mc_sync())
     log("batch: %d movies generated on the device in %.1fs" % (n_movies, time.time() - t0))
     ctx = mc.Context(0)
+    if tune:
+        seen = set()
+        for m in movies:
+            key = m["params"]["pad"]
+            if key not in seen:
+                seen.add(key)
+                ctx.autotune(m["dev"], m["shape"], m["dtype"], m["params"])
     res, br = ctx.align_batch(movies)  # warm-up
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -189,6 +218,7 @@ def main():
     ap.add_argument("--no-itr3", action="store_true")
     ap.add_argument("--frames", type=int, default=40)
     ap.add_argument("--no-batch", action="store_true", help="skip the batching form")
+    ap.add_argument("--no-tune", action="store_true", help="default launch shapes, no autotuner")
     ap.add_argument("--batch-movies", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--calib-copy", action="store_true",
@@ -230,6 +260,14 @@ def main():
     def step(prm, want=("frame_shift",)):
         return ctx.align_movie(dev, frames.shape, frames.dtype, prm, want=want, sum_dev=sum_dev)
 
+    tune = None
+    if not args.no_tune:
+        # launch shapes from the project's autotuner (untimed, on this movie)
+        t0 = time.time()
+        tune = ctx.autotune(dev, frames.shape, frames.dtype, params)
+        log("autotune in %.1fs: %s" % (time.time() - t0, {k: v for k, v in tune.items()
+                                                          if not k.startswith("ms_")}))
+
     for _ in range(args.warmup):
         step(params)
     mc.check(L.mc_sync())
@@ -265,13 +303,15 @@ def main():
         "sum": F * S + S,
         "image": S + 4 * n * n,
     }
-    alg = unit_bytes[dom]
+    alg = unit_bytes.get(dom, 0)
     per_launch_ms = kern[dom]["ms_per_launch"]
     # launches of the dominant group per movie
     lpm = kern[dom]["launches"] / args.steps
     achieved = alg / lpm / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else 0.0
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": committed_traffic(dom),
+                "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
                 "algorithmic_bytes_per_launch": alg / lpm}
     comp = compulsory_bytes(F, n, nx, ny, 4, P, 96, 0)
     movies_s = args.steps / dt
@@ -288,7 +328,7 @@ def main():
                                "(253 pairs), bft 150, pbx 96, itr 0",
                    "movies": 1, "frames": F, "nx": nx, "ny": ny, "pad": pad, "pairs": P,
                    "form": "plain FP32"},
-        "roofline": roofline, "whole_movie_roofline": whole, "kernels": kern,
+        "roofline": roofline, "whole_movie_roofline": whole, "kernels": kern, "autotune": tune,
         "stage_ms": {k: rep[k] for k in ("ms_spectra", "ms_pairs", "ms_solve", "ms_iter",
                                          "ms_sum", "ms_total")},
     }
@@ -346,7 +386,7 @@ def main():
     dev.free()
     sum_dev.free()
     if not args.no_batch:
-        out["batch"] = batch_leg(args.batch_movies, args.batch_steps)
+        out["batch"] = batch_leg(args.batch_movies, args.batch_steps, tune=not args.no_tune)
     print(json.dumps(out), flush=True)
 
 

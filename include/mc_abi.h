@@ -30,6 +30,7 @@ extern "C" {
 #define MC_ERR_NOMEM -3    /* device allocation failed                    */
 #define MC_ERR_SOLVE -4    /* least-squares system singular               */
 #define MC_ERR_STATE -5    /* call out of order (e.g. no spectra loaded)  */
+#define MC_ERR_IO -6       /* tuning cache file could not be read/written */
 
 /* Element types of frame data.  Codes 0, 1, 2, 6 are MRC modes
  * (src-1125/mrcheader.h:8-14); mode 0 is read as signed bytes as the
@@ -156,7 +157,11 @@ typedef struct {
   float grad_step_px; /* new: trust-region step of the refinement (px)  */
 } mc_params;
 
-#define MC_PREC_FP32 0
+#define MC_PREC_FP32 0      /* FP32 on the vector ALUs (pair row pass: FP32 MFMA) */
+#define MC_PREC_MFMA_HILO 1 /* pair stage DFTs as FP16 hi/lo MFMA products with
+                               FP32 accumulation (n in [512, 8192], box <= 96);
+                               FFTs of stage 1, sums and the FP64 peak fit as
+                               in MC_PREC_FP32 */
 
 /* Optional results (host pointers; NULL to skip).  Sizes use
  * F = nEnd-nStart+1, K = F-1, P = pair count, B = CCPeakSearchDim. */
@@ -199,7 +204,7 @@ int mc_align_movie(mc_context ctx, const mc_movie_desc *movie, const mc_params *
 
 /* Many movies (no reference counterpart: the reference aligns one stack per
  * process run, DFAlign.cu:423-453).  Movies with the same binned transform
- * size, peak box, B-factor and noise window form a group; a group is split
+ * size, peak box, B-factor, precision form and noise window form a group; a group is split
  * into sub-batches whose resident spectra fit in device memory
  * (MC_BATCH_MEM_MB, default 60% of the free memory).  In a sub-batch, stage 1
  * runs per movie, the pair stage of all movies is one persistent launch, the
@@ -224,6 +229,35 @@ typedef struct {
 
 int mc_align_batch(mc_context ctx, const mc_batch_item *items, int count,
                    mc_batch_report *brep);
+
+/* Autotuner (no reference counterpart).  Times the launch-shape candidates
+ * on this movie and keeps the fastest in the context for every later call
+ * with the same canvas size, binned size and element type (stage 1) and the
+ * same binned size, peak box and precision form (pair stage):
+ *   stage 1: row-pass shape (rows per workgroup x threads: 4x256, 2x256,
+ *            4x512 for canvases >= 4096) x frames per launch (1, 2, 4, 8);
+ *   pair stage (FP32 form): column kernel (M=128, M=128 with next-item
+ *            prefetch, M-generic) and row kernel (FP32 MFMA, vector ALU).
+ * Each candidate runs once untimed and 3 times timed (minimum kept).  With
+ * cache_path, one JSON line per tuned geometry is appended
+ * ({"arch", "cus", "nu", "n", "dtype", "box", "precision", "r2c", "chunk",
+ * "cols", "rows"}); mc_tune_load reads the lines of the same device type
+ * back (later lines win).  Without tuning, the defaults are 4x512 (4x256
+ * below 4096), 192 MB of row-pass scratch per launch, the M=128 column
+ * kernel without prefetch and the FP32 MFMA row kernel. */
+typedef struct {
+  int r2c_variant;        /* 1: 4x256, 2: 2x256, 3: 4x512                   */
+  int chunk_frames;       /* frames per stage-1 launch                      */
+  int pair_cols_variant;  /* 0: M=128, 1: M=128 + prefetch, 2: M-generic    */
+  int pair_rows_variant;  /* 0: FP32 MFMA, 1: vector ALU                    */
+  float ms_stage1[4][4];  /* [r2c variant][chunk 1, 2, 4, 8]; 0 = not run   */
+  float ms_pair_cols[3];  /* 0 = not run                                    */
+  float ms_pair_rows[2];
+} mc_tune_report;
+
+int mc_autotune(mc_context ctx, const mc_movie_desc *movie, const mc_params *p,
+                const char *cache_path, mc_tune_report *rep);
+int mc_tune_load(mc_context ctx, const char *cache_path, int *loaded);
 
 /* Stage entry points on the context's resident state ------------------- */
 /* Stage 1 only: frames -> resident spectra (sizes written to rep). */

@@ -84,6 +84,13 @@ class BatchReport(ctypes.Structure):
                 ("ms_total", ctypes.c_double)]
 
 
+class TuneReport(ctypes.Structure):
+    _fields_ = [("r2c_variant", ctypes.c_int), ("chunk_frames", ctypes.c_int),
+                ("pair_cols_variant", ctypes.c_int), ("pair_rows_variant", ctypes.c_int),
+                ("ms_stage1", (ctypes.c_float * 4) * 4), ("ms_pair_cols", ctypes.c_float * 3),
+                ("ms_pair_rows", ctypes.c_float * 2)]
+
+
 class MaskElem(ctypes.Structure):
     _fields_ = [("x", ctypes.c_int), ("y", ctypes.c_int), ("z", ctypes.c_float)]
 
@@ -98,7 +105,7 @@ ABI_SYMBOLS = [
     "mc_context_create", "mc_context_destroy", "mc_align_movie", "mc_load_frames",
     "mc_set_spectra", "mc_get_spectra", "mc_pair_shifts", "mc_find_peaks", "mc_sum_image",
     "mc_pair_list", "mc_lsq", "mc_threshold", "mc_kernel_timing", "mc_kernel_timing_get",
-    "mc_sharpness", "mc_align_batch",
+    "mc_sharpness", "mc_align_batch", "mc_autotune", "mc_tune_load",
 ]
 
 _lib = None
@@ -151,6 +158,8 @@ def lib():
         L.mc_kernel_timing_get.argtypes = [vp, i, _P(i), _P(d)]
         L.mc_sharpness.argtypes = [vp, vp, f, _P(d), vp]
         L.mc_align_batch.argtypes = [vp, _P(BatchItem), i, _P(BatchReport)]
+        L.mc_autotune.argtypes = [vp, _P(MovieDesc), _P(Params), ctypes.c_char_p, _P(TuneReport)]
+        L.mc_tune_load.argtypes = [vp, ctypes.c_char_p, _P(i)]
         _lib = L
     return _lib
 
@@ -333,6 +342,24 @@ class Context:
         check(self.L.mc_align_batch(self.h, items, n, ctypes.byref(br)))
         out = [self._finish(res, k[3]) for res, k in zip(results, keep)]
         return out, {k: getattr(br, k) for k, _ in BatchReport._fields_}
+
+    def autotune(self, dev_frames, shape, dtype, params, cache_path=None):
+        """Runs mc_autotune on this movie; returns the report as a dict."""
+        p = to_params(params)
+        m = self.movie_desc(dev_frames, shape, dtype)
+        r = TuneReport()
+        check(self.L.mc_autotune(self.h, ctypes.byref(m), ctypes.byref(p),
+                                 cache_path.encode() if cache_path else None, ctypes.byref(r)))
+        return {"r2c_variant": r.r2c_variant, "chunk_frames": r.chunk_frames,
+                "pair_cols_variant": r.pair_cols_variant,
+                "pair_rows_variant": r.pair_rows_variant,
+                "ms_stage1": [list(row) for row in r.ms_stage1],
+                "ms_pair_cols": list(r.ms_pair_cols), "ms_pair_rows": list(r.ms_pair_rows)}
+
+    def tune_load(self, cache_path):
+        n = ctypes.c_int()
+        check(self.L.mc_tune_load(self.h, cache_path.encode(), ctypes.byref(n)))
+        return n.value
 
     def load_frames(self, dev_frames, shape, dtype, params, dev_frames2=None):
         p = to_params(params)

@@ -90,8 +90,14 @@ struct mc_context_s {
   int device = 0;
   hipStream_t st = nullptr;
   DevBuf spec, rt, rowsum, fill, padP, pairs_d, tbuf, box, peaks, shifts, image, tmp, means,
-      sum2img, wk, gpart, gout, jobs;
+      sum2img, wk, gpart, gout, jobs, tmax;
   int graph_captures = 0, graph_replays = 0;
+  // autotuned launch shapes (mc_autotune / mc_tune_load)
+  struct Tune {
+    int nu, n, dtype, box, precision;
+    int r2c, chunk, cols, rows;
+  };
+  std::vector<Tune> tunes;
   // resident spectra
   int n = 0, nframe = 0, nslots = 0, off2 = 0, sum_slot = 0;
   size_t spec_stride = 0;
@@ -159,7 +165,7 @@ extern "C" int mc_context_destroy(mc_context ctx) {
   DevBuf* bufs[] = {&ctx->spec, &ctx->rt, &ctx->rowsum, &ctx->fill, &ctx->padP,
                     &ctx->pairs_d, &ctx->tbuf, &ctx->box, &ctx->peaks, &ctx->shifts,
                     &ctx->image, &ctx->tmp, &ctx->means, &ctx->sum2img, &ctx->wk,
-                    &ctx->gpart, &ctx->gout, &ctx->jobs};
+                    &ctx->gpart, &ctx->gout, &ctx->jobs, &ctx->tmax};
   for (DevBuf* b : bufs) b->release();
   for (hipEvent_t e : ctx->pool) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->st);
@@ -410,15 +416,34 @@ static int ensure_spectra(mc_context_s* c, int n, int nframe, int two) {
 
 static float2* slot(mc_context_s* c, int s) { return c->spec.as<float2>() + c->spec_stride * s; }
 
+static const mc_context_s::Tune* stage1_tune(const mc_context_s* c, int nu, int n, int dtype) {
+  for (const auto& t : c->tunes)
+    if (t.nu == nu && t.n == n && t.dtype == dtype) return &t;
+  return nullptr;
+}
+
+static const mc_context_s::Tune* pair_tune(const mc_context_s* c, int n, int box, int precision) {
+  for (const auto& t : c->tunes)
+    if (t.n == n && t.box == box && t.precision == precision) return &t;
+  return nullptr;
+}
+
 // Stage 1 for frames [first, first+count) of a buffer into slots [s0, ...).
+// variant / chunk < 0: the tuned shape for this geometry, else the default.
 static int run_stage1(mc_context_s* c, const void* frames, const mc_movie_desc* m,
-                      const mc_params* p, const Geom& g, int first, int count, int s0) {
+                      const mc_params* p, const Geom& g, int first, int count, int s0,
+                      int variant = -1, int chunk = -1) {
   const int nkx = g.n / 2 + 1;
   const size_t rt_per = mc::rt_frame_elems(g.n, g.nsu);
-  // Frames per launch pair: the row-pass scratch of a chunk is sized to stay
-  // within the 256 MB Infinity Cache between the row and column passes.
-  static const size_t chunk_mb = getenv("MC_STAGE1_CHUNK_MB") ? atoi(getenv("MC_STAGE1_CHUNK_MB")) : 192;
-  int chunk = (int)std::max<size_t>(1, (chunk_mb << 20) / (rt_per * sizeof(float2)));
+  const mc_context_s::Tune* tn = stage1_tune(c, g.nsu, g.n, m->dtype);
+  if (variant < 0) variant = tn ? tn->r2c : mc::R2C_AUTO;
+  if (chunk <= 0 && tn && tn->chunk > 0) chunk = tn->chunk;
+  if (chunk <= 0) {
+    // Frames per launch pair: the row-pass scratch of a chunk is sized to
+    // stay within the 256 MB Infinity Cache between the row and column passes.
+    static const size_t chunk_mb = getenv("MC_STAGE1_CHUNK_MB") ? atoi(getenv("MC_STAGE1_CHUNK_MB")) : 192;
+    chunk = (int)std::max<size_t>(1, (chunk_mb << 20) / (rt_per * sizeof(float2)));
+  }
   chunk = std::min(chunk, count);
   HIP_TRY(c->rt.ensure(sizeof(float2) * rt_per * chunk));
   HIP_TRY(c->rowsum.ensure(sizeof(float) * (size_t)g.nsu * chunk));
@@ -440,7 +465,8 @@ static int run_stage1(mc_context_s* c, const void* frames, const mc_movie_desc* 
     const int cnt = std::min(chunk, count - f);
     HIP_TRY(mc::launch_frames_to_spectra(rg, f, cnt, slot(c, s0 + f), c->spec_stride,
                                          c->rt.as<float2>(), c->rowsum.as<float>(),
-                                         c->fill.as<float>(), c->padP.as<float2>(), c->st));
+                                         c->fill.as<float>(), c->padP.as<float2>(), variant,
+                                         c->st));
   }
   return MC_OK;
 }
@@ -465,10 +491,11 @@ static int load_frames(mc_context_s* c, const mc_movie_desc* m, const mc_params*
 }
 
 // Pair-stage buffers for np pairs (before any launch, outside graph capture).
-static int pair_reserve(mc_context_s* c, int np, int n, int box) {
+static int pair_reserve(mc_context_s* c, int np, int n, int box, int precision) {
   const int H = n / 2 + 1;
   HIP_TRY(c->pairs_d.ensure(sizeof(int) * 2 * (size_t)np));
   HIP_TRY(c->tbuf.ensure(sizeof(float2) * (size_t)np * H * box));
+  if (precision == MC_PREC_MFMA_HILO) HIP_TRY(c->tmax.ensure(sizeof(float) * (size_t)np * H));
   HIP_TRY(c->box.ensure(sizeof(float) * (size_t)np * box * box));
   HIP_TRY(c->peaks.ensure(sizeof(double) * 3 * (size_t)np));
   return MC_OK;
@@ -476,15 +503,21 @@ static int pair_reserve(mc_context_s* c, int np, int n, int box) {
 
 // Column pass, row pass and peak search over the np slot pairs in pairs_d;
 // results stay on the device (peaks, box maps).  Capturable.
-static int pair_launch(mc_context_s* c, int np, int n, float bfactor, int box, int wnoise) {
+static int pair_launch(mc_context_s* c, int np, int n, float bfactor, int box, int wnoise,
+                       int precision, int cols = -1, int rows = -1) {
+  const mc_context_s::Tune* tn = pair_tune(c, n, box, precision);
+  if (cols < 0) cols = tn ? tn->cols : mc::PAIR_COLS_AUTO;
+  if (rows < 0) rows = tn ? tn->rows : mc::PAIR_ROWS_AUTO;
   {
     KScope ks(c, MC_KT_PAIR_COLS);
     HIP_TRY(mc::launch_pair_cols(c->spec.as<float2>(), c->spec_stride, c->pairs_d.as<int2>(), np,
-                                 n, bfactor, box, c->tbuf.as<float2>(), c->st));
+                                 n, bfactor, box, c->tbuf.as<float2>(), c->tmax.as<float>(),
+                                 precision, cols, c->st));
   }
   {
     KScope ks(c, MC_KT_PAIR_ROWS);
-    HIP_TRY(mc::launch_pair_rows(c->tbuf.as<float2>(), np, n, box, c->box.as<float>(), c->st));
+    HIP_TRY(mc::launch_pair_rows(c->tbuf.as<float2>(), c->tmax.as<float>(), np, n, box,
+                                 c->box.as<float>(), precision, rows, c->st));
   }
   {
     KScope ks(c, MC_KT_PEAKS);
@@ -496,13 +529,13 @@ static int pair_launch(mc_context_s* c, int np, int n, float bfactor, int box, i
 
 // Pair stage on resident slots: pairs (a, b) are slot indices.
 static int pair_stage(mc_context_s* c, const std::vector<int>& slots, float bfactor, int box,
-                      int wnoise, std::vector<double>& res, float* boxmaps_host) {
+                      int wnoise, int precision, std::vector<double>& res, float* boxmaps_host) {
   const int np = (int)slots.size() / 2;
   const int n = c->n;
-  MC_TRY(pair_reserve(c, np, n, box));
+  MC_TRY(pair_reserve(c, np, n, box, precision));
   HIP_TRY(hipMemcpyAsync(c->pairs_d.p, slots.data(), sizeof(int) * slots.size(),
                          hipMemcpyHostToDevice, c->st));
-  MC_TRY(pair_launch(c, np, n, bfactor, box, wnoise));
+  MC_TRY(pair_launch(c, np, n, bfactor, box, wnoise, precision));
   res.resize(3 * (size_t)np);
   HIP_TRY(hipMemcpyAsync(res.data(), c->peaks.p, sizeof(double) * 3 * np, hipMemcpyDeviceToHost,
                          c->st));
@@ -697,7 +730,7 @@ extern "C" int mc_pair_shifts(mc_context ctx, const int* pairs, int npairs, floa
     s[2 * q + 1] = pairs[2 * q + 1] + ctx->off2;
   }
   std::vector<double> res;
-  MC_TRY(pair_stage(ctx, s, bfactor, box, wnoise, res, boxmaps));
+  MC_TRY(pair_stage(ctx, s, bfactor, box, wnoise, MC_PREC_FP32, res, boxmaps));
   for (int q = 0; q < npairs; q++) {
     if (b) {
       b[2 * q] = res[3 * q];
@@ -784,6 +817,12 @@ static int prepare_run(MovieRun& r) {
   if (r.box < 16 || r.box > r.g.n || r.box > 256)
     return set_err(MC_ERR_ARG, "peak box %d must be in [16, min(n, 256)]", r.box);
   r.wnoise = p->NoisePeakSize - 1;  // DFAlign.cu:900
+  if (p->precision != MC_PREC_FP32 && p->precision != MC_PREC_MFMA_HILO)
+    return set_err(MC_ERR_ARG, "unknown precision form %d", p->precision);
+  if (!mc::pair_form_supported(p->precision, r.g.n, r.box))
+    return set_err(MC_ERR_ARG,
+                   "precision form %d needs n in [512, 8192] and peak box <= 96 (n %d, box %d)",
+                   p->precision, r.g.n, r.box);
   r.K = r.F - 1;
   r.P = mc_pair_list(r.F, p->FrameDistOffset, nullptr, 0);
   r.pairs.assign(2 * (size_t)r.P, 0);
@@ -976,17 +1015,17 @@ static bool graphs_enabled() {
 }
 
 static int round_launch(mc_context_s* c, int njobs, int np, int n, float bfactor, int box,
-                        int wnoise) {
+                        int wnoise, int precision) {
   {
     KScope ks(c, MC_KT_SUM);
     HIP_TRY(mc::launch_shift_sum_multi(c->spec.as<float2>(), c->spec_stride, n,
                                        c->jobs.as<int4>(), njobs, c->shifts.as<double2>(), c->st));
   }
-  return pair_launch(c, np, n, bfactor, box, wnoise);
+  return pair_launch(c, np, n, bfactor, box, wnoise, precision);
 }
 
 static int round_graph(mc_context_s* c, RoundGraph& gr, const std::vector<int>& key, int njobs,
-                       int np, int n, float bfactor, int box, int wnoise) {
+                       int np, int n, float bfactor, int box, int wnoise, int precision) {
   if (gr.exec && gr.key == key) return MC_OK;
   if (gr.exec) {
     (void)hipGraphExecDestroy(gr.exec);
@@ -995,7 +1034,7 @@ static int round_graph(mc_context_s* c, RoundGraph& gr, const std::vector<int>& 
   const bool kt = c->kt_on;
   c->kt_on = false;  // no event records inside the graph
   HIP_TRY(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-  const int rc = round_launch(c, njobs, np, n, bfactor, box, wnoise);
+  const int rc = round_launch(c, njobs, np, n, bfactor, box, wnoise, precision);
   hipGraph_t graph = nullptr;
   const hipError_t ec = hipStreamEndCapture(c->st, &graph);
   c->kt_on = kt;
@@ -1026,6 +1065,7 @@ static int align_group(mc_context_s* c, std::vector<MovieRun*>& rs, bool use_gra
   const int n = rs[0]->g.n;
   const int box = rs[0]->box, wnoise = rs[0]->wnoise;
   const float bfactor = rs[0]->p->bfactor;
+  const int precision = rs[0]->p->precision;
   const size_t stride = (size_t)(n / 2 + 1) * n;
   int nslots = 0, ftot = 0;
   for (MovieRun* r : rs) {
@@ -1097,7 +1137,7 @@ static int align_group(mc_context_s* c, std::vector<MovieRun*>& rs, bool use_gra
       slots.push_back(r->base + r->off2 + r->pairs[2 * q + 1]);
     }
   std::vector<double> res;
-  MC_TRY(pair_stage(c, slots, bfactor, box, wnoise, res, nullptr));
+  MC_TRY(pair_stage(c, slots, bfactor, box, wnoise, precision, res, nullptr));
   HIP_TRY(hipEventRecord(ev[2], c->st));
   {
     size_t po = 0;
@@ -1135,7 +1175,7 @@ static int align_group(mc_context_s* c, std::vector<MovieRun*>& rs, bool use_gra
     const int njobs = (int)act.size(), np = (int)islots.size() / 2;
     const bool fresh = !(gr.exec && gr.key == key);
     if (fresh) {
-      MC_TRY(pair_reserve(c, np, n, box));
+      MC_TRY(pair_reserve(c, np, n, box, precision));
       HIP_TRY(c->jobs.ensure(sizeof(int) * jobs.size()));
       HIP_TRY(hipMemcpyAsync(c->jobs.p, jobs.data(), sizeof(int) * jobs.size(),
                              hipMemcpyHostToDevice, c->st));
@@ -1146,13 +1186,13 @@ static int align_group(mc_context_s* c, std::vector<MovieRun*>& rs, bool use_gra
     HIP_TRY(hipMemcpyAsync(c->shifts.p, shifts.data(), sizeof(double) * shifts.size(),
                            hipMemcpyHostToDevice, c->st));
     if (use_graph) {
-      MC_TRY(round_graph(c, gr, key, njobs, np, n, bfactor, box, wnoise));
+      MC_TRY(round_graph(c, gr, key, njobs, np, n, bfactor, box, wnoise, precision));
       KScope ks(c, MC_KT_ROUNDS);
       HIP_TRY(hipGraphLaunch(gr.exec, c->st));
       c->graph_replays++;
     } else {
       gr.key = key;  // buffers uploaded for this set
-      MC_TRY(round_launch(c, njobs, np, n, bfactor, box, wnoise));
+      MC_TRY(round_launch(c, njobs, np, n, bfactor, box, wnoise, precision));
     }
     res.resize(3 * (size_t)np);
     HIP_TRY(hipMemcpyAsync(res.data(), c->peaks.p, sizeof(double) * 3 * np, hipMemcpyDeviceToHost,
@@ -1275,6 +1315,7 @@ extern "C" int mc_align_batch(mc_context ctx, const mc_batch_item* items, int co
     if (x.g.n != y.g.n) return x.g.n > y.g.n;
     if (x.box != y.box) return x.box < y.box;
     if (x.p->bfactor != y.p->bfactor) return x.p->bfactor < y.p->bfactor;
+    if (x.p->precision != y.p->precision) return x.p->precision < y.p->precision;
     return x.wnoise < y.wnoise;
   };
   std::stable_sort(order.begin(), order.end(), key_less);
@@ -1315,5 +1356,196 @@ extern "C" int mc_align_batch(mc_context ctx, const mc_batch_item* items, int co
     brep->graph_replays = ctx->graph_replays - rep0;
     brep->ms_total = ms;
   }
+  return MC_OK;
+}
+
+// ============================================================ autotuner
+// Launch shapes are chosen by timing the candidates on the caller's movie:
+// the row-pass shape and frames per stage-1 launch (all frames of the movie
+// through stage 1), then the pair column and row kernels (all pairs of the
+// movie).  Each candidate runs once untimed, then `reps` times between HIP
+// events on the context's stream; the minimum is kept.
+static const int kChunks[4] = {1, 2, 4, 8};
+
+static int device_arch(int dev, char* arch, size_t len, int* cus) {
+  hipDeviceProp_t pr;
+  HIP_TRY(hipGetDeviceProperties(&pr, dev));
+  snprintf(arch, len, "%s", pr.gcnArchName);
+  char* colon = strchr(arch, ':');
+  if (colon) *colon = 0;
+  *cus = pr.multiProcessorCount;
+  return MC_OK;
+}
+
+template <class F>
+static int time_min(mc_context_s* c, int reps, float* ms, F&& run) {
+  hipEvent_t a, b;
+  HIP_TRY(hipEventCreate(&a));
+  HIP_TRY(hipEventCreate(&b));
+  int rc = run();  // untimed
+  float best = 1e30f;
+  for (int r = 0; rc == MC_OK && r < reps; r++) {
+    if (hipEventRecord(a, c->st) != hipSuccess) rc = set_err(MC_ERR_DEVICE, "event record");
+    if (rc == MC_OK) rc = run();
+    if (rc == MC_OK && hipEventRecord(b, c->st) != hipSuccess) rc = set_err(MC_ERR_DEVICE, "event record");
+    if (rc == MC_OK && hipEventSynchronize(b) != hipSuccess) rc = set_err(MC_ERR_DEVICE, "event sync");
+    if (rc == MC_OK) best = std::min(best, elapsed(a, b));
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *ms = best;
+  return rc;
+}
+
+static void set_tune(mc_context_s* c, const mc_context_s::Tune& t) {
+  for (auto& u : c->tunes)
+    if (u.nu == t.nu && u.n == t.n && u.dtype == t.dtype && u.box == t.box &&
+        u.precision == t.precision) {
+      u = t;
+      return;
+    }
+  c->tunes.push_back(t);
+}
+
+extern "C" int mc_autotune(mc_context ctx, const mc_movie_desc* m, const mc_params* p,
+                           const char* cache_path, mc_tune_report* rep) {
+  if (!ctx || !m || !p) return set_err(MC_ERR_ARG, "NULL argument");
+  HIP_TRY(hipSetDevice(ctx->device));
+  mc_context_s* c = ctx;
+  MovieRun r;
+  mc_outputs none;
+  memset(&none, 0, sizeof none);
+  r.m = m;
+  r.p = p;
+  r.out = &none;
+  MC_TRY(prepare_run(r));
+  const int n = r.g.n, F = r.F, box = r.box;
+  const size_t stride = (size_t)(n / 2 + 1) * n;
+  HIP_TRY(c->spec.ensure(sizeof(float2) * stride * (F + 1)));
+  c->n = n;
+  c->spec_stride = stride;
+  c->nframe = 0;
+  const int reps = 3;
+  mc_tune_report tr;
+  memset(&tr, 0, sizeof tr);
+  // ---- stage 1
+  const int first = p->nStart - m->first_frame;
+  float best = 1e30f;
+  int bv = mc::R2C_AUTO, bc = 0;
+  for (int v = mc::R2C_R4_T256; v < mc::R2C_VARIANTS; v++) {
+    if (v == mc::R2C_R4_T512 && r.g.nsu < 4096) continue;
+    for (int k = 0; k < 4; k++) {
+      const int ch = kChunks[k];
+      if (ch > F && k > 0) continue;
+      float ms = 0;
+      MC_TRY(time_min(c, reps, &ms, [&] {
+        return run_stage1(c, m->frames, m, p, r.g, first, F, 0, v, ch);
+      }));
+      tr.ms_stage1[v][k] = ms;
+      if (ms < best) {
+        best = ms;
+        bv = v;
+        bc = ch;
+      }
+    }
+  }
+  // ---- pair stage (spectra of the last stage-1 run are resident)
+  std::vector<int> slots(r.pairs);
+  MC_TRY(pair_reserve(c, r.P, n, box, p->precision));
+  HIP_TRY(hipMemcpyAsync(c->pairs_d.p, slots.data(), sizeof(int) * slots.size(),
+                         hipMemcpyHostToDevice, c->st));
+  int bcol = mc::PAIR_COLS_AUTO, brow = mc::PAIR_ROWS_AUTO;
+  if (p->precision == MC_PREC_FP32) {
+    best = 1e30f;
+    for (int v = 0; v < mc::PAIR_COLS_VARIANTS; v++) {
+      if (v != mc::PAIR_COLS_GENERIC && !(n >= 512 && box <= 128)) continue;
+      float ms = 0;
+      MC_TRY(time_min(c, reps, &ms, [&] {
+        const hipError_t e = mc::launch_pair_cols(c->spec.as<float2>(), c->spec_stride,
+                                                  c->pairs_d.as<int2>(), r.P, n, p->bfactor, box,
+                                                  c->tbuf.as<float2>(), c->tmax.as<float>(),
+                                                  p->precision, v, c->st);
+        return e == hipSuccess ? MC_OK : set_err(MC_ERR_DEVICE, "pair columns: %s", hipGetErrorString(e));
+      }));
+      tr.ms_pair_cols[v] = ms;
+      if (ms < best) {
+        best = ms;
+        bcol = v;
+      }
+    }
+    best = 1e30f;
+    for (int v = 0; v < mc::PAIR_ROWS_VARIANTS; v++) {
+      if (v == mc::PAIR_ROWS_AUTO && box > 96) continue;
+      float ms = 0;
+      MC_TRY(time_min(c, reps, &ms, [&] {
+        const hipError_t e = mc::launch_pair_rows(c->tbuf.as<float2>(), c->tmax.as<float>(), r.P,
+                                                  n, box, c->box.as<float>(), p->precision, v,
+                                                  c->st);
+        return e == hipSuccess ? MC_OK : set_err(MC_ERR_DEVICE, "pair rows: %s", hipGetErrorString(e));
+      }));
+      tr.ms_pair_rows[v] = ms;
+      if (ms < best) {
+        best = ms;
+        brow = v;
+      }
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(c->st));
+  mc_context_s::Tune t{r.g.nsu, n, m->dtype, box, p->precision, bv, bc, bcol, brow};
+  set_tune(c, t);
+  tr.r2c_variant = bv;
+  tr.chunk_frames = bc;
+  tr.pair_cols_variant = bcol;
+  tr.pair_rows_variant = brow;
+  if (rep) *rep = tr;
+  if (cache_path && *cache_path) {
+    char arch[64];
+    int cus = 0;
+    MC_TRY(device_arch(c->device, arch, sizeof arch, &cus));
+    FILE* fp = fopen(cache_path, "a");
+    if (!fp) return set_err(MC_ERR_IO, "cannot append to %s", cache_path);
+    fprintf(fp,
+            "{\"arch\": \"%s\", \"cus\": %d, \"nu\": %d, \"n\": %d, \"dtype\": %d, \"box\": %d, "
+            "\"precision\": %d, \"r2c\": %d, \"chunk\": %d, \"cols\": %d, \"rows\": %d}\n",
+            arch, cus, t.nu, t.n, t.dtype, t.box, t.precision, t.r2c, t.chunk, t.cols, t.rows);
+    fclose(fp);
+  }
+  return MC_OK;
+}
+
+static bool json_int(const char* line, const char* key, int* v) {
+  char pat[40];
+  snprintf(pat, sizeof pat, "\"%s\":", key);
+  const char* q = strstr(line, pat);
+  return q && sscanf(q + strlen(pat), " %d", v) == 1;
+}
+
+extern "C" int mc_tune_load(mc_context ctx, const char* cache_path, int* loaded) {
+  if (!ctx || !cache_path) return set_err(MC_ERR_ARG, "NULL argument");
+  char arch[64];
+  int cus = 0;
+  MC_TRY(device_arch(ctx->device, arch, sizeof arch, &cus));
+  FILE* fp = fopen(cache_path, "r");
+  if (!fp) return set_err(MC_ERR_IO, "cannot open %s", cache_path);
+  char line[512], want[96];
+  snprintf(want, sizeof want, "\"arch\": \"%s\"", arch);
+  int cnt = 0;
+  while (fgets(line, sizeof line, fp)) {
+    mc_context_s::Tune t;
+    int lc = 0;
+    if (!strstr(line, want) || !json_int(line, "cus", &lc) || lc != cus) continue;
+    if (json_int(line, "nu", &t.nu) && json_int(line, "n", &t.n) &&
+        json_int(line, "dtype", &t.dtype) && json_int(line, "box", &t.box) &&
+        json_int(line, "precision", &t.precision) && json_int(line, "r2c", &t.r2c) &&
+        json_int(line, "chunk", &t.chunk) && json_int(line, "cols", &t.cols) &&
+        json_int(line, "rows", &t.rows) && t.r2c >= 0 && t.r2c < mc::R2C_VARIANTS &&
+        t.cols >= 0 && t.cols < mc::PAIR_COLS_VARIANTS && t.rows >= 0 &&
+        t.rows < mc::PAIR_ROWS_VARIANTS && t.chunk >= 0) {
+      set_tune(ctx, t);  // later lines win
+      cnt++;
+    }
+  }
+  fclose(fp);
+  if (loaded) *loaded = cnt;
   return MC_OK;
 }

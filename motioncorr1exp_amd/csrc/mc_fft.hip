@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -221,19 +222,34 @@ __global__ __launch_bounds__(NT) void k_c2c_cols(const float2* rt, size_t rt_fst
   }
 }
 
+template <int NU, int ROWS, int NT>
+static hipError_t launch_rows(const R2CGeom& g, int f0, int nframes, int nkx, float2* rt,
+                              size_t rt_fstride, float* rs, const float2* tw, hipStream_t st) {
+  dim3 gr((g.h + ROWS - 1) / ROWS, nframes);
+  switch (g.dtype) {
+#define MC_R2C(D) case D: hipLaunchKernelGGL((k_r2c_rows<NU, ROWS, NT, D>), gr, dim3(NT), 0, st, g, f0, nkx, rt, rt_fstride, rs, tw); break;
+    MC_R2C(DT_INT8) MC_R2C(DT_UINT8) MC_R2C(DT_INT16) MC_R2C(DT_UINT16) MC_R2C(DT_FLOAT32)
+#undef MC_R2C
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+constexpr int rows_nt(int rows, int n) {
+  return (rows * n / 16) < 64 ? 64 : ((rows * n / 16) > 256 ? 256 : rows * n / 16);
+}
+
 template <int NU>
 static hipError_t frames_to_spectra_t(const R2CGeom& g, int f0, int nframes, float2* spectra,
                                       size_t spec_stride, float2* rt, float* rowsum,
-                                      float* fill, const float2* padP, hipStream_t st) {
-  constexpr int ROWS = 4;
+                                      float* fill, const float2* padP, int variant,
+                                      hipStream_t st) {
   constexpr int N = NU / 2;
-  constexpr int NTR = (ROWS * N / 16) < 64 ? 64 : ((ROWS * N / 16) > 256 ? 256 : ROWS * N / 16);
   constexpr int NTC = (NU / 16) < 64 ? 64 : ((NU / 16) > 256 ? 256 : NU / 16);
   const float2* tw = twiddles(NU);
   if (!tw) return hipErrorOutOfMemory;
   const int nkx = g.n / 2 + 1;
   const size_t rt_fstride = rt_frame_elems(g.n, NU);
-  dim3 gr((g.h + ROWS - 1) / ROWS, nframes);
   float* rs = g.pad_mode ? rowsum : nullptr;
   R2CGeom gv = g;
   {
@@ -248,14 +264,24 @@ static hipError_t frames_to_spectra_t(const R2CGeom& g, int f0, int nframes, flo
     gv.vec = ((uintptr_t)g.src % a == 0 && g.row_pitch % a == 0 && g.frame_pitch % a == 0 &&
               (g.offx & 1) == 0) ? 1 : 0;
   }
-  switch (g.dtype) {
-    case DT_INT8: hipLaunchKernelGGL((k_r2c_rows<NU, ROWS, NTR, DT_INT8>), gr, dim3(NTR), 0, st, gv, f0, nkx, rt, rt_fstride, rs, tw); break;
-    case DT_UINT8: hipLaunchKernelGGL((k_r2c_rows<NU, ROWS, NTR, DT_UINT8>), gr, dim3(NTR), 0, st, gv, f0, nkx, rt, rt_fstride, rs, tw); break;
-    case DT_INT16: hipLaunchKernelGGL((k_r2c_rows<NU, ROWS, NTR, DT_INT16>), gr, dim3(NTR), 0, st, gv, f0, nkx, rt, rt_fstride, rs, tw); break;
-    case DT_UINT16: hipLaunchKernelGGL((k_r2c_rows<NU, ROWS, NTR, DT_UINT16>), gr, dim3(NTR), 0, st, gv, f0, nkx, rt, rt_fstride, rs, tw); break;
-    case DT_FLOAT32: hipLaunchKernelGGL((k_r2c_rows<NU, ROWS, NTR, DT_FLOAT32>), gr, dim3(NTR), 0, st, gv, f0, nkx, rt, rt_fstride, rs, tw); break;
-    default: return hipErrorInvalidValue;
+  // row-pass shape: 4 rows x 512 threads on canvases >= 4096 by default
+  if (variant == R2C_AUTO) variant = (4 * N / 16 >= 512) ? R2C_R4_T512 : R2C_R4_T256;
+  if (variant == R2C_R4_T512 && 4 * N / 16 < 512) variant = R2C_R4_T256;
+  hipError_t e = hipSuccess;
+  bool done = false;
+  if constexpr (4 * N / 16 >= 512) {
+    if (variant == R2C_R4_T512) {
+      e = launch_rows<NU, 4, 512>(gv, f0, nframes, nkx, rt, rt_fstride, rs, tw, st);
+      done = true;
+    }
   }
+  if (!done) {
+    if (variant == R2C_R2_T256)
+      e = launch_rows<NU, 2, rows_nt(2, N)>(gv, f0, nframes, nkx, rt, rt_fstride, rs, tw, st);
+    else
+      e = launch_rows<NU, 4, rows_nt(4, N)>(gv, f0, nframes, nkx, rt, rt_fstride, rs, tw, st);
+  }
+  if (e != hipSuccess) return e;
   if (g.pad_mode) {
     hipLaunchKernelGGL(k_fill, dim3(nframes), dim3(256), 0, st, rowsum, NU, g.h, g.w, fill);
   }
@@ -267,16 +293,11 @@ static hipError_t frames_to_spectra_t(const R2CGeom& g, int f0, int nframes, flo
 
 hipError_t launch_frames_to_spectra(const R2CGeom& g, int f0, int nframes, float2* spectra,
                                     size_t spec_stride, float2* rt, float* rowsum, float* fill,
-                                    const float2* padP, hipStream_t st) {
+                                    const float2* padP, int variant, hipStream_t st) {
   switch (g.nu) {
-    case 64: return frames_to_spectra_t<64>(g, f0, nframes, spectra, spec_stride, rt, rowsum, fill, padP, st);
-    case 128: return frames_to_spectra_t<128>(g, f0, nframes, spectra, spec_stride, rt, rowsum, fill, padP, st);
-    case 256: return frames_to_spectra_t<256>(g, f0, nframes, spectra, spec_stride, rt, rowsum, fill, padP, st);
-    case 512: return frames_to_spectra_t<512>(g, f0, nframes, spectra, spec_stride, rt, rowsum, fill, padP, st);
-    case 1024: return frames_to_spectra_t<1024>(g, f0, nframes, spectra, spec_stride, rt, rowsum, fill, padP, st);
-    case 2048: return frames_to_spectra_t<2048>(g, f0, nframes, spectra, spec_stride, rt, rowsum, fill, padP, st);
-    case 4096: return frames_to_spectra_t<4096>(g, f0, nframes, spectra, spec_stride, rt, rowsum, fill, padP, st);
-    case 8192: return frames_to_spectra_t<8192>(g, f0, nframes, spectra, spec_stride, rt, rowsum, fill, padP, st);
+#define MC_F2S(NU) case NU: return frames_to_spectra_t<NU>(g, f0, nframes, spectra, spec_stride, rt, rowsum, fill, padP, variant, st);
+    MC_F2S(64) MC_F2S(128) MC_F2S(256) MC_F2S(512) MC_F2S(1024) MC_F2S(2048) MC_F2S(4096) MC_F2S(8192)
+#undef MC_F2S
     default: return hipErrorInvalidValue;
   }
 }

@@ -35,9 +35,12 @@ const float2* twiddles(int n);
 // nframes frames starting at frame index f0.  rt: scratch of
 // nframes*rt_frame_elems(n, nu) float2; rowsum/fill: scratch floats
 // (nframes*nu, nframes).
+// Row-pass shapes (rows per workgroup x threads); R2C_AUTO picks 4 x 512 on
+// canvases >= 4096 and 4 x min(256, 2nu/16) below.
+enum { R2C_AUTO = 0, R2C_R4_T256 = 1, R2C_R2_T256 = 2, R2C_R4_T512 = 3, R2C_VARIANTS = 4 };
 hipError_t launch_frames_to_spectra(const R2CGeom& g, int f0, int nframes, float2* spectra,
                                     size_t spec_stride, float2* rt, float* rowsum,
-                                    float* fill, const float2* padP, hipStream_t st);
+                                    float* fill, const float2* padP, int variant, hipStream_t st);
 
 // Inverse 2-D transform of one spectrum ([kx][ky] layout, size n) to an
 // n x n real image scaled by `scale` (C2R with the DC/Nyquist projection).
@@ -48,11 +51,23 @@ hipError_t launch_spectrum_to_image(const float2* spec, int n, float scale, floa
 // Pair stage: for each pair p, T[p][kx][r] (column pass) then the box map
 // C[p][r][c] (row pass).  pairs: device int2 (a, b) indices into `spectra`
 // (frame slots of stride spec_stride); operand a is kept, b conjugated.
+// precision 0: FP32 (vector ALUs; row pass on FP32 MFMA); 1: the first DFT
+// stage of the column pass and the row pass as FP16 hi/lo MFMA products
+// (tmax: npairs*(n/2+1) floats of column maxima, written by the column pass
+// and read by the row pass).
+// Plain-form variants (the hi/lo form has one kernel per pass):
+//   column pass: AUTO = the M = 128 persistent kernel (n >= 512, box <= 128),
+//   M128_PF = the same with next-item register prefetch (fewer resident
+//   workgroups), GENERIC = the M-generic persistent kernel; row pass: AUTO =
+//   FP32 MFMA (box <= 96), VALU = register-tiled vector ALU kernel.
+enum { PAIR_COLS_AUTO = 0, PAIR_COLS_M128_PF = 1, PAIR_COLS_GENERIC = 2, PAIR_COLS_VARIANTS = 3 };
+enum { PAIR_ROWS_AUTO = 0, PAIR_ROWS_VALU = 1, PAIR_ROWS_VARIANTS = 2 };
+bool pair_form_supported(int precision, int n, int box);
 hipError_t launch_pair_cols(const float2* spectra, size_t spec_stride, const int2* pairs,
                             int npairs, int n, float bfactor, int box, float2* tbuf,
-                            hipStream_t st);
-hipError_t launch_pair_rows(const float2* tbuf, int npairs, int n, int box, float* boxmaps,
-                            hipStream_t st);
+                            float* tmax, int precision, int variant, hipStream_t st);
+hipError_t launch_pair_rows(const float2* tbuf, const float* tmax, int npairs, int n, int box,
+                            float* boxmaps, int precision, int variant, hipStream_t st);
 
 // Peaks: integer peak + FP64 16x16 -> 512x512 sub-pixel search per box map.
 // out: 3 doubles per pair (sx, sy, cc).

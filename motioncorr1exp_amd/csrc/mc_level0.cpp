@@ -174,7 +174,8 @@ extern "C" int mc_fft2d(float* buf, mc_plan plan) {
   g.n = n;
   g.pad_mode = 0;
   L0_TRY(mc::launch_frames_to_spectra(g, 0, 1, plan->spec, (size_t)(n / 2 + 1) * n, plan->rt,
-                                      plan->rowsum, plan->fill, nullptr, nullptr));
+                                      plan->rowsum, plan->fill, nullptr, mc::R2C_AUTO,
+                                      nullptr));
   L0_TRY(mc::launch_spec_to_padded(plan->spec, n, reinterpret_cast<float2*>(buf), nullptr));
   return MC_OK;
 }

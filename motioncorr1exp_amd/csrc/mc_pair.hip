@@ -21,6 +21,7 @@
 //    register-tiled on the vector ALUs.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <map>
@@ -32,6 +33,11 @@
 #include "mc_kernels.h"
 
 namespace mc {
+
+// MFMA operand and accumulator vectors
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef _Float16 v4h __attribute__((ext_vector_type(4)));
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 
 // Persistent: the grid is a multiple of 8 workgroups; workgroup b runs on
 // XCD b % 8 and walks a contiguous range of work items (kx major, pair
@@ -72,7 +78,7 @@ __global__ __launch_bounds__(NT) void k_pair_cols(const float2* __restrict__ spe
   float2 ctw[AS];
 #pragma unroll
   for (int i = 0; i < AS; i++) {
-    const int a = ch * AS + i;
+    const int a = i * SP + ch;
     ctw[i] = cact ? tw[(a * R) & (N - 1)] : make_float2(0.f, 0.f);
   }
   const int Rm = R & (M - 1);
@@ -105,7 +111,7 @@ __global__ __launch_bounds__(NT) void k_pair_cols(const float2* __restrict__ spe
     if (cact) {
       float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
-      for (int i = 0; i < AS; i++) acc = cadd(acc, cmul(ctw[i], buf[(ch * AS + i) * TS + lix<PB>(Rm)]));
+      for (int i = 0; i < AS; i++) acc = cadd(acc, cmul(ctw[i], buf[(i * SP + ch) * TS + lix<PB>(Rm)]));
       part[ch][cr] = acc;
     }
     __syncthreads();
@@ -128,7 +134,7 @@ __global__ __launch_bounds__(NT) void k_pair_cols(const float2* __restrict__ spe
 //    only on tid % 16 and stay in registers;
 //  combine: SP adjacent lanes share one output row and reduce with xor
 //    shuffles (no LDS round trip).
-template <int N, int SP>
+template <int N, int SP, bool PF>
 __global__ __launch_bounds__(256) void k_pair_cols_m128(const float2* __restrict__ spectra,
                                                         size_t spec_stride,
                                                         const int2* __restrict__ pairs,
@@ -171,17 +177,40 @@ __global__ __launch_bounds__(256) void k_pair_cols_m128(const float2* __restrict
   const int Rm = R & (M - 1);
   float2 ctw[AS];
 #pragma unroll
-  for (int i = 0; i < AS; i++) ctw[i] = tw[((ch * AS + i) * R) & (N - 1)];
+  for (int i = 0; i < AS; i++) ctw[i] = tw[((i * SP + ch) * R) & (N - 1)];
   const int xcd = blockIdx.x % 8, lid = blockIdx.x / 8, nper = gridDim.x / 8;
   const long long items = (long long)npairs * H;
   const long long i0 = items * xcd / 8, i1 = items * (xcd + 1) / 8;
-  for (long long it = i0 + lid; it < i1; it += nper) {
+  // With PF (software pipeline) the raw spectrum values of the next item are
+  // loaded into registers while the current item goes through stage 1 and
+  // the combine, so the global-load latency overlaps LDS work (at the cost
+  // of 64 more VGPRs, i.e. one workgroup less per CU).
+  float2 ra[P0][16], rb[P0][16];
+  auto fetch = [&](long long it) {
     const int kx = (int)(it / npairs), p = (int)(it % npairs);
     const int2 pr = pairs[p];
     const float2* sa = spectra + (size_t)pr.x * spec_stride + (size_t)kx * N;
     const float2* sb = spectra + (size_t)pr.y * spec_stride + (size_t)kx * N;
+#pragma unroll
+    for (int pp = 0; pp < P0; pp++) {
+      const int q = tid + pp * NT;
+      if (NB0 % NT == 0 || q < NB0) {
+        const int a = q % A, j = q / A;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+          const int e = a + A * (j + 8 * r);
+          ra[pp][r] = sa[e];
+          rb[pp][r] = sb[e];
+        }
+      }
+    }
+  };
+  if (PF && i0 + lid < i1) fetch(i0 + lid);
+  for (long long it = i0 + lid; it < i1; it += nper) {
+    const int kx = (int)(it / npairs), p = (int)(it % npairs);
     const float wx = wxy[kx];
-    // ---- stage 0 from global
+    if (!PF) fetch(it);
+    // ---- stage 0 from the prefetched registers
 #pragma unroll
     for (int pp = 0; pp < P0; pp++) {
       const int q = tid + pp * NT;
@@ -189,17 +218,13 @@ __global__ __launch_bounds__(256) void k_pair_cols_m128(const float2* __restrict
         const int a = q % A, j = q / A;
         float2 v[16];
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-          const int e = a + A * (j + 8 * r);
-          v[r] = cmulc(sa[e], sb[e]);
-        }
-#pragma unroll
-        for (int r = 0; r < 16; r++) v[r] = cscale(v[r], wx * wy[pp][r]);
+        for (int r = 0; r < 16; r++) v[r] = cscale(cmulc(ra[pp][r], rb[pp][r]), wx * wy[pp][r]);
         dft16<1>(v);
 #pragma unroll
         for (int r = 0; r < 16; r++) buf[a * TS + lix<PB>(16 * j + r)] = v[r];
       }
     }
+    if (PF && it + nper < i1) fetch(it + nper);
     __syncthreads();
     // ---- stage 1 in place (radix 8, Ns = 16)
 #pragma unroll
@@ -221,7 +246,7 @@ __global__ __launch_bounds__(256) void k_pair_cols_m128(const float2* __restrict
     // ---- combine T[R] = sum_a e^{2 pi i a R/N} Y_a[R mod M]
     float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
-    for (int i = 0; i < AS; i++) acc = cadd(acc, cmul(ctw[i], buf[(ch * AS + i) * TS + lix<PB>(Rm)]));
+    for (int i = 0; i < AS; i++) acc = cadd(acc, cmul(ctw[i], buf[(i * SP + ch) * TS + lix<PB>(Rm)]));
 #pragma unroll
     for (int o = 1; o < SP; o <<= 1) {
       acc.x += __shfl_xor(acc.x, o);
@@ -229,6 +254,194 @@ __global__ __launch_bounds__(256) void k_pair_cols_m128(const float2* __restrict
     }
     if (cact && ch == 0) tbuf[((size_t)p * H + kx) * box + cr] = acc;
     __syncthreads();  // buf is rewritten by the next item's stage 0
+  }
+}
+
+// ----------------------------------------------- MFMA FP16 hi/lo form
+// Same decimated pruned inverse as k_pair_cols_m128, with its first DFT
+// stage (16 points, stride 8A in ky) run on the matrix cores.  With
+// ky = c + 8A b1 (c in [0, 8A), b1 in [0, 16)) the stage is the complex
+// product Z[k1][c] = sum_b1 e^{2 pi i b1 k1/16} X[c + 8A b1], i.e. the real
+// product [Zr; Zi] = [[C, -S], [S, C]] [Xr; Xi] with K = 32, on 32-column
+// blocks of c.  Operands are split into FP16 pairs x = hi + lo (hi =
+// fp16(x), lo = fp16(x - hi)) and multiplied as hi*hi + hi*lo + lo*hi with
+// FP32 accumulation (v_mfma_f32_32x32x8f16): relative error about 2^-21 per
+// product.  Each column of X is scaled by a power of two that puts its
+// largest component in [2^13, 2^14) before the split and unscaled after,
+// so the FP16 range never clips.  Stage 2 (radix 8 with twiddles) and the
+// combine run on the vector ALUs in FP32 as in the plain form.
+// Lane layout (32x32x8 f16): lane l feeds A[l % 32][4 (l / 32) + t] and
+// B[4 (l / 32) + t][l % 32], t = 0..3; accumulator v of lane l is
+// C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].
+MC_DEV void split_hilo(float x, _Float16& hi, _Float16& lo) {
+  hi = (_Float16)x;
+  lo = (_Float16)(x - (float)hi);
+}
+
+template <int N, int SP>
+__global__ __launch_bounds__(256) void k_pair_cols_mfma(const float2* __restrict__ spectra,
+                                                        size_t spec_stride,
+                                                        const int2* __restrict__ pairs,
+                                                        int npairs,
+                                                        const float* __restrict__ wxy, int box,
+                                                        float2* __restrict__ tbuf,
+                                                        float* __restrict__ tmax,
+                                                        const float2* __restrict__ tw) {
+  constexpr int NT = 256;
+  constexpr int M = 128;
+  constexpr int A = N / M;
+  constexpr int H = N / 2 + 1;
+  constexpr int PB = 4;
+  constexpr int TS = lds_len<PB>(M) | 1;
+  constexpr int CB = 8 * A / 32;       // 32-column blocks of c (block cb -> wave cb % 4)
+  constexpr int CBW = (CB + 3) / 4;    // blocks per wave (at most)
+  constexpr int NB1 = A * 16;
+  constexpr int P1 = (NB1 + NT - 1) / NT;
+  constexpr int AS = A / SP;
+  static_assert(CB >= 1 && (8 * A) % 32 == 0 && A % SP == 0, "shape");
+  __shared__ float2 buf[A * TS];
+  __shared__ float red[NT / 64];
+  const int tid = threadIdx.x;
+  const int wv = tid / 64, l = tid % 64, li = l % 32, lh = l / 32;
+  // DFT-16 matrix fragments (rows (k1, re|im), columns (b1, re|im)), hi/lo
+  v4h dh[4], dl[4];
+  {
+    const int i = li;
+    const int k1 = i & 15;
+    const bool im_row = i >= 16;
+#pragma unroll
+    for (int ks = 0; ks < 4; ks++)
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const int kg = 8 * ks + 4 * lh + t;
+        const int b1 = kg & 15;
+        const bool im_col = kg >= 16;
+        const float2 e = tw[((b1 * k1) & 15) * (N / 16)];  // (cos, sin)(2 pi b1 k1/16)
+        float v;
+        if (!im_row) v = im_col ? -e.y : e.x;
+        else v = im_col ? e.x : e.y;
+        _Float16 h, lo;
+        split_hilo(v, h, lo);
+        dh[ks][t] = h;
+        dl[ks][t] = lo;
+      }
+  }
+  // per-lane ky weights: column c = 32 (wv + 4 u) + li, rows b1 = 4 lh + t
+  // and 8 + 4 lh + t
+  float wy[CBW][8];
+#pragma unroll
+  for (int u = 0; u < CBW; u++) {
+    const int cb = wv + 4 * u;
+    const int c = 32 * (cb < CB ? cb : 0) + li;
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+      const int b1 = (t < 4 ? 0 : 8) + 4 * lh + (t & 3);
+      wy[u][t] = wxy[H + c + 8 * A * b1];
+    }
+  }
+  float2 t1[8];
+#pragma unroll
+  for (int r = 0; r < 8; r++) t1[r] = tw[((tid % 16) * r * (N / M)) & (N - 1)];
+  const int ori = (N - box) / 2;
+  const int cr = tid / SP, ch = tid % SP;
+  const bool cact = cr < box;
+  const int R = ori + (cact ? cr : 0);
+  const int Rm = R & (M - 1);
+  float2 ctw[AS];
+#pragma unroll
+  for (int i = 0; i < AS; i++) ctw[i] = tw[((i * SP + ch) * R) & (N - 1)];
+  const int xcd = blockIdx.x % 8, lid = blockIdx.x / 8, nper = gridDim.x / 8;
+  const long long items = (long long)npairs * H;
+  const long long i0 = items * xcd / 8, i1 = items * (xcd + 1) / 8;
+  for (long long it = i0 + lid; it < i1; it += nper) {
+    const int kx = (int)(it / npairs), p = (int)(it % npairs);
+    const int2 pr = pairs[p];
+    const float2* sa = spectra + (size_t)pr.x * spec_stride + (size_t)kx * N;
+    const float2* sb = spectra + (size_t)pr.y * spec_stride + (size_t)kx * N;
+    const float wx = wxy[kx];
+#pragma unroll
+    for (int u = 0; u < CBW; u++) {
+      const int cb = wv + 4 * u;
+      if (cb >= CB) continue;  // uniform per wave
+      const int c = 32 * cb + li;
+      float2 x[8];
+      float m = 0.f;
+#pragma unroll
+      for (int t = 0; t < 8; t++) {
+        const int b1 = (t < 4 ? 0 : 8) + 4 * lh + (t & 3);
+        const int e = c + 8 * A * b1;
+        x[t] = cscale(cmulc(sa[e], sb[e]), wx * wy[u][t]);
+        m = fmaxf(m, fmaxf(fabsf(x[t].x), fabsf(x[t].y)));
+      }
+      m = fmaxf(m, __shfl_xor(m, 32));
+      int ex = 0;
+      (void)frexpf(m, &ex);
+      const int sh = m > 0.f ? 14 - ex : 0;
+      v4h bh[4], bl[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        _Float16 h, lo;
+        split_hilo(ldexpf(x[t].x, sh), h, lo);      // ks 0: Re, b1 = 4 lh + t
+        bh[0][t] = h; bl[0][t] = lo;
+        split_hilo(ldexpf(x[t + 4].x, sh), h, lo);  // ks 1: Re, b1 = 8 + 4 lh + t
+        bh[1][t] = h; bl[1][t] = lo;
+        split_hilo(ldexpf(x[t].y, sh), h, lo);      // ks 2: Im
+        bh[2][t] = h; bl[2][t] = lo;
+        split_hilo(ldexpf(x[t + 4].y, sh), h, lo);  // ks 3: Im
+        bh[3][t] = h; bl[3][t] = lo;
+      }
+      v16f acc;
+#pragma unroll
+      for (int v = 0; v < 16; v++) acc[v] = 0.f;
+#pragma unroll
+      for (int ks = 0; ks < 4; ks++) {
+        acc = __builtin_amdgcn_mfma_f32_32x32x8f16(dh[ks], bl[ks], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8f16(dl[ks], bh[ks], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8f16(dh[ks], bh[ks], acc, 0, 0, 0);
+      }
+      // rows 8 (v/4) + 4 lh + v%4: Re of k1 for v < 8, Im of the same k1 at v + 8
+      const int a = c % A, j = c / A;
+#pragma unroll
+      for (int v = 0; v < 8; v++) {
+        const int k1 = 8 * (v / 4) + 4 * lh + (v % 4);
+        buf[a * TS + lix<PB>(16 * j + k1)] = make_float2(ldexpf(acc[v], -sh), ldexpf(acc[v + 8], -sh));
+      }
+    }
+    __syncthreads();
+    // ---- stage 1 in place (radix 8, Ns = 16), FP32
+#pragma unroll
+    for (int pp = 0; pp < P1; pp++) {
+      const int q = tid + pp * NT;
+      if (NB1 % NT == 0 || q < NB1) {
+        const int a = q / 16, j = q % 16;
+        float2 v[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) v[r] = buf[a * TS + lix<PB>(j + 16 * r)];
+#pragma unroll
+        for (int r = 1; r < 8; r++) v[r] = cmul(v[r], t1[r]);
+        dft8<1>(v);
+#pragma unroll
+        for (int r = 0; r < 8; r++) buf[a * TS + lix<PB>(j + 16 * r)] = v[r];
+      }
+    }
+    __syncthreads();
+    // ---- combine T[R] = sum_a e^{2 pi i a R/N} Y_a[R mod M]
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < AS; i++) acc = cadd(acc, cmul(ctw[i], buf[(i * SP + ch) * TS + lix<PB>(Rm)]));
+#pragma unroll
+    for (int o = 1; o < SP; o <<= 1) {
+      acc.x += __shfl_xor(acc.x, o);
+      acc.y += __shfl_xor(acc.y, o);
+    }
+    float mx = (cact && ch == 0) ? fmaxf(fabsf(acc.x), fabsf(acc.y)) : 0.f;
+    if (cact && ch == 0) tbuf[((size_t)p * H + kx) * box + cr] = acc;
+    // column maximum of T for the scale of the row pass
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    if (l == 0) red[wv] = mx;
+    __syncthreads();  // also: buf is rewritten by the next item's stage 0
+    if (tid == 0) tmax[(size_t)p * H + kx] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
   }
 }
 
@@ -300,8 +513,6 @@ __global__ __launch_bounds__(256) void k_pair_rows(const float2* __restrict__ tb
 // in LDS in wave order.  Lane layout (32x32x2 f32): lane l feeds
 // A[l % 32][l / 32] and B[l / 32][l % 32]; accumulator v of lane l is
 // C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].
-typedef float v16f __attribute__((ext_vector_type(16)));
-
 template <int NB>
 __global__ __launch_bounds__(256) void k_pair_rows_mfma(const float2* __restrict__ tbuf, int n,
                                                         int box, float* __restrict__ boxmaps,
@@ -398,6 +609,141 @@ __global__ __launch_bounds__(256) void k_pair_rows_mfma(const float2* __restrict
   for (int e = threadIdx.x; e < box * box; e += 256) out[e] = red[(e / box) * W + e % box];
 }
 
+// Row pass of the MFMA FP16 hi/lo form: the GEMM of k_pair_rows_mfma with
+// v_mfma_f32_32x32x8f16 (four kx per instruction: K index 2 kk + (Re|Im)).
+// T of a pair is scaled by the power of two that puts max|T| (the column
+// maxima tmax written by k_pair_cols_mfma) in [2^13, 2^14); T and the
+// twiddles are split into FP16 hi/lo pairs when staged in LDS and
+// multiplied as hi*hi + hi*lo + lo*hi.
+template <int NB>
+__global__ __launch_bounds__(256) void k_pair_rows_mfma16(const float2* __restrict__ tbuf,
+                                                          const float* __restrict__ tmax, int n,
+                                                          int box, float* __restrict__ boxmaps,
+                                                          const float2* __restrict__ tw) {
+  constexpr int W = 32 * NB;
+  constexpr int KC = 32;
+  constexpr int PER = (KC * W + 255) / 256;
+  __shared__ h2 S4[4][2][KC][W];  // T hi, T lo, E hi, E lo; double-buffered
+  __shared__ float red[4];
+  auto Th = S4[0];
+  auto Tl = S4[1];
+  auto Eh = S4[2];
+  auto El = S4[3];
+  const int p = blockIdx.x;
+  const int H = n / 2 + 1;
+  const int ori = (n - box) / 2;
+  const float2* T = tbuf + (size_t)p * H * box;
+  const int wv = threadIdx.x / 64, l = threadIdx.x % 64;
+  const int li = l % 32, lh = l / 32;
+  // pair scale
+  float m = 0.f;
+  for (int k = threadIdx.x; k < H; k += 256) m = fmaxf(m, tmax[(size_t)p * H + k]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+  if (l == 0) red[wv] = m;
+  __syncthreads();
+  m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+  int ex = 0;
+  (void)frexpf(m, &ex);
+  const int sh = m > 0.f ? 14 - ex : 0;
+  v16f acc[NB][NB];
+#pragma unroll
+  for (int i = 0; i < NB; i++)
+#pragma unroll
+    for (int j = 0; j < NB; j++)
+#pragma unroll
+      for (int v = 0; v < 16; v++) acc[i][j][v] = 0.f;
+  float2 pt[PER], pe[PER];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int i = threadIdx.x + 256 * q;
+      const int kk = i / W, c = i % W, kx = k0 + kk;
+      float2 t = make_float2(0.f, 0.f), e = make_float2(0.f, 0.f);
+      if (i < KC * W && kx < H && c < box) {
+        const float al = (kx == 0 || 2 * kx == n) ? 1.f : 2.f;
+        t = T[(size_t)kx * box + c];
+        t = make_float2(ldexpf(al * t.x, sh), ldexpf(al * t.y, sh));
+        e = tw[(int)(((long long)kx * (ori + c)) & (n - 1))];
+        e.y = -e.y;
+      }
+      pt[q] = t;
+      pe[q] = e;
+    }
+  };
+  auto stash = [&](int b) {
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int i = threadIdx.x + 256 * q;
+      if (i < KC * W) {
+        _Float16 a0, a1, b0, b1;
+        split_hilo(pt[q].x, a0, a1);
+        split_hilo(pt[q].y, b0, b1);
+        Th[b][i / W][i % W] = h2{a0, b0};
+        Tl[b][i / W][i % W] = h2{a1, b1};
+        split_hilo(pe[q].x, a0, a1);
+        split_hilo(pe[q].y, b0, b1);
+        Eh[b][i / W][i % W] = h2{a0, b0};
+        El[b][i / W][i % W] = h2{a1, b1};
+      }
+    }
+  };
+  const int nch = (H + KC - 1) / KC;
+  fetch(0);
+  stash(0);
+  __syncthreads();
+  for (int ch = 0; ch < nch; ch++) {
+    const int b = ch & 1;
+    if (ch + 1 < nch) fetch((ch + 1) * KC);
+#pragma unroll
+    for (int ks = 0; ks < KC / 16; ks++) {
+      const int kk = wv * (KC / 4) + 4 * ks + 2 * lh;  // this lane's two bins
+      v4h ah[NB], al_[NB], bh[NB], bl[NB];
+#pragma unroll
+      for (int i = 0; i < NB; i++) {
+        const h2 x0 = Th[b][kk][32 * i + li], x1 = Th[b][kk + 1][32 * i + li];
+        const h2 y0 = Tl[b][kk][32 * i + li], y1 = Tl[b][kk + 1][32 * i + li];
+        ah[i] = v4h{x0[0], x0[1], x1[0], x1[1]};
+        al_[i] = v4h{y0[0], y0[1], y1[0], y1[1]};
+        const h2 e0 = Eh[b][kk][32 * i + li], e1 = Eh[b][kk + 1][32 * i + li];
+        const h2 f0 = El[b][kk][32 * i + li], f1 = El[b][kk + 1][32 * i + li];
+        bh[i] = v4h{e0[0], e0[1], e1[0], e1[1]};
+        bl[i] = v4h{f0[0], f0[1], f1[0], f1[1]};
+      }
+#pragma unroll
+      for (int i = 0; i < NB; i++)
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x8f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x8f16(al_[i], bh[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x8f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        }
+    }
+    if (ch + 1 < nch) stash(b ^ 1);
+    __syncthreads();
+  }
+  float* redm = reinterpret_cast<float*>(&S4[0][0][0][0]);
+  static_assert(sizeof(S4) >= sizeof(float) * W * W, "reduction buffer");
+  for (int w = 0; w < 4; w++) {
+    if (wv == w) {
+#pragma unroll
+      for (int i = 0; i < NB; i++)
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+#pragma unroll
+          for (int v = 0; v < 16; v++) {
+            const int r = 32 * i + 8 * (v / 4) + 4 * lh + (v % 4), c = 32 * j + li;
+            if (w == 0) redm[r * W + c] = acc[i][j][v];
+            else redm[r * W + c] += acc[i][j][v];
+          }
+    }
+    __syncthreads();
+  }
+  float* out = boxmaps + (size_t)p * box * box;
+  for (int e = threadIdx.x; e < box * box; e += 256)
+    out[e] = ldexpf(redm[(e / box) * W + e % box], -sh);
+}
+
 // Weight table: wxy[kx] = (-1)^kx exp(m kx^2), kx in [0, n/2]; wxy[n/2+1+ky]
 // = (-1)^{y'} exp(m y'^2) / sizec, so z/sizec = wxy[kx] * wxy[n/2+1+ky] with
 // m = -0.5 B/n^2 as in MkPosList (cufunc.cu:358-390).
@@ -429,15 +775,16 @@ static const float* weights(int n, float bfactor) {
   return d;
 }
 
-static int pair_grid() {
-  static int g = 0;
-  if (!g) {
-    int dev = 0, cus = 256;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    g = ((cus * 4 + 7) / 8) * 8;
-  }
-  return g;
+// Persistent grid: every workgroup resident at once (the item ranges are
+// static), a multiple of 8 so that each XCD gets the same number.
+template <class K>
+static int persistent_grid(K kernel, int nt) {
+  int dev = 0, cus = 256, per = 1;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, nt, 0) != hipSuccess || per < 1)
+    per = 1;
+  return std::max(8, (cus * per) / 8 * 8);
 }
 
 template <int N, int M, int SP>
@@ -447,33 +794,52 @@ static hipError_t pair_cols_t(const float2* spectra, size_t spec_stride, const i
   constexpr int NT = 256;
   const float2* tw = twiddles(N);
   if (!tw) return hipErrorOutOfMemory;
-  hipLaunchKernelGGL((k_pair_cols<N, M, NT, SP>), dim3(pair_grid()), dim3(NT), 0, st, spectra,
+  static const int grid = persistent_grid(k_pair_cols<N, M, NT, SP>, NT);
+  hipLaunchKernelGGL((k_pair_cols<N, M, NT, SP>), dim3(grid), dim3(NT), 0, st, spectra,
                      spec_stride, pairs, npairs, wxy, box, tbuf, tw);
   return hipGetLastError();
 }
 
 template <int N, int SP>
 static hipError_t pair_cols_m128(const float2* spectra, size_t spec_stride, const int2* pairs,
-                                 int npairs, const float* wxy, int box, float2* tbuf,
+                                 int npairs, const float* wxy, int box, float2* tbuf, bool pf,
                                  hipStream_t st) {
   const float2* tw = twiddles(N);
   if (!tw) return hipErrorOutOfMemory;
-  hipLaunchKernelGGL((k_pair_cols_m128<N, SP>), dim3(pair_grid()), dim3(256), 0, st, spectra,
-                     spec_stride, pairs, npairs, wxy, box, tbuf, tw);
+  static const int g1 = persistent_grid(k_pair_cols_m128<N, SP, true>, 256);
+  static const int g0 = persistent_grid(k_pair_cols_m128<N, SP, false>, 256);
+  if (pf)
+    hipLaunchKernelGGL((k_pair_cols_m128<N, SP, true>), dim3(g1), dim3(256), 0, st, spectra,
+                       spec_stride, pairs, npairs, wxy, box, tbuf, tw);
+  else
+    hipLaunchKernelGGL((k_pair_cols_m128<N, SP, false>), dim3(g0), dim3(256), 0, st, spectra,
+                       spec_stride, pairs, npairs, wxy, box, tbuf, tw);
+  return hipGetLastError();
+}
+
+template <int N, int SP>
+static hipError_t pair_cols_mfma(const float2* spectra, size_t spec_stride, const int2* pairs,
+                                 int npairs, const float* wxy, int box, float2* tbuf,
+                                 float* tmax, hipStream_t st) {
+  const float2* tw = twiddles(N);
+  if (!tw) return hipErrorOutOfMemory;
+  static const int g = persistent_grid(k_pair_cols_mfma<N, SP>, 256);
+  hipLaunchKernelGGL((k_pair_cols_mfma<N, SP>), dim3(g), dim3(256), 0, st, spectra, spec_stride,
+                     pairs, npairs, wxy, box, tbuf, tmax, tw);
   return hipGetLastError();
 }
 
 template <int N>
 static hipError_t pair_cols_n(const float2* spectra, size_t spec_stride, const int2* pairs,
-                              int npairs, const float* wxy, int box, float2* tbuf,
+                              int npairs, const float* wxy, int box, float2* tbuf, int variant,
                               hipStream_t st) {
   if constexpr (N >= 512) {
-    static const bool legacy = getenv("MC_PAIR_COLS_V2") != nullptr;
-    if (!legacy) {
+    if (variant != PAIR_COLS_GENERIC) {
+      const bool pf = variant == PAIR_COLS_M128_PF;
       if (box <= 64)
-        return pair_cols_m128<N, 4>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
+        return pair_cols_m128<N, 4>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, pf, st);
       if (box <= 128)
-        return pair_cols_m128<N, 2>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
+        return pair_cols_m128<N, 2>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, pf, st);
     }
   }
   if constexpr (N / 128 >= 4) {
@@ -499,34 +865,60 @@ static hipError_t pair_cols_n(const float2* spectra, size_t spec_stride, const i
   return hipErrorInvalidValue;
 }
 
+bool pair_form_supported(int precision, int n, int box) {
+  if (precision == 0) return box >= 1 && box <= n && box <= 256;
+  if (precision == 1) return n >= 512 && n <= 8192 && box >= 1 && box <= 96 && box <= n;
+  return false;
+}
+
 hipError_t launch_pair_cols(const float2* spectra, size_t spec_stride, const int2* pairs,
                             int npairs, int n, float bfactor, int box, float2* tbuf,
-                            hipStream_t st) {
+                            float* tmax, int precision, int variant, hipStream_t st) {
   if (npairs <= 0) return hipSuccess;
-  if (box < 1 || box > n || box > 256) return hipErrorInvalidValue;
+  if (!pair_form_supported(precision, n, box)) return hipErrorInvalidValue;
   const float* wxy = weights(n, bfactor);
   if (!wxy) return hipErrorOutOfMemory;
+  if (precision == 1) {
+    if (!tmax) return hipErrorInvalidValue;
+    switch (n) {
+#define MC_COLS_MFMA(NN) \
+  case NN: return box <= 64 ? pair_cols_mfma<NN, 4>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, tmax, st) \
+                            : pair_cols_mfma<NN, 2>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, tmax, st);
+      MC_COLS_MFMA(512) MC_COLS_MFMA(1024) MC_COLS_MFMA(2048) MC_COLS_MFMA(4096) MC_COLS_MFMA(8192)
+#undef MC_COLS_MFMA
+      default: return hipErrorInvalidValue;
+    }
+  }
   switch (n) {
-    case 64: return pair_cols_n<64>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
-    case 128: return pair_cols_n<128>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
-    case 256: return pair_cols_n<256>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
-    case 512: return pair_cols_n<512>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
-    case 1024: return pair_cols_n<1024>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
-    case 2048: return pair_cols_n<2048>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
-    case 4096: return pair_cols_n<4096>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
-    case 8192: return pair_cols_n<8192>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
+    case 64: return pair_cols_n<64>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
+    case 128: return pair_cols_n<128>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
+    case 256: return pair_cols_n<256>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
+    case 512: return pair_cols_n<512>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
+    case 1024: return pair_cols_n<1024>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
+    case 2048: return pair_cols_n<2048>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
+    case 4096: return pair_cols_n<4096>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
+    case 8192: return pair_cols_n<8192>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
     default: return hipErrorInvalidValue;
   }
 }
 
-hipError_t launch_pair_rows(const float2* tbuf, int npairs, int n, int box, float* boxmaps,
-                            hipStream_t st) {
+hipError_t launch_pair_rows(const float2* tbuf, const float* tmax, int npairs, int n, int box,
+                            float* boxmaps, int precision, int variant, hipStream_t st) {
   if (npairs <= 0) return hipSuccess;
-  if (box < 1 || box > n || box > 256) return hipErrorInvalidValue;
+  if (!pair_form_supported(precision, n, box)) return hipErrorInvalidValue;
   const float2* tw = twiddles(n);
   if (!tw) return hipErrorOutOfMemory;
-  static const bool valu = getenv("MC_PAIR_ROWS_VALU") != nullptr;
-  if (!valu) {
+  if (precision == 1) {
+    if (!tmax) return hipErrorInvalidValue;
+    switch ((box + 31) / 32) {
+#define MC_ROWS_MFMA16(B) \
+  case B: hipLaunchKernelGGL(k_pair_rows_mfma16<B>, dim3(npairs), dim3(256), 0, st, tbuf, tmax, n, box, boxmaps, tw); return hipGetLastError();
+      MC_ROWS_MFMA16(1) MC_ROWS_MFMA16(2) MC_ROWS_MFMA16(3)
+#undef MC_ROWS_MFMA16
+      default: return hipErrorInvalidValue;
+    }
+  }
+  if (variant != PAIR_ROWS_VALU) {
     switch ((box + 31) / 32) {
 #define MC_ROWS_MFMA(B) \
   case B: hipLaunchKernelGGL(k_pair_rows_mfma<B>, dim3(npairs), dim3(256), 0, st, tbuf, n, box, boxmaps, tw); return hipGetLastError();

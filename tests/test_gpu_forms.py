@@ -1,7 +1,10 @@
-"""GPU tests of the differentiation form: the adjoint of the phase-shift-and-
-sum sharpness objective (mc_sharpness) against the FP64 oracle gradient on
-the same spectra and against central finite differences (h = 1e-3 px), and
-the gradient refinement inside mc_align_movie (grad_steps > 0)."""
+"""GPU tests of the new forms.  Differentiation: the adjoint of the
+phase-shift-and-sum sharpness objective (mc_sharpness) against the FP64
+oracle gradient on the same spectra and against central finite differences
+(h = 1e-3 px), and the gradient refinement inside mc_align_movie
+(grad_steps > 0).  Precision: the MFMA FP16 hi/lo form of the pair stage
+against the FP64 oracle (box maps) and against the FP32 form (shifts within
+0.01 px RMS, sums within 1e-4)."""
 import numpy as np
 import pytest
 
@@ -78,3 +81,82 @@ def test_refinement_increases_sharpness(ctx):
     e1 = r1["frame_shift"] - r1["frame_shift"][0] - truth
     # refinement leaves the 1/32 grid and does not move away from the truth
     assert np.sqrt((e1 ** 2).mean()) <= np.sqrt((e0 ** 2).mean()) + 0.01
+
+
+# --------------------------------------------------- MFMA FP16 hi/lo form
+def _forms(ctx, dev, shape, dtype, prm, want):
+    r0 = ctx.align_movie(dev, shape, dtype, dict(prm, precision=0), want=want)
+    r1 = ctx.align_movie(dev, shape, dtype, dict(prm, precision=1), want=want)
+    return r0, r1
+
+
+def test_mfma_form_box_maps_match_fp64_oracle(ctx):
+    """Box maps of the hi/lo form against the FP64 oracle: within 1e-5 of
+    each map's maximum, the bar of the FP32 form."""
+    F, n = 6, 512
+    frames, drift = make_movie(n, n, F, dose=2.0, step=0.7, seed=91)
+    d = mc.DeviceArray(frames)
+    prm = mc.default_params(F, FrameDistOffset=1, CCPeakSearchDim=64, precision=1)
+    r = ctx.align_movie(d, frames.shape, frames.dtype, prm, want=("boxmaps", "pair_b"))
+    o = Oracle("f64").align_movie(frames, dict(bin=1.0, fod=1, bfactor=150.0, box=64, full_c2r=1,
+                                               nthreads=8))
+    bm, obm = r["boxmaps"], o["boxmaps"]
+    scale = np.abs(obm).reshape(len(obm), -1).max(1)[:, None, None]
+    assert (np.abs(bm - obm) / scale).max() < 1e-5
+    db = np.abs(r["pair_b"] - o["pair_b"])
+    assert np.all((db < 1e-9) | (np.abs(db - 1 / 32) < 1e-9))
+
+
+@pytest.mark.parametrize("n,F,box", [(1024, 8, 96), (4096, 10, 96)])
+def test_mfma_form_matches_fp32_form(ctx, n, F, box):
+    """Per-frame shifts of the hi/lo form within 0.01 px RMS of the FP32 form
+    sums within 1e-4 relative L2, at the 4096 canvas."""
+    from motioncorr1exp_amd.synth import make_movie_gpu
+
+    dev, drift = make_movie_gpu(n, n, F, dose=1.0, step=0.7, seed=n + F, dtype=np.float32)
+    prm = mc.default_params(F, FrameDistOffset=2, CCPeakSearchDim=box, iter=2)
+    want = ("frame_shift", "sum_image", "pair_b")
+    r0, r1 = _forms(ctx, dev, (F, n, n), np.float32, prm, want)
+    d = r1["frame_shift"] - r0["frame_shift"]
+    assert np.sqrt(np.mean(d ** 2)) <= 0.01
+    db = np.abs(r1["pair_b"] - r0["pair_b"])
+    assert np.all((db < 1e-9) | (np.abs(db - 1 / 32) < 1e-9))
+    rel = np.linalg.norm(r1["sum_image"] - r0["sum_image"]) / np.linalg.norm(r0["sum_image"])
+    assert rel < 1e-4, rel
+    dev.free()
+
+
+def test_mfma_form_rejects_unsupported_geometry(ctx):
+    frames, _ = make_movie(256, 256, 4, dose=2.0, step=0.5, seed=5)
+    d = mc.DeviceArray(frames)
+    with pytest.raises(mc.MCError, match="precision form 1"):
+        ctx.align_movie(d, frames.shape, frames.dtype,
+                        mc.default_params(4, CCPeakSearchDim=64, precision=1))
+
+
+# ---------------------------------------------------------------- autotuner
+def test_autotune_picks_valid_shapes_and_keeps_results(ctx, tmp_path):
+    """mc_autotune times every candidate, keeps the fastest, and the tuned
+    path gives the same shifts as the default shapes (1/32-grid flips aside);
+    the cache file round-trips through mc_tune_load on a fresh context."""
+    F, n = 8, 1024
+    frames, drift = make_movie(n, n, F, dose=2.0, step=0.7, seed=123)
+    d = mc.DeviceArray(frames)
+    prm = mc.default_params(F, FrameDistOffset=2, CCPeakSearchDim=96)
+    c0 = mc.Context(0)
+    ref = c0.align_movie(d, frames.shape, frames.dtype, prm, want=("frame_shift", "sum_image"))
+    c0.close()
+    cache = str(tmp_path / "tune.jsonl")
+    rep = ctx.autotune(d, frames.shape, frames.dtype, prm, cache_path=cache)
+    assert rep["r2c_variant"] in (1, 2) and rep["chunk_frames"] in (1, 2, 4, 8)
+    assert rep["pair_cols_variant"] in (0, 1, 2) and rep["pair_rows_variant"] in (0, 1)
+    best = min(v for row in rep["ms_stage1"] for v in row if v > 0)
+    assert rep["ms_stage1"][rep["r2c_variant"]][[1, 2, 4, 8].index(rep["chunk_frames"])] == best
+    tuned = ctx.align_movie(d, frames.shape, frames.dtype, prm, want=("frame_shift", "sum_image"))
+    dd = np.abs(tuned["frame_shift"] - ref["frame_shift"])
+    assert np.all((dd < 1e-6) | (np.abs(dd - 1 / 32) < 1e-6) | (np.abs(dd - 1 / 16) < 1e-6))
+    c1 = mc.Context(0)
+    assert c1.tune_load(cache) == 1
+    again = c1.align_movie(d, frames.shape, frames.dtype, prm, want=("frame_shift",))
+    assert np.array_equal(again["frame_shift"], tuned["frame_shift"])
+    c1.close()
