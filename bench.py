@@ -131,7 +131,7 @@ def committed_traffic(group):
     return tot if seen else None
 
 
-def batch_leg(n_movies, steps, seed=1004, tune=True):
+def batch_leg(n_movies, steps, seed=1004, tune=True, tune_cache=None):
     """Batching form: n_movies movies of seeded uneven size (square sides
     uniform over the even values 1024..4096, padded to the next power of two
     with the region mean), 8..50 frames, dose uniform in [0.5, 1.5],
@@ -162,11 +162,18 @@ def batch_leg(n_movies, steps, seed=1004, tune=True):
     log("batch: %d movies generated on the device in %.1fs" % (n_movies, time.time() - t0))
     ctx = mc.Context(0)
     if tune:
-        seen = set()
+        # shipped table first; geometries it lacks are tuned on their first movie
+        have = set()
+        if tune_cache and os.path.exists(tune_cache):
+            ctx.tune_load(tune_cache)
+            with open(tune_cache) as f:
+                for line in f:
+                    e = json.loads(line)
+                    have.add((e["nu"], e["dtype"], e["box"], e["precision"]))
         for m in movies:
-            key = m["params"]["pad"]
-            if key not in seen:
-                seen.add(key)
+            key = (m["params"]["pad"], mc.MC_DTYPE[np.dtype(m["dtype"])], 96, 0)
+            if key not in have:
+                have.add(key)
                 ctx.autotune(m["dev"], m["shape"], m["dtype"], m["params"])
     res, br = ctx.align_batch(movies)  # warm-up
     t0 = time.perf_counter()
@@ -219,6 +226,9 @@ def main():
     ap.add_argument("--frames", type=int, default=40)
     ap.add_argument("--no-batch", action="store_true", help="skip the batching form")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes, no autotuner")
+    ap.add_argument("--retune", action="store_true", help="autotune even if the table has an entry")
+    ap.add_argument("--tune-cache", default=os.path.join(ROOT, "motioncorr1exp_amd", "tune_mi355x.jsonl"),
+                    help="autotuner table loaded before timing (mc_tune_load)")
     ap.add_argument("--batch-movies", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--calib-copy", action="store_true",
@@ -262,11 +272,20 @@ def main():
 
     tune = None
     if not args.no_tune:
-        # launch shapes from the project's autotuner (untimed, on this movie)
-        t0 = time.time()
-        tune = ctx.autotune(dev, frames.shape, frames.dtype, params)
-        log("autotune in %.1fs: %s" % (time.time() - t0, {k: v for k, v in tune.items()
-                                                          if not k.startswith("ms_")}))
+        # launch shapes from the project's autotuner: the shipped table for
+        # this device when it has this geometry, else tuned now on this movie
+        # (untimed)
+        loaded = 0
+        if args.tune_cache and os.path.exists(args.tune_cache) and not args.retune:
+            loaded = ctx.tune_load(args.tune_cache)
+        if loaded:
+            tune = {"source": os.path.relpath(args.tune_cache, ROOT), "entries": loaded}
+        else:
+            t0 = time.time()
+            tune = ctx.autotune(dev, frames.shape, frames.dtype, params)
+            tune["source"] = "tuned in this run"
+            log("autotune in %.1fs" % (time.time() - t0))
+        log("launch shapes: %s" % {k: v for k, v in tune.items() if not k.startswith("ms_")})
 
     for _ in range(args.warmup):
         step(params)
@@ -386,7 +405,8 @@ def main():
     dev.free()
     sum_dev.free()
     if not args.no_batch:
-        out["batch"] = batch_leg(args.batch_movies, args.batch_steps, tune=not args.no_tune)
+        out["batch"] = batch_leg(args.batch_movies, args.batch_steps, tune=not args.no_tune,
+                                 tune_cache=None if args.retune else args.tune_cache)
     print(json.dumps(out), flush=True)
 
 

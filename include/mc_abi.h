@@ -235,9 +235,12 @@ int mc_align_batch(mc_context ctx, const mc_batch_item *items, int count,
  * with the same canvas size, binned size and element type (stage 1) and the
  * same binned size, peak box and precision form (pair stage):
  *   stage 1: row-pass shape (rows per workgroup x threads: 4x256, 2x256,
- *            4x512 for canvases >= 4096) x frames per launch (1, 2, 4, 8);
+ *            4x512 for canvases >= 4096) x frames per launch (1, 2, 4, 8)
+ *            x column pass (one workgroup per column, or persistent with
+ *            register twiddles);
  *   pair stage (FP32 form): column kernel (M=128, M=128 with next-item
- *            prefetch, M-generic) and row kernel (FP32 MFMA, vector ALU).
+ *            prefetch, M-generic, M=128 at 4 waves per SIMD) and row kernel
+ *            (FP32 MFMA, vector ALU).
  * Each candidate runs once untimed and 3 times timed (minimum kept).  With
  * cache_path, one JSON line per tuned geometry is appended
  * ({"arch", "cus", "nu", "n", "dtype", "box", "precision", "r2c", "chunk",
@@ -246,12 +249,15 @@ int mc_align_batch(mc_context ctx, const mc_batch_item *items, int count,
  * below 4096), 192 MB of row-pass scratch per launch, the M=128 column
  * kernel without prefetch and the FP32 MFMA row kernel. */
 typedef struct {
-  int r2c_variant;        /* 1: 4x256, 2: 2x256, 3: 4x512                   */
+  int r2c_variant;        /* 1: 4x256, 2: 2x256, 3: 4x512; +8: persistent
+                             column pass with register twiddles           */
   int chunk_frames;       /* frames per stage-1 launch                      */
-  int pair_cols_variant;  /* 0: M=128, 1: M=128 + prefetch, 2: M-generic    */
+  int pair_cols_variant;  /* 0: M=128, 1: M=128 + prefetch, 2: M-generic,
+                             3: M=128 at 4 waves per SIMD                 */
   int pair_rows_variant;  /* 0: FP32 MFMA, 1: vector ALU                    */
-  float ms_stage1[4][4];  /* [r2c variant][chunk 1, 2, 4, 8]; 0 = not run   */
-  float ms_pair_cols[3];  /* 0 = not run                                    */
+  float ms_stage1[2][4][4]; /* [column pass: per column, persistent]
+                               [row shape][chunk 1, 2, 4, 8]; 0 = not run  */
+  float ms_pair_cols[4];  /* 0 = not run                                    */
   float ms_pair_rows[2];
 } mc_tune_report;
 

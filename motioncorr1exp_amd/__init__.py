@@ -87,7 +87,7 @@ class BatchReport(ctypes.Structure):
 class TuneReport(ctypes.Structure):
     _fields_ = [("r2c_variant", ctypes.c_int), ("chunk_frames", ctypes.c_int),
                 ("pair_cols_variant", ctypes.c_int), ("pair_rows_variant", ctypes.c_int),
-                ("ms_stage1", (ctypes.c_float * 4) * 4), ("ms_pair_cols", ctypes.c_float * 3),
+                ("ms_stage1", ((ctypes.c_float * 4) * 4) * 2), ("ms_pair_cols", ctypes.c_float * 4),
                 ("ms_pair_rows", ctypes.c_float * 2)]
 
 
@@ -353,7 +353,7 @@ class Context:
         return {"r2c_variant": r.r2c_variant, "chunk_frames": r.chunk_frames,
                 "pair_cols_variant": r.pair_cols_variant,
                 "pair_rows_variant": r.pair_rows_variant,
-                "ms_stage1": [list(row) for row in r.ms_stage1],
+                "ms_stage1": [[list(row) for row in blk] for blk in r.ms_stage1],
                 "ms_pair_cols": list(r.ms_pair_cols), "ms_pair_rows": list(r.ms_pair_rows)}
 
     def tune_load(self, cache_path):

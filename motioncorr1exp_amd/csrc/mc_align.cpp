@@ -1432,23 +1432,25 @@ extern "C" int mc_autotune(mc_context ctx, const mc_movie_desc* m, const mc_para
   const int first = p->nStart - m->first_frame;
   float best = 1e30f;
   int bv = mc::R2C_AUTO, bc = 0;
-  for (int v = mc::R2C_R4_T256; v < mc::R2C_VARIANTS; v++) {
-    if (v == mc::R2C_R4_T512 && r.g.nsu < 4096) continue;
-    for (int k = 0; k < 4; k++) {
-      const int ch = kChunks[k];
-      if (ch > F && k > 0) continue;
-      float ms = 0;
-      MC_TRY(time_min(c, reps, &ms, [&] {
-        return run_stage1(c, m->frames, m, p, r.g, first, F, 0, v, ch);
-      }));
-      tr.ms_stage1[v][k] = ms;
-      if (ms < best) {
-        best = ms;
-        bv = v;
-        bc = ch;
+  for (int cp = 0; cp < 2; cp++)
+    for (int v = mc::R2C_R4_T256; v < mc::R2C_VARIANTS; v++) {
+      if (v == mc::R2C_R4_T512 && r.g.nsu < 4096) continue;
+      for (int k = 0; k < 4; k++) {
+        const int ch = kChunks[k];
+        if (ch > F && k > 0) continue;
+        const int vv = v | (cp ? mc::R2C_COLS_P : 0);
+        float ms = 0;
+        MC_TRY(time_min(c, reps, &ms, [&] {
+          return run_stage1(c, m->frames, m, p, r.g, first, F, 0, vv, ch);
+        }));
+        tr.ms_stage1[cp][v][k] = ms;
+        if (ms < best) {
+          best = ms;
+          bv = vv;
+          bc = ch;
+        }
       }
     }
-  }
   // ---- pair stage (spectra of the last stage-1 run are resident)
   std::vector<int> slots(r.pairs);
   MC_TRY(pair_reserve(c, r.P, n, box, p->precision));
@@ -1538,7 +1540,8 @@ extern "C" int mc_tune_load(mc_context ctx, const char* cache_path, int* loaded)
         json_int(line, "dtype", &t.dtype) && json_int(line, "box", &t.box) &&
         json_int(line, "precision", &t.precision) && json_int(line, "r2c", &t.r2c) &&
         json_int(line, "chunk", &t.chunk) && json_int(line, "cols", &t.cols) &&
-        json_int(line, "rows", &t.rows) && t.r2c >= 0 && t.r2c < mc::R2C_VARIANTS &&
+        json_int(line, "rows", &t.rows) && t.r2c >= 0 &&
+        (t.r2c & ~mc::R2C_COLS_P) < mc::R2C_VARIANTS &&
         t.cols >= 0 && t.cols < mc::PAIR_COLS_VARIANTS && t.rows >= 0 &&
         t.rows < mc::PAIR_ROWS_VARIANTS && t.chunk >= 0) {
       set_tune(ctx, t);  // later lines win

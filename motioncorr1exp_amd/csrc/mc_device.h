@@ -238,6 +238,97 @@ MC_DEV void fft_lds(float2* buf, const float2* tw, int ntw) {
   fft_lds_t<N, NT, S, NB, TS>(buf, TwTable{tw, ntw}, ntw);
 }
 
+// ------------------------------------- register-resident stage twiddles
+// For kernels whose threads keep the same butterflies from item to item
+// (persistent loops), the twiddles of stage STAGE are computed once per
+// thread and held in registers: w[p][r] = W_{Ns R}^{(j%Ns) r} for the
+// butterfly j of slot p.  Stage 0 (Ns = 1) has none.  A stage beyond the
+// plan is an empty placeholder.
+template <int N, int NT, int S, int NB, int STAGE, bool USED = (STAGE < stage_count(N))>
+struct StageTw {
+  static constexpr int R = 1, PER = 1;
+  float2 w[1][1];
+  template <class TW> MC_DEV void init(const TW&, int) {}
+};
+template <int N, int NT, int S, int NB, int STAGE>
+struct StageTw<N, NT, S, NB, STAGE, true> {
+  static constexpr int R = stage_radix(N, STAGE);
+  static constexpr int Ns = stage_ns(N, STAGE);
+  static constexpr int NBF = NB * (N / R);
+  static constexpr int PER = (NBF + NT - 1) / NT;
+  float2 w[PER][R];
+  template <class TW> MC_DEV void init(const TW& tw, int ntw) {
+#pragma unroll
+    for (int p = 0; p < PER; p++) {
+      const int j = (threadIdx.x + p * NT) % (N / R);
+      const int k = j % Ns;
+      const int step = ntw / (Ns * R);
+      w[p][0] = make_float2(1.f, 0.f);
+#pragma unroll
+      for (int r = 1; r < R; r++) w[p][r] = twv<S>(tw, k * r * step);
+    }
+  }
+};
+
+template <int N, int NT, int S, int NB, int STAGE, int TS, int PB, class WT>
+MC_DEV void fft_lds_stage_rw(float2* buf, const WT& tw) {
+  constexpr int R = stage_radix(N, STAGE);
+  constexpr int Ns = stage_ns(N, STAGE);
+  constexpr int NBF = NB * (N / R);
+  constexpr int PER = (NBF + NT - 1) / NT;
+  float2 v[PER][R];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int p = 0; p < PER; p++) {
+    const int q = tid + p * NT;
+    if (NBF % NT == 0 || q < NBF) {
+      const int b = q / (N / R), j = q % (N / R);
+#pragma unroll
+      for (int r = 0; r < R; r++) v[p][r] = buf[b * TS + lix<PB>(j + r * (N / R))];
+      if constexpr (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; r++) v[p][r] = cmul(v[p][r], tw.w[p][r]);
+      }
+      dft<R, S>(v[p]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < PER; p++) {
+    const int q = tid + p * NT;
+    if (NBF % NT == 0 || q < NBF) {
+      const int b = q / (N / R), j = q % (N / R);
+      const int o = stockham_out<R, Ns>(j);
+#pragma unroll
+      for (int r = 0; r < R; r++) buf[b * TS + lix<PB>(o + r * Ns)] = v[p][r];
+    }
+  }
+  __syncthreads();
+}
+
+// All twiddles of an N-point in-LDS FFT (NB transforms, NT threads) held in
+// registers.
+template <int N, int NT, int S, int NB>
+struct FftTw {
+  StageTw<N, NT, S, NB, 0> s0;  // placeholder-sized: stage 0 has no twiddles
+  StageTw<N, NT, S, NB, 1> s1;
+  StageTw<N, NT, S, NB, 2> s2;
+  StageTw<N, NT, S, NB, 3> s3;
+  template <class TW> MC_DEV void init(const TW& tw, int ntw) {
+    s1.init(tw, ntw);
+    s2.init(tw, ntw);
+    s3.init(tw, ntw);
+  }
+};
+
+template <int N, int NT, int S, int NB, int TS, int PB>
+MC_DEV void fft_lds_rw(float2* buf, const FftTw<N, NT, S, NB>& tw) {
+  fft_lds_stage_rw<N, NT, S, NB, 0, TS, PB>(buf, tw.s0);
+  if constexpr (stage_count(N) > 1) fft_lds_stage_rw<N, NT, S, NB, 1, TS, PB>(buf, tw.s1);
+  if constexpr (stage_count(N) > 2) fft_lds_stage_rw<N, NT, S, NB, 2, TS, PB>(buf, tw.s2);
+  if constexpr (stage_count(N) > 3) fft_lds_stage_rw<N, NT, S, NB, 3, TS, PB>(buf, tw.s3);
+}
+
 // --------------------------------------------------------- XCD remapping
 // gfx950 dispatches consecutive workgroups round-robin over 8 XCDs, each
 // with its own L2.  Remap so that a contiguous range of logical block ids

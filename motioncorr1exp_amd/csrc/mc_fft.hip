@@ -222,6 +222,64 @@ __global__ __launch_bounds__(NT) void k_c2c_cols(const float2* rt, size_t rt_fst
   }
 }
 
+// Persistent form of the column pass: each workgroup walks a contiguous
+// range of (frame, kx) columns owned by its XCD and keeps the FFT twiddles
+// of its threads in registers (no twiddle-table reads in the loop).
+template <int NU, int NT>
+__global__ __launch_bounds__(NT) void k_c2c_cols_p(const float2* rt, size_t rt_fstride, int nkx,
+                                                   int nitems, int h_data, const float* fill,
+                                                   const float2* __restrict__ padP, int n,
+                                                   float2* spec, size_t spec_stride,
+                                                   const float2* __restrict__ gtw) {
+  constexpr int PB = 4;
+  __shared__ float2 buf[lds_len<PB>(NU)];
+  FftTw<NU, NT, -1, 1> ftw;
+  ftw.init(TwTable{gtw, NU}, NU);
+  const int xcd = blockIdx.x % 8, lid = blockIdx.x / 8, nper = gridDim.x / 8;
+  const int i0 = (int)((long long)nitems * xcd / 8), i1 = (int)((long long)nitems * (xcd + 1) / 8);
+  for (int L = i0 + lid; L < i1; L += nper) {
+    const int kx = L % nkx, f = L / nkx;
+    const float2* src = rt + (size_t)f * rt_fstride + (size_t)(kx >> 2) * NU * 4 + (kx & 3);
+    float fv = 0.f;
+    float2 P = make_float2(0.f, 0.f);
+    if (fill) {
+      fv = fill[f];
+      P = padP[kx];
+    }
+    for (int i = threadIdx.x; i < NU; i += NT) {
+      float2 v = make_float2(0.f, 0.f);
+      if (i < h_data) {
+        v = src[(size_t)i * 4];
+        if (fill) v = cadd(v, cscale(P, fv));
+      } else if (fill && kx == 0) {
+        v.x = fv * (float)NU;
+      }
+      buf[lix<PB>(i)] = v;
+    }
+    __syncthreads();
+    fft_lds_rw<NU, NT, -1, 1, NU, PB>(buf, ftw);
+    float2* dst = spec + (size_t)f * spec_stride + (size_t)kx * n;
+    for (int i = threadIdx.x; i < n; i += NT) {
+      const int si = i < n / 2 ? i : NU - n + i;
+      dst[i] = buf[lix<PB>(si)];
+    }
+    __syncthreads();  // buf is rewritten by the next column
+  }
+}
+
+template <class K>
+static int resident_grid(K kernel, int nt, int nitems) {
+  int dev = 0, cus = 256, per = 1;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, nt, 0) != hipSuccess || per < 1)
+    per = 1;
+  int g = (cus * per) / 8 * 8;
+  const int need = (nitems + 7) / 8 * 8;
+  if (g > need) g = need;
+  return g < 8 ? 8 : g;
+}
+
 template <int NU, int ROWS, int NT>
 static hipError_t launch_rows(const R2CGeom& g, int f0, int nframes, int nkx, float2* rt,
                               size_t rt_fstride, float* rs, const float2* tw, hipStream_t st) {
@@ -264,6 +322,9 @@ static hipError_t frames_to_spectra_t(const R2CGeom& g, int f0, int nframes, flo
     gv.vec = ((uintptr_t)g.src % a == 0 && g.row_pitch % a == 0 && g.frame_pitch % a == 0 &&
               (g.offx & 1) == 0) ? 1 : 0;
   }
+  // bit R2C_COLS_P selects the persistent column pass
+  const bool cols_persistent = (variant & R2C_COLS_P) != 0;
+  variant &= ~R2C_COLS_P;
   // row-pass shape: 4 rows x 512 threads on canvases >= 4096 by default
   if (variant == R2C_AUTO) variant = (4 * N / 16 >= 512) ? R2C_R4_T512 : R2C_R4_T256;
   if (variant == R2C_R4_T512 && 4 * N / 16 < 512) variant = R2C_R4_T256;
@@ -285,9 +346,17 @@ static hipError_t frames_to_spectra_t(const R2CGeom& g, int f0, int nframes, flo
   if (g.pad_mode) {
     hipLaunchKernelGGL(k_fill, dim3(nframes), dim3(256), 0, st, rowsum, NU, g.h, g.w, fill);
   }
-  hipLaunchKernelGGL((k_c2c_cols<NU, NTC>), dim3(nkx * nframes), dim3(NTC), 0, st, rt,
-                     rt_fstride, nkx, g.h, g.pad_mode ? fill : nullptr, padP, g.n, spectra,
-                     spec_stride, tw);
+  if (cols_persistent) {
+    const int nitems = nkx * nframes;
+    const int grid = resident_grid(k_c2c_cols_p<NU, NTC>, NTC, nitems);
+    hipLaunchKernelGGL((k_c2c_cols_p<NU, NTC>), dim3(grid), dim3(NTC), 0, st, rt, rt_fstride, nkx,
+                       nitems, g.h, g.pad_mode ? fill : nullptr, padP, g.n, spectra, spec_stride,
+                       tw);
+  } else {
+    hipLaunchKernelGGL((k_c2c_cols<NU, NTC>), dim3(nkx * nframes), dim3(NTC), 0, st, rt,
+                       rt_fstride, nkx, g.h, g.pad_mode ? fill : nullptr, padP, g.n, spectra,
+                       spec_stride, tw);
+  }
   return hipGetLastError();
 }
 

@@ -37,7 +37,10 @@ const float2* twiddles(int n);
 // (nframes*nu, nframes).
 // Row-pass shapes (rows per workgroup x threads); R2C_AUTO picks 4 x 512 on
 // canvases >= 4096 and 4 x min(256, 2nu/16) below.
-enum { R2C_AUTO = 0, R2C_R4_T256 = 1, R2C_R2_T256 = 2, R2C_R4_T512 = 3, R2C_VARIANTS = 4 };
+// R2C_COLS_P (or-ed in) selects the persistent column pass with register
+// twiddles instead of one workgroup per column.
+enum { R2C_AUTO = 0, R2C_R4_T256 = 1, R2C_R2_T256 = 2, R2C_R4_T512 = 3, R2C_VARIANTS = 4,
+       R2C_COLS_P = 8 };
 hipError_t launch_frames_to_spectra(const R2CGeom& g, int f0, int nframes, float2* spectra,
                                     size_t spec_stride, float2* rt, float* rowsum,
                                     float* fill, const float2* padP, int variant, hipStream_t st);
@@ -58,9 +61,11 @@ hipError_t launch_spectrum_to_image(const float2* spec, int n, float scale, floa
 // Plain-form variants (the hi/lo form has one kernel per pass):
 //   column pass: AUTO = the M = 128 persistent kernel (n >= 512, box <= 128),
 //   M128_PF = the same with next-item register prefetch (fewer resident
-//   workgroups), GENERIC = the M-generic persistent kernel; row pass: AUTO =
-//   FP32 MFMA (box <= 96), VALU = register-tiled vector ALU kernel.
-enum { PAIR_COLS_AUTO = 0, PAIR_COLS_M128_PF = 1, PAIR_COLS_GENERIC = 2, PAIR_COLS_VARIANTS = 3 };
+//   workgroups), GENERIC = the M-generic persistent kernel, M128_W4 = M = 128
+//   compiled for 4 waves per SIMD (128 VGPRs); row pass: AUTO = FP32 MFMA
+//   (box <= 96), VALU = register-tiled vector ALU kernel.
+enum { PAIR_COLS_AUTO = 0, PAIR_COLS_M128_PF = 1, PAIR_COLS_GENERIC = 2, PAIR_COLS_M128_W4 = 3,
+       PAIR_COLS_VARIANTS = 4 };
 enum { PAIR_ROWS_AUTO = 0, PAIR_ROWS_VALU = 1, PAIR_ROWS_VARIANTS = 2 };
 bool pair_form_supported(int precision, int n, int box);
 hipError_t launch_pair_cols(const float2* spectra, size_t spec_stride, const int2* pairs,

@@ -134,8 +134,8 @@ __global__ __launch_bounds__(NT) void k_pair_cols(const float2* __restrict__ spe
 //    only on tid % 16 and stay in registers;
 //  combine: SP adjacent lanes share one output row and reduce with xor
 //    shuffles (no LDS round trip).
-template <int N, int SP, bool PF>
-__global__ __launch_bounds__(256) void k_pair_cols_m128(const float2* __restrict__ spectra,
+template <int N, int SP, bool PF, int MINW>
+__global__ __launch_bounds__(256, (N <= 4096 ? MINW : 1)) void k_pair_cols_m128(const float2* __restrict__ spectra,
                                                         size_t spec_stride,
                                                         const int2* __restrict__ pairs,
                                                         int npairs,
@@ -516,7 +516,7 @@ __global__ __launch_bounds__(256) void k_pair_rows(const float2* __restrict__ tb
 template <int NB>
 __global__ __launch_bounds__(256) void k_pair_rows_mfma(const float2* __restrict__ tbuf, int n,
                                                         int box, float* __restrict__ boxmaps,
-                                                        const float2* __restrict__ tw) {
+                                                        const float2* __restrict__ etab) {
   constexpr int W = 32 * NB;
   constexpr int KC = 32;
   constexpr int PER = (KC * W + 255) / 256;
@@ -524,7 +524,6 @@ __global__ __launch_bounds__(256) void k_pair_rows_mfma(const float2* __restrict
   __shared__ float2 Es[2][KC][W];
   const int p = blockIdx.x;
   const int H = n / 2 + 1;
-  const int ori = (n - box) / 2;
   const float2* T = tbuf + (size_t)p * H * box;
   const int wv = threadIdx.x / 64, l = threadIdx.x % 64;
   const int li = l % 32, lh = l / 32;
@@ -545,7 +544,7 @@ __global__ __launch_bounds__(256) void k_pair_rows_mfma(const float2* __restrict
       if (i < KC * W && kx < H && c < box) {
         const float al = (kx == 0 || 2 * kx == n) ? 1.f : 2.f;
         t = cscale(T[(size_t)kx * box + c], al);
-        e = tw[(int)(((long long)kx * (ori + c)) & (n - 1))];
+        e = etab[(size_t)kx * box + c];
       }
       pt[q] = t;
       pe[q] = e;
@@ -619,7 +618,7 @@ template <int NB>
 __global__ __launch_bounds__(256) void k_pair_rows_mfma16(const float2* __restrict__ tbuf,
                                                           const float* __restrict__ tmax, int n,
                                                           int box, float* __restrict__ boxmaps,
-                                                          const float2* __restrict__ tw) {
+                                                          const float2* __restrict__ etab) {
   constexpr int W = 32 * NB;
   constexpr int KC = 32;
   constexpr int PER = (KC * W + 255) / 256;
@@ -631,7 +630,6 @@ __global__ __launch_bounds__(256) void k_pair_rows_mfma16(const float2* __restri
   auto El = S4[3];
   const int p = blockIdx.x;
   const int H = n / 2 + 1;
-  const int ori = (n - box) / 2;
   const float2* T = tbuf + (size_t)p * H * box;
   const int wv = threadIdx.x / 64, l = threadIdx.x % 64;
   const int li = l % 32, lh = l / 32;
@@ -664,7 +662,7 @@ __global__ __launch_bounds__(256) void k_pair_rows_mfma16(const float2* __restri
         const float al = (kx == 0 || 2 * kx == n) ? 1.f : 2.f;
         t = T[(size_t)kx * box + c];
         t = make_float2(ldexpf(al * t.x, sh), ldexpf(al * t.y, sh));
-        e = tw[(int)(((long long)kx * (ori + c)) & (n - 1))];
+        e = etab[(size_t)kx * box + c];
         e.y = -e.y;
       }
       pt[q] = t;
@@ -775,6 +773,36 @@ static const float* weights(int n, float bfactor) {
   return d;
 }
 
+// Row-pass twiddles of a box: etab[kx * box + c] = e^{2 pi i kx R_c/n},
+// R_c = (n - box)/2 + c, kx in [0, n/2] (double on the host, the same values
+// as the n-point table), so the MFMA row kernels stream them like T.
+static const float2* row_twiddles(int n, int box) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int>, float2*> cache;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(mu);
+  auto key = std::make_tuple(dev, n, box);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  const int H = n / 2 + 1, ori = (n - box) / 2;
+  std::vector<float2> h((size_t)H * box);
+  for (int kx = 0; kx < H; kx++)
+    for (int c = 0; c < box; c++) {
+      const long long e = ((long long)kx * (ori + c)) & (n - 1);
+      const double a = 2.0 * M_PI * (double)e / (double)n;
+      h[(size_t)kx * box + c] = make_float2((float)cos(a), (float)sin(a));
+    }
+  float2* d = nullptr;
+  if (hipMalloc(&d, sizeof(float2) * h.size()) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    return nullptr;
+  }
+  cache[key] = d;
+  return d;
+}
+
 // Persistent grid: every workgroup resident at once (the item ranges are
 // static), a multiple of 8 so that each XCD gets the same number.
 template <class K>
@@ -802,17 +830,21 @@ static hipError_t pair_cols_t(const float2* spectra, size_t spec_stride, const i
 
 template <int N, int SP>
 static hipError_t pair_cols_m128(const float2* spectra, size_t spec_stride, const int2* pairs,
-                                 int npairs, const float* wxy, int box, float2* tbuf, bool pf,
+                                 int npairs, const float* wxy, int box, float2* tbuf, int mode,
                                  hipStream_t st) {
   const float2* tw = twiddles(N);
   if (!tw) return hipErrorOutOfMemory;
-  static const int g1 = persistent_grid(k_pair_cols_m128<N, SP, true>, 256);
-  static const int g0 = persistent_grid(k_pair_cols_m128<N, SP, false>, 256);
-  if (pf)
-    hipLaunchKernelGGL((k_pair_cols_m128<N, SP, true>), dim3(g1), dim3(256), 0, st, spectra,
+  static const int g1 = persistent_grid(k_pair_cols_m128<N, SP, true, 1>, 256);
+  static const int g0 = persistent_grid(k_pair_cols_m128<N, SP, false, 1>, 256);
+  static const int g4 = persistent_grid(k_pair_cols_m128<N, SP, false, 4>, 256);
+  if (mode == 1)
+    hipLaunchKernelGGL((k_pair_cols_m128<N, SP, true, 1>), dim3(g1), dim3(256), 0, st, spectra,
+                       spec_stride, pairs, npairs, wxy, box, tbuf, tw);
+  else if (mode == 2)
+    hipLaunchKernelGGL((k_pair_cols_m128<N, SP, false, 4>), dim3(g4), dim3(256), 0, st, spectra,
                        spec_stride, pairs, npairs, wxy, box, tbuf, tw);
   else
-    hipLaunchKernelGGL((k_pair_cols_m128<N, SP, false>), dim3(g0), dim3(256), 0, st, spectra,
+    hipLaunchKernelGGL((k_pair_cols_m128<N, SP, false, 1>), dim3(g0), dim3(256), 0, st, spectra,
                        spec_stride, pairs, npairs, wxy, box, tbuf, tw);
   return hipGetLastError();
 }
@@ -835,11 +867,11 @@ static hipError_t pair_cols_n(const float2* spectra, size_t spec_stride, const i
                               hipStream_t st) {
   if constexpr (N >= 512) {
     if (variant != PAIR_COLS_GENERIC) {
-      const bool pf = variant == PAIR_COLS_M128_PF;
+      const int mode = variant == PAIR_COLS_M128_PF ? 1 : (variant == PAIR_COLS_M128_W4 ? 2 : 0);
       if (box <= 64)
-        return pair_cols_m128<N, 4>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, pf, st);
+        return pair_cols_m128<N, 4>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, mode, st);
       if (box <= 128)
-        return pair_cols_m128<N, 2>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, pf, st);
+        return pair_cols_m128<N, 2>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, mode, st);
     }
   }
   if constexpr (N / 128 >= 4) {
@@ -908,11 +940,13 @@ hipError_t launch_pair_rows(const float2* tbuf, const float* tmax, int npairs, i
   if (!pair_form_supported(precision, n, box)) return hipErrorInvalidValue;
   const float2* tw = twiddles(n);
   if (!tw) return hipErrorOutOfMemory;
+  const float2* etab = (precision == 1 || variant != PAIR_ROWS_VALU) ? row_twiddles(n, box) : nullptr;
+  if ((precision == 1 || variant != PAIR_ROWS_VALU) && !etab) return hipErrorOutOfMemory;
   if (precision == 1) {
     if (!tmax) return hipErrorInvalidValue;
     switch ((box + 31) / 32) {
 #define MC_ROWS_MFMA16(B) \
-  case B: hipLaunchKernelGGL(k_pair_rows_mfma16<B>, dim3(npairs), dim3(256), 0, st, tbuf, tmax, n, box, boxmaps, tw); return hipGetLastError();
+  case B: hipLaunchKernelGGL(k_pair_rows_mfma16<B>, dim3(npairs), dim3(256), 0, st, tbuf, tmax, n, box, boxmaps, etab); return hipGetLastError();
       MC_ROWS_MFMA16(1) MC_ROWS_MFMA16(2) MC_ROWS_MFMA16(3)
 #undef MC_ROWS_MFMA16
       default: return hipErrorInvalidValue;
@@ -921,7 +955,7 @@ hipError_t launch_pair_rows(const float2* tbuf, const float* tmax, int npairs, i
   if (variant != PAIR_ROWS_VALU) {
     switch ((box + 31) / 32) {
 #define MC_ROWS_MFMA(B) \
-  case B: hipLaunchKernelGGL(k_pair_rows_mfma<B>, dim3(npairs), dim3(256), 0, st, tbuf, n, box, boxmaps, tw); return hipGetLastError();
+  case B: hipLaunchKernelGGL(k_pair_rows_mfma<B>, dim3(npairs), dim3(256), 0, st, tbuf, n, box, boxmaps, etab); return hipGetLastError();
       MC_ROWS_MFMA(1) MC_ROWS_MFMA(2) MC_ROWS_MFMA(3)
 #undef MC_ROWS_MFMA
       default: break;

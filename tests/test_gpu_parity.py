@@ -178,3 +178,33 @@ def test_drift_recovered(ctx):
     expect = drift - drift[0]
     expect[0] = 0
     assert np.abs(fs[1:] - expect[1:]).max() < 0.15
+
+
+@pytest.mark.parametrize("r2c,chunk", [(1, 1), (2, 2), (1 | 8, 4), (2 | 8, 8)])
+def test_spectra_match_oracle_every_launch_shape(cfg1, r2c, chunk, tmp_path):
+    """Every stage-1 launch shape the autotuner can pick (row shape 4x256 /
+    2x256, per-column or persistent column pass, frames per launch) gives
+    the FP64 oracle's spectra to the same bar.  The shape is forced through
+    a tuning table: mc_autotune writes this device's line, which is then
+    rewritten with the shape under test and loaded into a fresh context."""
+    import json
+
+    frames, drift, o = cfg1
+    d = mc.DeviceArray(frames)
+    params = mc.default_params(10, FrameDistOffset=4)
+    c0 = mc.Context(0)
+    path = str(tmp_path / "t.jsonl")
+    c0.autotune(d, frames.shape, frames.dtype, params, cache_path=path)
+    c0.close()
+    e = json.loads(open(path).read().splitlines()[0])
+    e["r2c"], e["chunk"] = r2c, chunk
+    with open(path, "w") as f:
+        f.write(json.dumps(e) + "\n")
+    c = mc.Context(0)
+    assert c.tune_load(path) == 1
+    c.load_frames(d, frames.shape, frames.dtype, params)
+    spec = c.get_spectra(10, 512).view(np.complex64)
+    c.close()
+    ref = o["spectra"].view(np.complex128)
+    err = np.abs(spec - ref).max() / np.abs(ref).max()
+    assert err < 5e-7, err
