@@ -110,6 +110,8 @@ def cpu_baseline(frames_shape, pad, P, F, nthreads, budget_s=20.0):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_3838x3710x40_pad4096.txt")
+# canvas sizes of the batching form: the supported transform sizes 2^k and 3 * 2^k
+PAD_SIZES = (1024, 1536, 2048, 3072, 4096)
 
 
 def committed_traffic(group):
@@ -133,8 +135,8 @@ def committed_traffic(group):
 
 def batch_leg(n_movies, steps, seed=1004, tune=True, tune_cache=None):
     """Batching form: n_movies movies of seeded uneven size (square sides
-    uniform over the even values 1024..4096, padded to the next power of two
-    with the region mean), 8..50 frames, dose uniform in [0.5, 1.5],
+    uniform over the even values 1024..4096, padded with the region mean to
+    the next transform size of 1024, 1536, 2048, 3072, 4096), 8..50 frames, dose uniform in [0.5, 1.5],
     random-walk drift N(0, 0.7^2), uint8 counts generated on the device;
     bin 1, fod 2, bft 150, pbx 96, 3 align-to-sum rounds.  Times
     mc_align_batch (one call = one step over all movies) and the same movies
@@ -152,7 +154,7 @@ def batch_leg(n_movies, steps, seed=1004, tune=True, tune_cache=None):
         side = int(rng.integers(512, 2049)) * 2
         F = int(rng.integers(8, 51))
         dose = float(rng.uniform(0.5, 1.5))
-        pad = 1 << (side - 1).bit_length()
+        pad = min(z for z in PAD_SIZES if z >= side)
         dev, drift = make_movie_gpu(side, side, F, dose=dose, step=0.7, seed=seed * 100 + i)
         prm = mc.default_params(F, pad=pad, bin=1.0, FrameDistOffset=2, bfactor=150.0,
                                 CCPeakSearchDim=96, iter=3)
@@ -206,7 +208,7 @@ def batch_leg(n_movies, steps, seed=1004, tune=True, tune_cache=None):
            "movies_per_transform_size": {str(k): v for k, v in sorted(sizes.items())},
            "batch_report": br, "max_shift_diff_batch_vs_single_px": same,
            "vs_truth_rms_px": float(np.sqrt(np.mean(err ** 2))),
-           "workload": "%d movies, sides U{1024..4096} even padded to 2^k, frames U{8..50}, "
+           "workload": "%d movies, sides U{1024..4096} even padded to 2^k or 3*2^k, frames U{8..50}, "
                        "dose U[0.5,1.5], uint8 counts, bin 1, fod 2, bft 150, pbx 96, itr 3"
                        % n_movies}
     for m in movies:
@@ -236,8 +238,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU per box; --gpus %d is not supported "
-              "(movies are independent: multi-GPU is movie-level data parallelism, "
-              "see DESIGN.md)" % args.gpus, file=sys.stderr)
+              "(movies are independent: several GPUs would each take whole movies from "
+              "a shared queue)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -72,7 +72,7 @@ int mc_mem_d2d(void *dst, const void *src, size_t bytes);
 int mc_mem_bin_d2d(float *dst, const float *src, int dst_n, int src_n);
 int mc_mem_bin_d2h(float *dst_host, const float *src, int dst_n, int src_n);
 /* GPUFFTPlan / GPUIFFTPlan (cufunc.cu:251-264): n x n in-place plans.
- * Supported n: powers of two from 64 to 8192. */
+ * Supported n: powers of two from 64 to 8192, and 768, 1536, 3072, 6144. */
 int mc_fft_plan(mc_plan *plan, int n);
 int mc_ifft_plan(mc_plan *plan, int n);
 /* GPUFFT2d / GPUIFFT2d (cufunc.cu:272-282): unnormalised, in place. */
@@ -240,7 +240,7 @@ int mc_align_batch(mc_context ctx, const mc_batch_item *items, int count,
  *            register twiddles);
  *   pair stage (FP32 form): column kernel (M=128, M=128 with next-item
  *            prefetch, M-generic, M=128 at 4 waves per SIMD) and row kernel
- *            (FP32 MFMA, vector ALU).
+ *            (FP32 MFMA with 4 or 8 waves per pair, vector ALU).
  * Each candidate runs once untimed and 3 times timed (minimum kept).  With
  * cache_path, one JSON line per tuned geometry is appended
  * ({"arch", "cus", "nu", "n", "dtype", "box", "precision", "r2c", "chunk",
@@ -254,11 +254,12 @@ typedef struct {
   int chunk_frames;       /* frames per stage-1 launch                      */
   int pair_cols_variant;  /* 0: M=128, 1: M=128 + prefetch, 2: M-generic,
                              3: M=128 at 4 waves per SIMD                 */
-  int pair_rows_variant;  /* 0: FP32 MFMA, 1: vector ALU                    */
+  int pair_rows_variant;  /* 0: FP32 MFMA (4 waves), 1: vector ALU,
+                             2: FP32 MFMA (8 waves)                        */
   float ms_stage1[2][4][4]; /* [column pass: per column, persistent]
                                [row shape][chunk 1, 2, 4, 8]; 0 = not run  */
   float ms_pair_cols[4];  /* 0 = not run                                    */
-  float ms_pair_rows[2];
+  float ms_pair_rows[3];
 } mc_tune_report;
 
 int mc_autotune(mc_context ctx, const mc_movie_desc *movie, const mc_params *p,

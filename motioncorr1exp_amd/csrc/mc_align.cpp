@@ -59,7 +59,10 @@ extern "C" const char* mc_last_error(void) { return g_err.c_str(); }
 extern "C" const char* mc_version(void) { return "motioncorr1exp_amd 0.1 (gfx950)"; }
 
 static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
-static bool fft_size_ok(int n) { return is_pow2(n) && n >= 64 && n <= 8192; }
+// Transform sizes: powers of two in [64, 8192] and 3 * 2^k in [768, 6144].
+static bool fft_size_ok(int n) {
+  return (is_pow2(n) && n >= 64 && n <= 8192) || n == 768 || n == 1536 || n == 3072 || n == 6144;
+}
 
 // ------------------------------------------------------- device buffers
 struct DevBuf {
@@ -90,7 +93,7 @@ struct mc_context_s {
   int device = 0;
   hipStream_t st = nullptr;
   DevBuf spec, rt, rowsum, fill, padP, pairs_d, tbuf, box, peaks, shifts, image, tmp, means,
-      sum2img, wk, gpart, gout, jobs, tmax;
+      sum2img, wk, gpart, gout, jobs, tmax, rpart;
   int graph_captures = 0, graph_replays = 0;
   // autotuned launch shapes (mc_autotune / mc_tune_load)
   struct Tune {
@@ -165,7 +168,7 @@ extern "C" int mc_context_destroy(mc_context ctx) {
   DevBuf* bufs[] = {&ctx->spec, &ctx->rt, &ctx->rowsum, &ctx->fill, &ctx->padP,
                     &ctx->pairs_d, &ctx->tbuf, &ctx->box, &ctx->peaks, &ctx->shifts,
                     &ctx->image, &ctx->tmp, &ctx->means, &ctx->sum2img, &ctx->wk,
-                    &ctx->gpart, &ctx->gout, &ctx->jobs, &ctx->tmax};
+                    &ctx->gpart, &ctx->gout, &ctx->jobs, &ctx->tmax, &ctx->rpart};
   for (DevBuf* b : bufs) b->release();
   for (hipEvent_t e : ctx->pool) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->st);
@@ -254,7 +257,7 @@ static int movie_geometry(const mc_movie_desc* m, const mc_params* p, Geom* g) {
   if (!fft_size_ok(g->nsu) || !fft_size_ok(g->n))
     return set_err(MC_ERR_ARG,
                    "transform size %d (binned %d) unsupported: sizes must be powers of two "
-                   "in [64, 8192]; use the pad geometry",
+                   "in [64, 8192] or 768, 1536, 3072, 6144; use the pad geometry",
                    g->nsu, g->n);
   return MC_OK;
 }
@@ -496,6 +499,8 @@ static int pair_reserve(mc_context_s* c, int np, int n, int box, int precision) 
   HIP_TRY(c->pairs_d.ensure(sizeof(int) * 2 * (size_t)np));
   HIP_TRY(c->tbuf.ensure(sizeof(float2) * (size_t)np * H * box));
   if (precision == MC_PREC_MFMA_HILO) HIP_TRY(c->tmax.ensure(sizeof(float) * (size_t)np * H));
+  const int ns = mc::pair_rows_split(n);
+  if (ns > 1) HIP_TRY(c->rpart.ensure(sizeof(float) * (size_t)ns * np * box * box));
   HIP_TRY(c->box.ensure(sizeof(float) * (size_t)np * box * box));
   HIP_TRY(c->peaks.ensure(sizeof(double) * 3 * (size_t)np));
   return MC_OK;
@@ -517,7 +522,9 @@ static int pair_launch(mc_context_s* c, int np, int n, float bfactor, int box, i
   {
     KScope ks(c, MC_KT_PAIR_ROWS);
     HIP_TRY(mc::launch_pair_rows(c->tbuf.as<float2>(), c->tmax.as<float>(), np, n, box,
-                                 c->box.as<float>(), precision, rows, c->st));
+                                 c->box.as<float>(), precision, rows,
+                                 c->rpart.as<float>(),
+                                 c->st));
   }
   {
     KScope ks(c, MC_KT_PEAKS);
@@ -1477,12 +1484,13 @@ extern "C" int mc_autotune(mc_context ctx, const mc_movie_desc* m, const mc_para
     }
     best = 1e30f;
     for (int v = 0; v < mc::PAIR_ROWS_VARIANTS; v++) {
-      if (v == mc::PAIR_ROWS_AUTO && box > 96) continue;
+      if (v != mc::PAIR_ROWS_VALU && box > 96) continue;
       float ms = 0;
       MC_TRY(time_min(c, reps, &ms, [&] {
-        const hipError_t e = mc::launch_pair_rows(c->tbuf.as<float2>(), c->tmax.as<float>(), r.P,
-                                                  n, box, c->box.as<float>(), p->precision, v,
-                                                  c->st);
+        const hipError_t e = mc::launch_pair_rows(
+            c->tbuf.as<float2>(), c->tmax.as<float>(), r.P, n, box, c->box.as<float>(),
+            p->precision, v, c->rpart.as<float>(),
+            c->st);
         return e == hipSuccess ? MC_OK : set_err(MC_ERR_DEVICE, "pair rows: %s", hipGetErrorString(e));
       }));
       tr.ms_pair_rows[v] = ms;

@@ -107,21 +107,36 @@ template <int S> MC_DEV void dft16(float2* v) {
   }
 }
 
+// X[k] = sum_n v[n] W3^{S n k}: W3 = -1/2 + S i sqrt(3)/2.
+template <int S> MC_DEV void dft3(float2* v) {
+  const float h = 0.86602540378443864676f;  // sqrt(3)/2
+  const float2 a = v[0], b = v[1], c = v[2];
+  const float2 s = cadd(b, c), d = csub(b, c);
+  const float2 m = make_float2(a.x - 0.5f * s.x, a.y - 0.5f * s.y);
+  const float2 r = make_float2(-S * h * d.y, S * h * d.x);  // S i h (b - c)
+  v[0] = cadd(a, s);
+  v[1] = cadd(m, r);
+  v[2] = csub(m, r);
+}
+
 template <int R, int S> MC_DEV void dft(float2* v) {
   if constexpr (R == 2) dft2<S>(v);
+  else if constexpr (R == 3) dft3<S>(v);
   else if constexpr (R == 4) dft4<S>(v);
   else if constexpr (R == 8) dft8<S>(v);
   else if constexpr (R == 16) dft16<S>(v);
-  else static_assert(R == 2 || R == 4 || R == 8 || R == 16, "radix");
+  else static_assert(R == 2 || R == 3 || R == 4 || R == 8 || R == 16, "radix");
 }
 
 // ---------------------------------------------------- Stockham stage plans
-// Radix of stage s for a power-of-two length N (balanced plans).
+// Lengths N = 2^k or 3 * 2^k.  Power-of-two plans are balanced; a length
+// 3 * 2^k runs the plan of 2^k followed by one radix-3 stage.
 __host__ __device__ constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
-__host__ __device__ constexpr int stage_count(int N) {
+__host__ __device__ constexpr bool is_pow2c(int n) { return n > 0 && (n & (n - 1)) == 0; }
+__host__ __device__ constexpr int pow2_stage_count(int N) {
   return N <= 16 ? 1 : (N == 32 || N == 64 || N == 128 || N == 256) ? 2 : (N <= 4096 ? 3 : 4);
 }
-__host__ __device__ constexpr int stage_radix(int N, int s) {
+__host__ __device__ constexpr int pow2_stage_radix(int N, int s) {
   // 16:[16] 32:[8,4] 64:[8,8] 128:[16,8] 256:[16,16] 512:[8,8,8]
   // 1024:[16,8,8] 2048:[16,16,8] 4096:[16,16,16] 8192:[16,16,8,4]
   return N <= 16 ? N
@@ -135,6 +150,15 @@ __host__ __device__ constexpr int stage_radix(int N, int s) {
        : N == 4096 ? 16
        : (s < 2 ? 16 : (s == 2 ? 8 : 4));
 }
+__host__ __device__ constexpr int stage_count(int N) {
+  return is_pow2c(N) ? pow2_stage_count(N) : pow2_stage_count(N / 3) + 1;
+}
+__host__ __device__ constexpr int stage_radix(int N, int s) {
+  return is_pow2c(N) ? pow2_stage_radix(N, s)
+                     : (s < pow2_stage_count(N / 3) ? pow2_stage_radix(N / 3, s) : 3);
+}
+// Table index modulo the table size (any size; a power of two masks).
+MC_DEV int tw_mod(int e, int ntw) { return (ntw & (ntw - 1)) == 0 ? (e & (ntw - 1)) : (e % ntw); }
 __host__ __device__ constexpr int stage_ns(int N, int s) {
   return s == 0 ? 1 : stage_ns(N, s - 1) * stage_radix(N, s - 1);
 }
@@ -143,7 +167,7 @@ __host__ __device__ constexpr int stage_ns(int N, int s) {
 struct TwTable {  // full table (global or LDS)
   const float2* t;
   int ntw;
-  MC_DEV float2 operator()(int e) const { return t[e & (ntw - 1)]; }
+  MC_DEV float2 operator()(int e) const { return t[tw_mod(e, ntw)]; }
 };
 // Two-level table in LDS: w(e) = lo[e & (L-1)] * hi[e >> log2 L].  With
 // correctly rounded entries the product is within ~1.5 ulp.
@@ -152,7 +176,7 @@ struct TwSplit {
   const float2* hi;
   int lbits, ntw;
   MC_DEV float2 operator()(int e) const {
-    e &= ntw - 1;
+    e = tw_mod(e, ntw);
     return cmul(lo[e & ((1 << lbits) - 1)], hi[e >> lbits]);
   }
 };

@@ -127,7 +127,7 @@ __global__ __launch_bounds__(NT) void k_r2c_rows(R2CGeom g, int f0, int nkx, flo
   for (int i = threadIdx.x; i < ROWS * NP; i += NT) {
     const int r = i / NP, k = i % NP;
     float2* Z = buf + r * TS;
-    const float2 zk = Z[lix<PB>(k)], zm = Z[lix<PB>((N - k) & (N - 1))];
+    const float2 zk = Z[lix<PB>(k)], zm = Z[lix<PB>(k ? N - k : 0)];
     // bin k
     {
       const float2 zc = cconj(zm);
@@ -366,6 +366,7 @@ hipError_t launch_frames_to_spectra(const R2CGeom& g, int f0, int nframes, float
   switch (g.nu) {
 #define MC_F2S(NU) case NU: return frames_to_spectra_t<NU>(g, f0, nframes, spectra, spec_stride, rt, rowsum, fill, padP, variant, st);
     MC_F2S(64) MC_F2S(128) MC_F2S(256) MC_F2S(512) MC_F2S(1024) MC_F2S(2048) MC_F2S(4096) MC_F2S(8192)
+    MC_F2S(768) MC_F2S(1536) MC_F2S(3072) MC_F2S(6144)
 #undef MC_F2S
     default: return hipErrorInvalidValue;
   }
@@ -464,6 +465,10 @@ hipError_t launch_spectrum_to_image(const float2* spec, int n, float scale, floa
     case 2048: return spectrum_to_image_t<2048>(spec, scale, img, img_pitch, tmp, st);
     case 4096: return spectrum_to_image_t<4096>(spec, scale, img, img_pitch, tmp, st);
     case 8192: return spectrum_to_image_t<8192>(spec, scale, img, img_pitch, tmp, st);
+    case 768: return spectrum_to_image_t<768>(spec, scale, img, img_pitch, tmp, st);
+    case 1536: return spectrum_to_image_t<1536>(spec, scale, img, img_pitch, tmp, st);
+    case 3072: return spectrum_to_image_t<3072>(spec, scale, img, img_pitch, tmp, st);
+    case 6144: return spectrum_to_image_t<6144>(spec, scale, img, img_pitch, tmp, st);
     default: return hipErrorInvalidValue;
   }
 }

@@ -63,16 +63,21 @@ hipError_t launch_spectrum_to_image(const float2* spec, int n, float scale, floa
 //   M128_PF = the same with next-item register prefetch (fewer resident
 //   workgroups), GENERIC = the M-generic persistent kernel, M128_W4 = M = 128
 //   compiled for 4 waves per SIMD (128 VGPRs); row pass: AUTO = FP32 MFMA
-//   (box <= 96), VALU = register-tiled vector ALU kernel.
+//   with 4 waves per pair (box <= 96), VALU = register-tiled vector ALU
+//   kernel, MFMA8 = FP32 MFMA with 8 waves per pair.
 enum { PAIR_COLS_AUTO = 0, PAIR_COLS_M128_PF = 1, PAIR_COLS_GENERIC = 2, PAIR_COLS_M128_W4 = 3,
        PAIR_COLS_VARIANTS = 4 };
-enum { PAIR_ROWS_AUTO = 0, PAIR_ROWS_VALU = 1, PAIR_ROWS_VARIANTS = 2 };
+enum { PAIR_ROWS_AUTO = 0, PAIR_ROWS_VALU = 1, PAIR_ROWS_MFMA8 = 2, PAIR_ROWS_VARIANTS = 3 };
 bool pair_form_supported(int precision, int n, int box);
 hipError_t launch_pair_cols(const float2* spectra, size_t spec_stride, const int2* pairs,
                             int npairs, int n, float bfactor, int box, float2* tbuf,
                             float* tmax, int precision, int variant, hipStream_t st);
+// part: scratch of the MFMA row pass, pair_rows_split(n) * npairs * box *
+// box floats (required when pair_rows_split(n) > 1).
+int pair_rows_split(int n);
 hipError_t launch_pair_rows(const float2* tbuf, const float* tmax, int npairs, int n, int box,
-                            float* boxmaps, int precision, int variant, hipStream_t st);
+                            float* boxmaps, int precision, int variant, float* part,
+                            hipStream_t st);
 
 // Peaks: integer peak + FP64 16x16 -> 512x512 sub-pixel search per box map.
 // out: 3 doubles per pair (sx, sy, cc).

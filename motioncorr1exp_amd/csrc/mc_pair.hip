@@ -58,6 +58,7 @@ __global__ __launch_bounds__(NT) void k_pair_cols(const float2* __restrict__ spe
   constexpr int TS = lds_len<PB>(M) | 1;
   constexpr int EPT = (N + NT - 1) / NT;  // elements per thread
   constexpr int AS = A / SP;              // combine terms per thread
+  static_assert(A * M == N && A % SP == 0, "decimation");
   __shared__ float2 buf[A * TS];
   __shared__ float2 twm[M];
   __shared__ float2 part[SP][256];
@@ -79,7 +80,7 @@ __global__ __launch_bounds__(NT) void k_pair_cols(const float2* __restrict__ spe
 #pragma unroll
   for (int i = 0; i < AS; i++) {
     const int a = i * SP + ch;
-    ctw[i] = cact ? tw[(a * R) & (N - 1)] : make_float2(0.f, 0.f);
+    ctw[i] = cact ? tw[(a * R) % N] : make_float2(0.f, 0.f);
   }
   const int Rm = R & (M - 1);
   const int xcd = blockIdx.x % 8, lid = blockIdx.x / 8, nper = gridDim.x / 8;
@@ -168,7 +169,7 @@ __global__ __launch_bounds__(256, (N <= 4096 ? MINW : 1)) void k_pair_cols_m128(
   // stage-1 twiddles W_128^{k r}, k = tid % 16 (inverse sign)
   float2 t1[8];
 #pragma unroll
-  for (int r = 0; r < 8; r++) t1[r] = tw[((tid % 16) * r * (N / M)) & (N - 1)];
+  for (int r = 0; r < 8; r++) t1[r] = tw[((tid % 16) * r * (N / M)) % N];
   // combine: lane group (SP lanes) per output row
   const int ori = (N - box) / 2;
   const int cr = tid / SP, ch = tid % SP;
@@ -177,7 +178,7 @@ __global__ __launch_bounds__(256, (N <= 4096 ? MINW : 1)) void k_pair_cols_m128(
   const int Rm = R & (M - 1);
   float2 ctw[AS];
 #pragma unroll
-  for (int i = 0; i < AS; i++) ctw[i] = tw[((i * SP + ch) * R) & (N - 1)];
+  for (int i = 0; i < AS; i++) ctw[i] = tw[((i * SP + ch) * R) % N];
   const int xcd = blockIdx.x % 8, lid = blockIdx.x / 8, nper = gridDim.x / 8;
   const long long items = (long long)npairs * H;
   const long long i0 = items * xcd / 8, i1 = items * (xcd + 1) / 8;
@@ -341,7 +342,7 @@ __global__ __launch_bounds__(256) void k_pair_cols_mfma(const float2* __restrict
   }
   float2 t1[8];
 #pragma unroll
-  for (int r = 0; r < 8; r++) t1[r] = tw[((tid % 16) * r * (N / M)) & (N - 1)];
+  for (int r = 0; r < 8; r++) t1[r] = tw[((tid % 16) * r * (N / M)) % N];
   const int ori = (N - box) / 2;
   const int cr = tid / SP, ch = tid % SP;
   const bool cact = cr < box;
@@ -349,7 +350,7 @@ __global__ __launch_bounds__(256) void k_pair_cols_mfma(const float2* __restrict
   const int Rm = R & (M - 1);
   float2 ctw[AS];
 #pragma unroll
-  for (int i = 0; i < AS; i++) ctw[i] = tw[((i * SP + ch) * R) & (N - 1)];
+  for (int i = 0; i < AS; i++) ctw[i] = tw[((i * SP + ch) * R) % N];
   const int xcd = blockIdx.x % 8, lid = blockIdx.x / 8, nper = gridDim.x / 8;
   const long long items = (long long)npairs * H;
   const long long i0 = items * xcd / 8, i1 = items * (xcd + 1) / 8;
@@ -471,7 +472,7 @@ __global__ __launch_bounds__(256) void k_pair_rows(const float2* __restrict__ tb
       if (kx < H && c < box) {
         const float al = (kx == 0 || 2 * kx == n) ? 1.f : 2.f;
         t = cscale(T[(size_t)kx * box + c], al);
-        e = tw[(int)(((long long)kx * (ori + c)) & (n - 1))];
+        e = tw[(int)(((long long)kx * (ori + c)) % n)];
       }
       Ts[kk][c] = t;
       Es[kk][c] = e;
@@ -513,13 +514,16 @@ __global__ __launch_bounds__(256) void k_pair_rows(const float2* __restrict__ tb
 // in LDS in wave order.  Lane layout (32x32x2 f32): lane l feeds
 // A[l % 32][l / 32] and B[l / 32][l % 32]; accumulator v of lane l is
 // C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].
-template <int NB>
-__global__ __launch_bounds__(256) void k_pair_rows_mfma(const float2* __restrict__ tbuf, int n,
+template <int NB, int NT>
+__global__ __launch_bounds__(NT) void k_pair_rows_mfma(const float2* __restrict__ tbuf, int n,
                                                         int box, float* __restrict__ boxmaps,
-                                                        const float2* __restrict__ etab) {
+                                                        const float2* __restrict__ etab,
+                                                        float* __restrict__ part, int npairs) {
   constexpr int W = 32 * NB;
   constexpr int KC = 32;
-  constexpr int PER = (KC * W + 255) / 256;
+  constexpr int NW = NT / 64;  // waves; each takes KC/NW bins of every chunk
+  static_assert(KC % NW == 0, "chunk split");
+  constexpr int PER = (KC * W + NT - 1) / NT;
   __shared__ float2 Ts[2][KC][W];
   __shared__ float2 Es[2][KC][W];
   const int p = blockIdx.x;
@@ -538,7 +542,7 @@ __global__ __launch_bounds__(256) void k_pair_rows_mfma(const float2* __restrict
   auto fetch = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-      const int i = threadIdx.x + 256 * q;
+      const int i = threadIdx.x + NT * q;
       const int kk = i / W, c = i % W, kx = k0 + kk;
       float2 t = make_float2(0.f, 0.f), e = make_float2(0.f, 0.f);
       if (i < KC * W && kx < H && c < box) {
@@ -553,22 +557,26 @@ __global__ __launch_bounds__(256) void k_pair_rows_mfma(const float2* __restrict
   auto stash = [&](int b) {
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-      const int i = threadIdx.x + 256 * q;
+      const int i = threadIdx.x + NT * q;
       if (i < KC * W) {
         Ts[b][i / W][i % W] = pt[q];
         Es[b][i / W][i % W] = pe[q];
       }
     }
   };
-  const int nch = (H + KC - 1) / KC;
-  fetch(0);
+  // blockIdx.y of gridDim.y takes a contiguous share of the chunks; with
+  // more than one share the partial maps go to part[y][p] and are summed by
+  // k_sum_parts in share order
+  const int nall = (H + KC - 1) / KC;
+  const int c0 = nall * blockIdx.y / gridDim.y, c1 = nall * (blockIdx.y + 1) / gridDim.y;
+  fetch(c0 * KC);
   stash(0);
   __syncthreads();
-  for (int ch = 0; ch < nch; ch++) {
-    const int b = ch & 1;
-    if (ch + 1 < nch) fetch((ch + 1) * KC);
+  for (int ch = c0; ch < c1; ch++) {
+    const int b = (ch - c0) & 1;
+    if (ch + 1 < c1) fetch((ch + 1) * KC);
 #pragma unroll 2
-    for (int kk = wv * (KC / 4); kk < (wv + 1) * (KC / 4); kk++) {
+    for (int kk = wv * (KC / NW); kk < (wv + 1) * (KC / NW); kk++) {
       float a[NB], bb[NB];
 #pragma unroll
       for (int i = 0; i < NB; i++) {
@@ -583,13 +591,13 @@ __global__ __launch_bounds__(256) void k_pair_rows_mfma(const float2* __restrict
         for (int j = 0; j < NB; j++)
           acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], bb[j], acc[i][j], 0, 0, 0);
     }
-    if (ch + 1 < nch) stash(b ^ 1);
+    if (ch + 1 < c1) stash(b ^ 1);
     __syncthreads();
   }
-  // ordered sum of the four partial maps in LDS (reuses the T buffers)
+  // ordered sum of the NW partial maps in LDS (reuses the T buffers)
   float* red = reinterpret_cast<float*>(&Ts[0][0][0]);
   static_assert(sizeof(Ts) >= sizeof(float) * W * W, "reduction buffer");
-  for (int w = 0; w < 4; w++) {
+  for (int w = 0; w < NW; w++) {
     if (wv == w) {
 #pragma unroll
       for (int i = 0; i < NB; i++)
@@ -604,8 +612,19 @@ __global__ __launch_bounds__(256) void k_pair_rows_mfma(const float2* __restrict
     }
     __syncthreads();
   }
-  float* out = boxmaps + (size_t)p * box * box;
-  for (int e = threadIdx.x; e < box * box; e += 256) out[e] = red[(e / box) * W + e % box];
+  float* out = gridDim.y > 1 ? part + ((size_t)blockIdx.y * npairs + p) * box * box
+                            : boxmaps + (size_t)p * box * box;
+  for (int e = threadIdx.x; e < box * box; e += NT) out[e] = red[(e / box) * W + e % box];
+}
+
+// Sum of the partial box maps of a split row pass, in share order.
+__global__ __launch_bounds__(256) void k_sum_parts(const float* __restrict__ part, int nsplit,
+                                                   size_t total, float* __restrict__ boxmaps) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= total) return;
+  float acc = part[e];
+  for (int y = 1; y < nsplit; y++) acc += part[(size_t)y * total + e];
+  boxmaps[e] = acc;
 }
 
 // Row pass of the MFMA FP16 hi/lo form: the GEMM of k_pair_rows_mfma with
@@ -789,7 +808,7 @@ static const float2* row_twiddles(int n, int box) {
   std::vector<float2> h((size_t)H * box);
   for (int kx = 0; kx < H; kx++)
     for (int c = 0; c < box; c++) {
-      const long long e = ((long long)kx * (ori + c)) & (n - 1);
+      const long long e = ((long long)kx * (ori + c)) % n;
       const double a = 2.0 * M_PI * (double)e / (double)n;
       h[(size_t)kx * box + c] = make_float2((float)cos(a), (float)sin(a));
     }
@@ -868,25 +887,27 @@ static hipError_t pair_cols_n(const float2* spectra, size_t spec_stride, const i
   if constexpr (N >= 512) {
     if (variant != PAIR_COLS_GENERIC) {
       const int mode = variant == PAIR_COLS_M128_PF ? 1 : (variant == PAIR_COLS_M128_W4 ? 2 : 0);
-      if (box <= 64)
-        return pair_cols_m128<N, 4>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, mode, st);
+      if constexpr ((N / 128) % 4 == 0) {
+        if (box <= 64)
+          return pair_cols_m128<N, 4>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, mode, st);
+      }
       if (box <= 128)
         return pair_cols_m128<N, 2>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, mode, st);
     }
   }
-  if constexpr (N / 128 >= 4) {
+  if constexpr (N % 512 == 0) {
     if (box <= 64)
       return pair_cols_t<N, 128, 4>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
   }
-  if constexpr (N / 128 >= 2) {
+  if constexpr (N % 256 == 0) {
     if (box <= 128)
       return pair_cols_t<N, 128, 2>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
   }
-  if constexpr (N >= 128) {
+  if constexpr (N % 128 == 0) {
     if (box <= 128)
       return pair_cols_t<N, 128, 1>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
   }
-  if constexpr (N >= 256) {
+  if constexpr (N % 256 == 0) {
     if (box <= 256)
       return pair_cols_t<N, 256, 1>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
   }
@@ -898,8 +919,9 @@ static hipError_t pair_cols_n(const float2* spectra, size_t spec_stride, const i
 }
 
 bool pair_form_supported(int precision, int n, int box) {
-  if (precision == 0) return box >= 1 && box <= n && box <= 256;
-  if (precision == 1) return n >= 512 && n <= 8192 && box >= 1 && box <= 96 && box <= n;
+  if (precision == 0) return box >= 1 && box <= n && box <= 256 && (n % 128 == 0 || n == 64);
+  // the hi/lo column kernel needs 32-column blocks of 8 n/128 columns
+  if (precision == 1) return n >= 512 && n <= 8192 && n % 512 == 0 && box >= 1 && box <= 96;
   return false;
 }
 
@@ -917,6 +939,7 @@ hipError_t launch_pair_cols(const float2* spectra, size_t spec_stride, const int
   case NN: return box <= 64 ? pair_cols_mfma<NN, 4>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, tmax, st) \
                             : pair_cols_mfma<NN, 2>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, tmax, st);
       MC_COLS_MFMA(512) MC_COLS_MFMA(1024) MC_COLS_MFMA(2048) MC_COLS_MFMA(4096) MC_COLS_MFMA(8192)
+      MC_COLS_MFMA(1536) MC_COLS_MFMA(3072) MC_COLS_MFMA(6144)
 #undef MC_COLS_MFMA
       default: return hipErrorInvalidValue;
     }
@@ -930,12 +953,25 @@ hipError_t launch_pair_cols(const float2* spectra, size_t spec_stride, const int
     case 2048: return pair_cols_n<2048>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
     case 4096: return pair_cols_n<4096>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
     case 8192: return pair_cols_n<8192>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
+    case 768: return pair_cols_n<768>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
+    case 1536: return pair_cols_n<1536>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
+    case 3072: return pair_cols_n<3072>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
+    case 6144: return pair_cols_n<6144>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
     default: return hipErrorInvalidValue;
   }
 }
 
+// Shares of the bins per pair in the MFMA row pass: a function of n only,
+// so a pair's box map has the same summation order whatever the number of
+// pairs in the launch (a batch gives every movie the result it gets alone).
+int pair_rows_split(int n) {
+  const int nch = (n / 2 + 1 + 31) / 32;
+  return std::max(1, std::min(8, nch / 4));
+}
+
 hipError_t launch_pair_rows(const float2* tbuf, const float* tmax, int npairs, int n, int box,
-                            float* boxmaps, int precision, int variant, hipStream_t st) {
+                            float* boxmaps, int precision, int variant, float* part,
+                            hipStream_t st) {
   if (npairs <= 0) return hipSuccess;
   if (!pair_form_supported(precision, n, box)) return hipErrorInvalidValue;
   const float2* tw = twiddles(n);
@@ -952,14 +988,28 @@ hipError_t launch_pair_rows(const float2* tbuf, const float* tmax, int npairs, i
       default: return hipErrorInvalidValue;
     }
   }
-  if (variant != PAIR_ROWS_VALU) {
+  if (variant != PAIR_ROWS_VALU && box <= 96) {
+    // few pairs: split each pair's bins over ns workgroups (partials in part)
+    const int ns = pair_rows_split(n);
+    if (ns > 1 && !part) return hipErrorInvalidValue;
     switch ((box + 31) / 32) {
 #define MC_ROWS_MFMA(B) \
-  case B: hipLaunchKernelGGL(k_pair_rows_mfma<B>, dim3(npairs), dim3(256), 0, st, tbuf, n, box, boxmaps, etab); return hipGetLastError();
+  case B: \
+    if (variant == PAIR_ROWS_MFMA8) \
+      hipLaunchKernelGGL((k_pair_rows_mfma<B, 512>), dim3(npairs, ns), dim3(512), 0, st, tbuf, n, box, boxmaps, etab, part, npairs); \
+    else \
+      hipLaunchKernelGGL((k_pair_rows_mfma<B, 256>), dim3(npairs, ns), dim3(256), 0, st, tbuf, n, box, boxmaps, etab, part, npairs); \
+    break;
       MC_ROWS_MFMA(1) MC_ROWS_MFMA(2) MC_ROWS_MFMA(3)
 #undef MC_ROWS_MFMA
-      default: break;
+      default: return hipErrorInvalidValue;
     }
+    if (ns > 1) {
+      const size_t total = (size_t)npairs * box * box;
+      hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, part,
+                         ns, total, boxmaps);
+    }
+    return hipGetLastError();
   }
   const int bt = (box + 15) / 16;
   switch (bt) {

@@ -33,10 +33,13 @@ MC_DEV ValIdx better(ValIdx a, ValIdx b) {
   return a;
 }
 
-__global__ __launch_bounds__(256) void k_find_peaks(const float* __restrict__ boxmaps, int box,
-                                                    int n, int wnoise, double* __restrict__ out) {
-  __shared__ double sv[256];
-  __shared__ int si[256];
+// NT threads (256 or 512): each thread evaluates 512 / NT rows of the map.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_find_peaks(const float* __restrict__ boxmaps, int box,
+                                                   int n, int wnoise, double* __restrict__ out) {
+  static_assert(NT == 256 || NT == 512, "threads");
+  __shared__ double sv[NT];
+  __shared__ int si[NT];
   __shared__ double sub[16][16];
   __shared__ double2 G[16][9];
   __shared__ double2 t512[512];
@@ -44,14 +47,14 @@ __global__ __launch_bounds__(256) void k_find_peaks(const float* __restrict__ bo
   const int p = blockIdx.x;
   const int tid = threadIdx.x;
   const float* bm = boxmaps + (size_t)p * box * box;
-  for (int i = tid; i < 512; i += 256) {
+  for (int i = tid; i < 512; i += NT) {
     double s, c;
     sincospi((double)i / 256.0, &s, &c);
     t512[i] = make_double2(c, s);
   }
   // 1. integer peak (cufunc.cu:458-473)
   ValIdx best = {-1e9, 0x7fffffff};
-  for (int i = tid; i < box * box; i += 256) {
+  for (int i = tid; i < box * box; i += NT) {
     const int r = i / box, c = i % box;
     if (abs(c - box / 2) <= wnoise && abs(r - box / 2) <= wnoise) continue;
     const double v = (double)bm[i];
@@ -60,7 +63,7 @@ __global__ __launch_bounds__(256) void k_find_peaks(const float* __restrict__ bo
   sv[tid] = best.v;
   si[tid] = best.i;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
+  for (int o = NT / 2; o > 0; o >>= 1) {
     if (tid < o) {
       ValIdx a{sv[tid], si[tid]}, b{sv[tid + o], si[tid + o]};
       a = better(a, b);
@@ -85,7 +88,7 @@ __global__ __launch_bounds__(256) void k_find_peaks(const float* __restrict__ bo
   }
   __syncthreads();
   const int offx = soff[0], offy = soff[1];
-  sub[tid / 16][tid % 16] = (double)bm[(size_t)(offy + tid / 16) * box + offx + tid % 16];
+  if (tid < 256) sub[tid / 16][tid % 16] = (double)bm[(size_t)(offy + tid / 16) * box + offx + tid % 16];
   __syncthreads();
   // 2. 16 x 16 R2C (forward) and cosine mask, times the C2R row weight
   if (tid < 144) {
@@ -112,8 +115,8 @@ __global__ __launch_bounds__(256) void k_find_peaks(const float* __restrict__ bo
   const int ox = (box / 2 - offx) * scale, oy = (box / 2 - offy) * scale;
   ValIdx bb = {-1e300, 0x7fffffff};
   double u0 = 0.0;
-  for (int qq = 0; qq < 2; qq++) {
-    const int q = tid + 256 * qq;
+  for (int qq = 0; qq < 512 / NT; qq++) {
+    const int q = tid + NT * qq;
     double2 Hq[9];
 #pragma unroll
     for (int kx = 0; kx < 9; kx++) Hq[kx] = make_double2(0.0, 0.0);
@@ -144,7 +147,7 @@ __global__ __launch_bounds__(256) void k_find_peaks(const float* __restrict__ bo
   sv[tid] = bb.v;
   si[tid] = bb.i;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
+  for (int o = NT / 2; o > 0; o >>= 1) {
     if (tid < o) {
       ValIdx a{sv[tid], si[tid]}, b{sv[tid + o], si[tid + o]};
       a = better(a, b);
@@ -172,7 +175,8 @@ hipError_t launch_find_peaks(const float* boxmaps, int npairs, int box, int n, i
                              double* out, hipStream_t st) {
   if (npairs <= 0) return hipSuccess;
   if (box < 16) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_find_peaks, dim3(npairs), dim3(256), 0, st, boxmaps, box, n, wnoise, out);
+  hipLaunchKernelGGL(k_find_peaks<512>, dim3(npairs), dim3(512), 0, st, boxmaps, box, n, wnoise,
+                     out);
   return hipGetLastError();
 }
 

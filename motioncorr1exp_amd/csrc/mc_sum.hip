@@ -14,6 +14,10 @@
 
 namespace mc {
 
+// Frames are consumed in groups of U whose loads are issued together (the
+// loop over frames is otherwise a chain of dependent load-use latencies);
+// the accumulation order is the frame order either way.
+template <int U>
 __global__ __launch_bounds__(256) void k_shift_sum(const float2* __restrict__ spectra,
                                                    size_t spec_stride, int nframes, int n,
                                                    const double2* __restrict__ shifts,
@@ -26,17 +30,27 @@ __global__ __launch_bounds__(256) void k_shift_sum(const float2* __restrict__ sp
   const int yy = ky < n / 2 ? ky : ky - n;
   const double inv_n = 1.0 / n;
   float2 acc = make_float2(0.f, 0.f), acc2 = make_float2(0.f, 0.f);
-  for (int f = 0; f < nframes; f++) {
-    const double2 s = shifts[f];
-    if (s.x != s.x) continue;  // NaN: frame not summed
-    const float2 v = spectra[(size_t)f * spec_stride + e];
-    double u = (s.x * kx + s.y * yy) * inv_n;
-    u -= floor(u);
-    float sn, cs;
-    sincospif(2.0f * (float)u, &sn, &cs);
-    const float2 t = cmul(v, make_float2(cs, sn));
-    acc = cadd(acc, t);
-    if (sum2 && f <= f2_last) acc2 = cadd(acc2, t);
+  for (int f0 = 0; f0 < nframes; f0 += U) {
+    float2 v[U];
+    double2 sh[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      sh[u] = f0 + u < nframes ? shifts[f0 + u] : make_double2(NAN, NAN);
+      v[u] = sh[u].x == sh[u].x ? spectra[(size_t)(f0 + u) * spec_stride + e] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int f = f0 + u;
+      const double2 s = sh[u];
+      if (s.x != s.x) continue;  // NaN: frame not summed (or past the end)
+      double w = (s.x * kx + s.y * yy) * inv_n;
+      w -= floor(w);
+      float sn, cs;
+      sincospif(2.0f * (float)w, &sn, &cs);
+      const float2 t = cmul(v[u], make_float2(cs, sn));
+      acc = cadd(acc, t);
+      if (sum2 && f <= f2_last) acc2 = cadd(acc2, t);
+    }
   }
   sum[e] = acc;
   if (sum2) sum2[e] = acc2;
@@ -46,7 +60,7 @@ hipError_t launch_shift_sum(const float2* spectra, size_t spec_stride, int nfram
                             const double2* shifts, float2* sum, float2* sum2, int f2_last,
                             hipStream_t st) {
   const size_t total = (size_t)(n / 2 + 1) * n;
-  hipLaunchKernelGGL(k_shift_sum, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_shift_sum<8>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
                      spectra, spec_stride, nframes, n, shifts, sum, sum2, f2_last);
   return hipGetLastError();
 }
@@ -66,15 +80,26 @@ __global__ __launch_bounds__(256) void k_shift_sum_multi(float2* spectra, size_t
   const int yy = ky < n / 2 ? ky : ky - n;
   const double inv_n = 1.0 / n;
   float2 acc = make_float2(0.f, 0.f);
-  for (int f = 0; f < jb.y; f++) {
-    const double2 s = shifts[jb.z + f];
-    if (s.x != s.x) continue;
-    const float2 v = spectra[(size_t)(jb.x + f) * spec_stride + e];
-    double u = (s.x * kx + s.y * yy) * inv_n;
-    u -= floor(u);
-    float sn, cs;
-    sincospif(2.0f * (float)u, &sn, &cs);
-    acc = cadd(acc, cmul(v, make_float2(cs, sn)));
+  constexpr int U = 8;
+  for (int f0 = 0; f0 < jb.y; f0 += U) {
+    float2 v[U];
+    double2 sh[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      sh[u] = f0 + u < jb.y ? shifts[jb.z + f0 + u] : make_double2(NAN, NAN);
+      v[u] = sh[u].x == sh[u].x ? spectra[(size_t)(jb.x + f0 + u) * spec_stride + e]
+                                : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const double2 s = sh[u];
+      if (s.x != s.x) continue;
+      double w = (s.x * kx + s.y * yy) * inv_n;
+      w -= floor(w);
+      float sn, cs;
+      sincospif(2.0f * (float)w, &sn, &cs);
+      acc = cadd(acc, cmul(v[u], make_float2(cs, sn)));
+    }
   }
   spectra[(size_t)jb.w * spec_stride + e] = acc;
 }

@@ -34,7 +34,7 @@ def _grid_flips(a, b):
 
 
 # ----------------------------------------------------------------- level 0
-@pytest.mark.parametrize("n", [64, 512, 2048])
+@pytest.mark.parametrize("n", [64, 512, 2048, 768, 3072])
 def test_level0_fft_matches_numpy(n):
     rng = np.random.default_rng(n)
     img = rng.standard_normal((n, n)).astype(np.float32)
@@ -208,3 +208,43 @@ def test_spectra_match_oracle_every_launch_shape(cfg1, r2c, chunk, tmp_path):
     ref = o["spectra"].view(np.complex128)
     err = np.abs(spec - ref).max() / np.abs(ref).max()
     assert err < 5e-7, err
+
+
+@pytest.mark.parametrize("geom", ["crop768", "pad1536"])
+def test_three_times_power_of_two_sizes_match_oracle(ctx, o64, geom):
+    """Transform sizes 3 * 2^k (radix-3 stage): whole-movie parity with the
+    FP64 oracle, as for the power-of-two sizes: spectra within 5e-7 of the
+    maximum, box maps within 1e-5, shifts within 0.01 px (1/32 flips
+    counted), sum within 1e-4."""
+    if geom == "crop768":
+        frames, drift = make_movie(800, 780, 8, dose=2.0, step=0.8, seed=303)
+        op = dict(bin=1.0, crop_nsam=768, fod=2, bfactor=150.0, box=96, full_c2r=1, iter=1,
+                  nthreads=8)
+        gp = mc.default_params(8, FrameDistOffset=2, iter=1, crop_nsam=768)
+        n = 768
+    else:
+        frames, drift = make_movie(1400, 1300, 6, dose=2.0, step=0.8, seed=304)
+        op = dict(bin=1.0, pad=1536, fod=1, bfactor=150.0, box=96, full_c2r=1, iter=1,
+                  nthreads=8)
+        gp = mc.default_params(6, FrameDistOffset=1, iter=1, pad=1536)
+        n = 1536
+    o = o64.align_movie(frames, op, want_spectra=True)
+    assert o["n"] == n
+    d = mc.DeviceArray(frames)
+    F = frames.shape[0]
+    ctx.load_frames(d, frames.shape, frames.dtype, gp)
+    spec = ctx.get_spectra(F, n).view(np.complex64)
+    ref = o["spectra"].view(np.complex128)
+    assert np.abs(spec - ref).max() / np.abs(ref).max() < 5e-7
+    res = ctx.align_movie(d, frames.shape, frames.dtype, gp,
+                          want=("boxmaps", "pair_b", "frame_shift", "sum_image"))
+    bm, obm = res["boxmaps"], o["boxmaps"]
+    scale = np.abs(obm).reshape(len(obm), -1).max(1)[:, None, None]
+    assert (np.abs(bm - obm) / scale).max() < 1e-5
+    db = np.abs(res["pair_b"] - o["pair_b"])
+    assert np.all((db < 1e-9) | (np.abs(db - 1 / 32) < 1e-9))
+    dfs = np.abs(res["frame_shift"] - o["frame_shift"])
+    dfs = dfs[np.isfinite(dfs)]
+    assert np.all((dfs <= 0.0101) | (np.abs(dfs - 1 / 32) < 1e-9) | (np.abs(dfs - 1 / 16) < 1e-9))
+    rel = np.linalg.norm(res["sum_image"] - o["sum_image"]) / np.linalg.norm(o["sum_image"])
+    assert rel < 1e-4, rel
