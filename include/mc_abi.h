@@ -241,7 +241,9 @@ int mc_align_batch(mc_context ctx, const mc_batch_item *items, int count,
  *   pair stage (FP32 form): column kernel (M=128, M=128 with next-item
  *            prefetch, M-generic, M=128 at 4 waves per SIMD) and row kernel
  *            (FP32 MFMA with 4 or 8 waves per pair, vector ALU).
- * Each candidate runs once untimed and 3 times timed (minimum kept).  With
+ * Each candidate runs once untimed and 3 times timed (minimum kept); the
+ * context's resident spectra are overwritten, so the stage entry points need
+ * mc_load_frames again afterwards.  With
  * cache_path, one JSON line per tuned geometry is appended
  * ({"arch", "cus", "nu", "n", "dtype", "box", "precision", "r2c", "chunk",
  * "cols", "rows"}); mc_tune_load reads the lines of the same device type

@@ -72,12 +72,8 @@ bool pair_form_supported(int precision, int n, int box);
 hipError_t launch_pair_cols(const float2* spectra, size_t spec_stride, const int2* pairs,
                             int npairs, int n, float bfactor, int box, float2* tbuf,
                             float* tmax, int precision, int variant, hipStream_t st);
-// part: scratch of the MFMA row pass, pair_rows_split(n) * npairs * box *
-// box floats (required when pair_rows_split(n) > 1).
-int pair_rows_split(int n);
 hipError_t launch_pair_rows(const float2* tbuf, const float* tmax, int npairs, int n, int box,
-                            float* boxmaps, int precision, int variant, float* part,
-                            hipStream_t st);
+                            float* boxmaps, int precision, int variant, hipStream_t st);
 
 // Peaks: integer peak + FP64 16x16 -> 512x512 sub-pixel search per box map.
 // out: 3 doubles per pair (sx, sy, cc).

@@ -517,114 +517,104 @@ __global__ __launch_bounds__(256) void k_pair_rows(const float2* __restrict__ tb
 template <int NB, int NT>
 __global__ __launch_bounds__(NT) void k_pair_rows_mfma(const float2* __restrict__ tbuf, int n,
                                                         int box, float* __restrict__ boxmaps,
-                                                        const float2* __restrict__ etab,
-                                                        float* __restrict__ part, int npairs) {
+                                                        const float2* __restrict__ etab) {
   constexpr int W = 32 * NB;
   constexpr int KC = 32;
   constexpr int NW = NT / 64;  // waves; each takes KC/NW bins of every chunk
   static_assert(KC % NW == 0, "chunk split");
-  constexpr int PER = (KC * W + NT - 1) / NT;
-  __shared__ float2 Ts[2][KC][W];
+  constexpr int PT = (KC * 32 + NT - 1) / NT;  // T elements (this row block) per thread
+  constexpr int PE = (KC * W + NT - 1) / NT;   // twiddles per thread
+  __shared__ float2 Ts[2][KC][32];
   __shared__ float2 Es[2][KC][W];
   const int p = blockIdx.x;
+  const int rb = blockIdx.y;  // 32-row block of the box map
   const int H = n / 2 + 1;
   const float2* T = tbuf + (size_t)p * H * box;
   const int wv = threadIdx.x / 64, l = threadIdx.x % 64;
   const int li = l % 32, lh = l / 32;
-  v16f acc[NB][NB];
+  v16f acc[NB];
 #pragma unroll
-  for (int i = 0; i < NB; i++)
+  for (int j = 0; j < NB; j++)
 #pragma unroll
-    for (int j = 0; j < NB; j++)
-#pragma unroll
-      for (int v = 0; v < 16; v++) acc[i][j][v] = 0.f;
-  float2 pt[PER], pe[PER];
+    for (int v = 0; v < 16; v++) acc[j][v] = 0.f;
+  float2 pt[PT], pe[PE];
   auto fetch = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < PER; q++) {
+    for (int q = 0; q < PT; q++) {
       const int i = threadIdx.x + NT * q;
-      const int kk = i / W, c = i % W, kx = k0 + kk;
-      float2 t = make_float2(0.f, 0.f), e = make_float2(0.f, 0.f);
-      if (i < KC * W && kx < H && c < box) {
+      const int kk = i / 32, r = 32 * rb + i % 32, kx = k0 + kk;
+      float2 t = make_float2(0.f, 0.f);
+      if (i < KC * 32 && kx < H && r < box) {
         const float al = (kx == 0 || 2 * kx == n) ? 1.f : 2.f;
-        t = cscale(T[(size_t)kx * box + c], al);
-        e = etab[(size_t)kx * box + c];
+        t = cscale(T[(size_t)kx * box + r], al);
       }
       pt[q] = t;
+    }
+#pragma unroll
+    for (int q = 0; q < PE; q++) {
+      const int i = threadIdx.x + NT * q;
+      const int kk = i / W, c = i % W, kx = k0 + kk;
+      float2 e = make_float2(0.f, 0.f);
+      if (i < KC * W && kx < H && c < box) e = etab[(size_t)kx * box + c];
       pe[q] = e;
     }
   };
   auto stash = [&](int b) {
 #pragma unroll
-    for (int q = 0; q < PER; q++) {
+    for (int q = 0; q < PT; q++) {
       const int i = threadIdx.x + NT * q;
-      if (i < KC * W) {
-        Ts[b][i / W][i % W] = pt[q];
-        Es[b][i / W][i % W] = pe[q];
-      }
+      if (i < KC * 32) Ts[b][i / 32][i % 32] = pt[q];
+    }
+#pragma unroll
+    for (int q = 0; q < PE; q++) {
+      const int i = threadIdx.x + NT * q;
+      if (i < KC * W) Es[b][i / W][i % W] = pe[q];
     }
   };
-  // blockIdx.y of gridDim.y takes a contiguous share of the chunks; with
-  // more than one share the partial maps go to part[y][p] and are summed by
-  // k_sum_parts in share order
-  const int nall = (H + KC - 1) / KC;
-  const int c0 = nall * blockIdx.y / gridDim.y, c1 = nall * (blockIdx.y + 1) / gridDim.y;
-  fetch(c0 * KC);
+  const int nch = (H + KC - 1) / KC;
+  fetch(0);
   stash(0);
   __syncthreads();
-  for (int ch = c0; ch < c1; ch++) {
-    const int b = (ch - c0) & 1;
-    if (ch + 1 < c1) fetch((ch + 1) * KC);
+  for (int ch = 0; ch < nch; ch++) {
+    const int b = ch & 1;
+    if (ch + 1 < nch) fetch((ch + 1) * KC);
 #pragma unroll 2
     for (int kk = wv * (KC / NW); kk < (wv + 1) * (KC / NW); kk++) {
-      float a[NB], bb[NB];
+      const float2 tv = Ts[b][kk][li];
+      const float a = lh ? tv.y : tv.x;
+      float bb[NB];
 #pragma unroll
-      for (int i = 0; i < NB; i++) {
-        const float2 tv = Ts[b][kk][32 * i + li];
-        const float2 ev = Es[b][kk][32 * i + li];
-        a[i] = lh ? tv.y : tv.x;
-        bb[i] = lh ? -ev.y : ev.x;
+      for (int j = 0; j < NB; j++) {
+        const float2 ev = Es[b][kk][32 * j + li];
+        bb[j] = lh ? -ev.y : ev.x;
       }
 #pragma unroll
-      for (int i = 0; i < NB; i++)
-#pragma unroll
-        for (int j = 0; j < NB; j++)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], bb[j], acc[i][j], 0, 0, 0);
+      for (int j = 0; j < NB; j++)
+        acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bb[j], acc[j], 0, 0, 0);
     }
-    if (ch + 1 < c1) stash(b ^ 1);
+    if (ch + 1 < nch) stash(b ^ 1);
     __syncthreads();
   }
-  // ordered sum of the NW partial maps in LDS (reuses the T buffers)
-  float* red = reinterpret_cast<float*>(&Ts[0][0][0]);
-  static_assert(sizeof(Ts) >= sizeof(float) * W * W, "reduction buffer");
+  // ordered sum of the NW partial row blocks in LDS (reuses the buffers)
+  float* red = reinterpret_cast<float*>(&Es[0][0][0]);
+  static_assert(sizeof(Es) >= sizeof(float) * 32 * W, "reduction buffer");
   for (int w = 0; w < NW; w++) {
     if (wv == w) {
 #pragma unroll
-      for (int i = 0; i < NB; i++)
+      for (int j = 0; j < NB; j++)
 #pragma unroll
-        for (int j = 0; j < NB; j++)
-#pragma unroll
-          for (int v = 0; v < 16; v++) {
-            const int r = 32 * i + 8 * (v / 4) + 4 * lh + (v % 4), c = 32 * j + li;
-            if (w == 0) red[r * W + c] = acc[i][j][v];
-            else red[r * W + c] += acc[i][j][v];
-          }
+        for (int v = 0; v < 16; v++) {
+          const int r = 8 * (v / 4) + 4 * lh + (v % 4), c = 32 * j + li;
+          if (w == 0) red[r * W + c] = acc[j][v];
+          else red[r * W + c] += acc[j][v];
+        }
     }
     __syncthreads();
   }
-  float* out = gridDim.y > 1 ? part + ((size_t)blockIdx.y * npairs + p) * box * box
-                            : boxmaps + (size_t)p * box * box;
-  for (int e = threadIdx.x; e < box * box; e += NT) out[e] = red[(e / box) * W + e % box];
-}
-
-// Sum of the partial box maps of a split row pass, in share order.
-__global__ __launch_bounds__(256) void k_sum_parts(const float* __restrict__ part, int nsplit,
-                                                   size_t total, float* __restrict__ boxmaps) {
-  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (e >= total) return;
-  float acc = part[e];
-  for (int y = 1; y < nsplit; y++) acc += part[(size_t)y * total + e];
-  boxmaps[e] = acc;
+  float* out = boxmaps + (size_t)p * box * box;
+  const int rows = min(32, box - 32 * rb);
+  for (int e = threadIdx.x; e < rows * box; e += NT)
+    out[(size_t)(32 * rb + e / box) * box + e % box] = red[(e / box) * W + e % box];
 }
 
 // Row pass of the MFMA FP16 hi/lo form: the GEMM of k_pair_rows_mfma with
@@ -961,17 +951,8 @@ hipError_t launch_pair_cols(const float2* spectra, size_t spec_stride, const int
   }
 }
 
-// Shares of the bins per pair in the MFMA row pass: a function of n only,
-// so a pair's box map has the same summation order whatever the number of
-// pairs in the launch (a batch gives every movie the result it gets alone).
-int pair_rows_split(int n) {
-  const int nch = (n / 2 + 1 + 31) / 32;
-  return std::max(1, std::min(8, nch / 4));
-}
-
 hipError_t launch_pair_rows(const float2* tbuf, const float* tmax, int npairs, int n, int box,
-                            float* boxmaps, int precision, int variant, float* part,
-                            hipStream_t st) {
+                            float* boxmaps, int precision, int variant, hipStream_t st) {
   if (npairs <= 0) return hipSuccess;
   if (!pair_form_supported(precision, n, box)) return hipErrorInvalidValue;
   const float2* tw = twiddles(n);
@@ -989,25 +970,19 @@ hipError_t launch_pair_rows(const float2* tbuf, const float* tmax, int npairs, i
     }
   }
   if (variant != PAIR_ROWS_VALU && box <= 96) {
-    // few pairs: split each pair's bins over ns workgroups (partials in part)
-    const int ns = pair_rows_split(n);
-    if (ns > 1 && !part) return hipErrorInvalidValue;
-    switch ((box + 31) / 32) {
+    // one workgroup per (pair, 32-row block of the box)
+    const int nrb = (box + 31) / 32;
+    switch (nrb) {
 #define MC_ROWS_MFMA(B) \
   case B: \
     if (variant == PAIR_ROWS_MFMA8) \
-      hipLaunchKernelGGL((k_pair_rows_mfma<B, 512>), dim3(npairs, ns), dim3(512), 0, st, tbuf, n, box, boxmaps, etab, part, npairs); \
+      hipLaunchKernelGGL((k_pair_rows_mfma<B, 512>), dim3(npairs, nrb), dim3(512), 0, st, tbuf, n, box, boxmaps, etab); \
     else \
-      hipLaunchKernelGGL((k_pair_rows_mfma<B, 256>), dim3(npairs, ns), dim3(256), 0, st, tbuf, n, box, boxmaps, etab, part, npairs); \
+      hipLaunchKernelGGL((k_pair_rows_mfma<B, 256>), dim3(npairs, nrb), dim3(256), 0, st, tbuf, n, box, boxmaps, etab); \
     break;
       MC_ROWS_MFMA(1) MC_ROWS_MFMA(2) MC_ROWS_MFMA(3)
 #undef MC_ROWS_MFMA
       default: return hipErrorInvalidValue;
-    }
-    if (ns > 1) {
-      const size_t total = (size_t)npairs * box * box;
-      hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, part,
-                         ns, total, boxmaps);
     }
     return hipGetLastError();
   }
