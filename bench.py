@@ -110,8 +110,8 @@ def cpu_baseline(frames_shape, pad, P, F, nthreads, budget_s=20.0):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_3838x3710x40_pad4096.txt")
-# canvas sizes of the batching form: the supported transform sizes 2^k and 3 * 2^k
-PAD_SIZES = (1024, 1536, 2048, 3072, 4096)
+# canvas sizes of the batching form: the supported transform sizes from 1024
+PAD_SIZES = (1024, 1280, 1536, 1920, 2048, 2560, 3072, 3840, 4096)
 
 
 def committed_traffic(group):
@@ -136,7 +136,7 @@ def committed_traffic(group):
 def batch_leg(n_movies, steps, seed=1004, tune=True, tune_cache=None):
     """Batching form: n_movies movies of seeded uneven size (square sides
     uniform over the even values 1024..4096, padded with the region mean to
-    the next transform size of 1024, 1536, 2048, 3072, 4096), 8..50 frames, dose uniform in [0.5, 1.5],
+    the next supported transform size), 8..50 frames, dose uniform in [0.5, 1.5],
     random-walk drift N(0, 0.7^2), uint8 counts generated on the device;
     bin 1, fod 2, bft 150, pbx 96, 3 align-to-sum rounds.  Times
     mc_align_batch (one call = one step over all movies) and the same movies
@@ -208,7 +208,7 @@ def batch_leg(n_movies, steps, seed=1004, tune=True, tune_cache=None):
            "movies_per_transform_size": {str(k): v for k, v in sorted(sizes.items())},
            "batch_report": br, "max_shift_diff_batch_vs_single_px": same,
            "vs_truth_rms_px": float(np.sqrt(np.mean(err ** 2))),
-           "workload": "%d movies, sides U{1024..4096} even padded to 2^k or 3*2^k, frames U{8..50}, "
+           "workload": "%d movies, sides U{1024..4096} even padded to the next 2^k*{1,3,5,15}, frames U{8..50}, "
                        "dose U[0.5,1.5], uint8 counts, bin 1, fod 2, bft 150, pbx 96, itr 3"
                        % n_movies}
     for m in movies:
@@ -225,6 +225,7 @@ def main():
     ap.add_argument("--no-oracle", action="store_true", help="skip the FP64 error check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-itr3", action="store_true")
+    ap.add_argument("--no-precision", action="store_true", help="skip the MFMA hi/lo form")
     ap.add_argument("--frames", type=int, default=40)
     ap.add_argument("--no-batch", action="store_true", help="skip the batching form")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes, no autotuner")
@@ -383,6 +384,30 @@ def main():
                        "niter": r3["report"]["niter"]}
         log("itr3: %.3f ms/movie" % (dt3 * 1e3 / k3))
 
+    if not args.no_precision:
+        # precision form: pair-stage DFTs as FP16 hi/lo MFMA products
+        p1 = dict(params, precision=1)
+        step(p1)
+        mc.check(L.mc_sync())
+        ctx.kernel_timing(True)
+        t0 = time.perf_counter()
+        k1 = max(2, args.steps // 2)
+        for _ in range(k1):
+            r1 = step(p1)
+        mc.check(L.mc_sync())
+        dt1 = time.perf_counter() - t0
+        kern1 = {g: ctx.kernel_time(g)[1] / k1 for g in mc.KT}
+        ctx.kernel_timing(False)
+        d1 = r1["frame_shift"] - res["frame_shift"]
+        out["precision_form"] = {
+            "form": "MFMA FP16 hi/lo pair stage (FP32 accumulation)",
+            "movies_per_s": k1 / dt1, "ms_per_movie": dt1 * 1e3 / k1,
+            "kernels_ms_per_movie": kern1,
+            "shift_rms_diff_vs_fp32_px": float(np.sqrt(np.nanmean(d1 ** 2))),
+        }
+        log("precision form: %.3f ms/movie, shift RMS vs FP32 %.2e px"
+            % (dt1 * 1e3 / k1, out["precision_form"]["shift_rms_diff_vs_fp32_px"]))
+
     if not args.no_oracle:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from _oracle import Oracle
@@ -411,6 +436,15 @@ def main():
         }
         out["shift_rms_err_px"] = out["accuracy"]["shift_rms_err_px"]
         log("accuracy %s" % out["accuracy"])
+        if "precision_form" in out:
+            p1 = dict(params, precision=1)
+            f1 = ctx.align_movie(dev, frames.shape, frames.dtype, p1,
+                                 want=("frame_shift", "sum_image"))
+            d1 = f1["frame_shift"] - o["frame_shift"]
+            d1 = d1[np.isfinite(d1).all(1)]
+            out["precision_form"]["shift_rms_err_vs_fp64_oracle_px"] = float(np.sqrt(np.mean(d1 ** 2)))
+            out["precision_form"]["sum_rel_err_vs_fp64_oracle"] = float(
+                np.linalg.norm(f1["sum_image"] - o["sum_image"]) / np.linalg.norm(o["sum_image"]))
 
     if not args.no_cpu:
         nthreads = min(os.cpu_count() or 1, 16)

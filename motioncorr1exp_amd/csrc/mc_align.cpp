@@ -59,9 +59,15 @@ extern "C" const char* mc_last_error(void) { return g_err.c_str(); }
 extern "C" const char* mc_version(void) { return "motioncorr1exp_amd 0.1 (gfx950)"; }
 
 static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
-// Transform sizes: powers of two in [64, 8192] and 3 * 2^k in [768, 6144].
+// Transform sizes: powers of two in [64, 8192] and 2^k * {3, 5, 15} from 768.
 static bool fft_size_ok(int n) {
-  return (is_pow2(n) && n >= 64 && n <= 8192) || n == 768 || n == 1536 || n == 3072 || n == 6144;
+  switch (n) {
+    case 768: case 1280: case 1536: case 1920: case 2560: case 3072: case 3840: case 5120:
+    case 6144:
+      return true;
+    default:
+      return is_pow2(n) && n >= 64 && n <= 8192;
+  }
 }
 
 // ------------------------------------------------------- device buffers
@@ -257,7 +263,8 @@ static int movie_geometry(const mc_movie_desc* m, const mc_params* p, Geom* g) {
   if (!fft_size_ok(g->nsu) || !fft_size_ok(g->n))
     return set_err(MC_ERR_ARG,
                    "transform size %d (binned %d) unsupported: sizes must be powers of two "
-                   "in [64, 8192] or 768, 1536, 3072, 6144; use the pad geometry",
+                   "in [64, 8192] or 768, 1280, 1536, 1920, 2560, 3072, 3840, 5120, 6144; "
+                   "use the pad geometry",
                    g->nsu, g->n);
   return MC_OK;
 }

@@ -119,18 +119,42 @@ template <int S> MC_DEV void dft3(float2* v) {
   v[2] = csub(m, r);
 }
 
+// X[k] = sum_n v[n] W5^{S n k} (Winograd-style: 2 real rotations shared).
+template <int S> MC_DEV void dft5(float2* v) {
+  const float c1 = 0.30901699437494742410f;   // cos(2 pi/5)
+  const float c2 = -0.80901699437494742410f;  // cos(4 pi/5)
+  const float s1 = 0.95105651629515357212f;   // sin(2 pi/5)
+  const float s2 = 0.58778525229247312917f;   // sin(4 pi/5)
+  const float2 a = v[0];
+  const float2 p1 = cadd(v[1], v[4]), m1 = csub(v[1], v[4]);
+  const float2 p2 = cadd(v[2], v[3]), m2 = csub(v[2], v[3]);
+  const float2 r1 = make_float2(a.x + c1 * p1.x + c2 * p2.x, a.y + c1 * p1.y + c2 * p2.y);
+  const float2 r2 = make_float2(a.x + c2 * p1.x + c1 * p2.x, a.y + c2 * p1.y + c1 * p2.y);
+  // S i (s1 m1 + s2 m2) and S i (s2 m1 - s1 m2)
+  const float2 q1 = make_float2(s1 * m1.x + s2 * m2.x, s1 * m1.y + s2 * m2.y);
+  const float2 q2 = make_float2(s2 * m1.x - s1 * m2.x, s2 * m1.y - s1 * m2.y);
+  const float2 i1 = make_float2(-S * q1.y, S * q1.x), i2 = make_float2(-S * q2.y, S * q2.x);
+  v[0] = cadd(a, cadd(p1, p2));
+  v[1] = cadd(r1, i1);
+  v[4] = csub(r1, i1);
+  v[2] = cadd(r2, i2);
+  v[3] = csub(r2, i2);
+}
+
 template <int R, int S> MC_DEV void dft(float2* v) {
   if constexpr (R == 2) dft2<S>(v);
   else if constexpr (R == 3) dft3<S>(v);
   else if constexpr (R == 4) dft4<S>(v);
+  else if constexpr (R == 5) dft5<S>(v);
   else if constexpr (R == 8) dft8<S>(v);
   else if constexpr (R == 16) dft16<S>(v);
-  else static_assert(R == 2 || R == 3 || R == 4 || R == 8 || R == 16, "radix");
+  else static_assert(R == 2 || R == 3 || R == 4 || R == 5 || R == 8 || R == 16, "radix");
 }
 
 // ---------------------------------------------------- Stockham stage plans
-// Lengths N = 2^k or 3 * 2^k.  Power-of-two plans are balanced; a length
-// 3 * 2^k runs the plan of 2^k followed by one radix-3 stage.
+// Lengths N = 2^k * m, m in {1, 3, 5, 15}.  Power-of-two plans are
+// balanced; the odd factor adds a radix-3 and/or a radix-5 stage after the
+// plan of 2^k.
 __host__ __device__ constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
 __host__ __device__ constexpr bool is_pow2c(int n) { return n > 0 && (n & (n - 1)) == 0; }
 __host__ __device__ constexpr int pow2_stage_count(int N) {
@@ -150,12 +174,15 @@ __host__ __device__ constexpr int pow2_stage_radix(int N, int s) {
        : N == 4096 ? 16
        : (s < 2 ? 16 : (s == 2 ? 8 : 4));
 }
+__host__ __device__ constexpr int odd_part(int N) { return N % 2 ? N : odd_part(N / 2); }
 __host__ __device__ constexpr int stage_count(int N) {
-  return is_pow2c(N) ? pow2_stage_count(N) : pow2_stage_count(N / 3) + 1;
+  return pow2_stage_count(N / odd_part(N)) + (odd_part(N) % 3 == 0 ? 1 : 0) +
+         (odd_part(N) % 5 == 0 ? 1 : 0);
 }
 __host__ __device__ constexpr int stage_radix(int N, int s) {
-  return is_pow2c(N) ? pow2_stage_radix(N, s)
-                     : (s < pow2_stage_count(N / 3) ? pow2_stage_radix(N / 3, s) : 3);
+  return s < pow2_stage_count(N / odd_part(N))
+             ? pow2_stage_radix(N / odd_part(N), s)
+             : ((s == pow2_stage_count(N / odd_part(N)) && odd_part(N) % 3 == 0) ? 3 : 5);
 }
 // Table index modulo the table size (any size; a power of two masks).
 MC_DEV int tw_mod(int e, int ntw) { return (ntw & (ntw - 1)) == 0 ? (e & (ntw - 1)) : (e % ntw); }

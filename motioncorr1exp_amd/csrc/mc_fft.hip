@@ -367,6 +367,7 @@ hipError_t launch_frames_to_spectra(const R2CGeom& g, int f0, int nframes, float
 #define MC_F2S(NU) case NU: return frames_to_spectra_t<NU>(g, f0, nframes, spectra, spec_stride, rt, rowsum, fill, padP, variant, st);
     MC_F2S(64) MC_F2S(128) MC_F2S(256) MC_F2S(512) MC_F2S(1024) MC_F2S(2048) MC_F2S(4096) MC_F2S(8192)
     MC_F2S(768) MC_F2S(1536) MC_F2S(3072) MC_F2S(6144)
+    MC_F2S(1280) MC_F2S(2560) MC_F2S(5120) MC_F2S(1920) MC_F2S(3840)
 #undef MC_F2S
     default: return hipErrorInvalidValue;
   }
@@ -469,6 +470,11 @@ hipError_t launch_spectrum_to_image(const float2* spec, int n, float scale, floa
     case 1536: return spectrum_to_image_t<1536>(spec, scale, img, img_pitch, tmp, st);
     case 3072: return spectrum_to_image_t<3072>(spec, scale, img, img_pitch, tmp, st);
     case 6144: return spectrum_to_image_t<6144>(spec, scale, img, img_pitch, tmp, st);
+    case 1280: return spectrum_to_image_t<1280>(spec, scale, img, img_pitch, tmp, st);
+    case 2560: return spectrum_to_image_t<2560>(spec, scale, img, img_pitch, tmp, st);
+    case 5120: return spectrum_to_image_t<5120>(spec, scale, img, img_pitch, tmp, st);
+    case 1920: return spectrum_to_image_t<1920>(spec, scale, img, img_pitch, tmp, st);
+    case 3840: return spectrum_to_image_t<3840>(spec, scale, img, img_pitch, tmp, st);
     default: return hipErrorInvalidValue;
   }
 }

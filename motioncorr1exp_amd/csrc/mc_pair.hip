@@ -881,8 +881,13 @@ static hipError_t pair_cols_n(const float2* spectra, size_t spec_stride, const i
         if (box <= 64)
           return pair_cols_m128<N, 4>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, mode, st);
       }
-      if (box <= 128)
-        return pair_cols_m128<N, 2>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, mode, st);
+      if constexpr ((N / 128) % 2 == 0) {
+        if (box <= 128)
+          return pair_cols_m128<N, 2>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, mode, st);
+      } else {
+        if (box <= 128)
+          return pair_cols_m128<N, 1>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, mode, st);
+      }
     }
   }
   if constexpr (N % 512 == 0) {
@@ -929,7 +934,8 @@ hipError_t launch_pair_cols(const float2* spectra, size_t spec_stride, const int
   case NN: return box <= 64 ? pair_cols_mfma<NN, 4>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, tmax, st) \
                             : pair_cols_mfma<NN, 2>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, tmax, st);
       MC_COLS_MFMA(512) MC_COLS_MFMA(1024) MC_COLS_MFMA(2048) MC_COLS_MFMA(4096) MC_COLS_MFMA(8192)
-      MC_COLS_MFMA(1536) MC_COLS_MFMA(3072) MC_COLS_MFMA(6144)
+      MC_COLS_MFMA(1536) MC_COLS_MFMA(3072) MC_COLS_MFMA(6144) MC_COLS_MFMA(2560)
+      MC_COLS_MFMA(5120)
 #undef MC_COLS_MFMA
       default: return hipErrorInvalidValue;
     }
@@ -947,6 +953,11 @@ hipError_t launch_pair_cols(const float2* spectra, size_t spec_stride, const int
     case 1536: return pair_cols_n<1536>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
     case 3072: return pair_cols_n<3072>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
     case 6144: return pair_cols_n<6144>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
+    case 1280: return pair_cols_n<1280>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
+    case 2560: return pair_cols_n<2560>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
+    case 5120: return pair_cols_n<5120>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
+    case 1920: return pair_cols_n<1920>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
+    case 3840: return pair_cols_n<3840>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
     default: return hipErrorInvalidValue;
   }
 }

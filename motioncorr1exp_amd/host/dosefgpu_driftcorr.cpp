@@ -258,7 +258,7 @@ static void usage() {
       {"-fnm", "FileName.mrc", "Gain reference"},
       {"-itr", "0", "Number of iteration for iterative alignment"},
       {"-ith", "0.0", "Convergence threshold for iterative alignment (RMSD shift)"},
-      {"-pad", "0", "Pad the frame into an N x N canvas (2^k or 3*2^k) instead of square crop."},
+      {"-pad", "0", "Pad the frame into an N x N canvas (2^k or 2^k*{3,5,15}) instead of square crop."},
   };
   printf("Dose Fractionation Tool:\nDrift correction (MI355X)\n\n");
   printf("    Input: InputStack.mrc [OPTION VALUE] ...\n");

@@ -6,8 +6,8 @@ read back by mc_tune_load / Context.tune_load).
 Each geometry is tuned by mc_autotune on a synthetic movie of its shape
 generated in device memory (random-wave phantom, Poisson counts).  The
 default list covers the benchmark workloads: 3838x3710x40 float32 padded to
-4096; 4096x4096x24 uint8 with 2x binning; uint8 movies padded to 1024,
-1536, 2048, 3072 and 4096 (batched form); 512x512x10 float32.
+4096; 4096x4096x24 uint8 with 2x binning; uint8 movies padded to every
+canvas size of the batched form (1024 ... 4096); 512x512x10 float32.
 """
 import argparse
 import os
@@ -22,9 +22,13 @@ GEOMETRIES = [
     (3838, 3710, 40, np.float32, 4096, 1.0, 2, 96),
     (4096, 4096, 24, np.uint8, 0, 2.0, 11, 96),
     (4000, 4000, 30, np.uint8, 4096, 1.0, 2, 96),
+    (3800, 3800, 30, np.uint8, 3840, 1.0, 2, 96),
     (3000, 3000, 30, np.uint8, 3072, 1.0, 2, 96),
+    (2500, 2500, 30, np.uint8, 2560, 1.0, 2, 96),
     (2000, 2000, 30, np.uint8, 2048, 1.0, 2, 96),
+    (1900, 1900, 30, np.uint8, 1920, 1.0, 2, 96),
     (1500, 1500, 30, np.uint8, 1536, 1.0, 2, 96),
+    (1250, 1250, 30, np.uint8, 1280, 1.0, 2, 96),
     (1024, 1024, 30, np.uint8, 1024, 1.0, 2, 96),
     (512, 512, 10, np.float32, 0, 1.0, 4, 96),
 ]

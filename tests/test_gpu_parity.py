@@ -34,7 +34,7 @@ def _grid_flips(a, b):
 
 
 # ----------------------------------------------------------------- level 0
-@pytest.mark.parametrize("n", [64, 512, 2048, 768, 3072])
+@pytest.mark.parametrize("n", [64, 512, 2048, 768, 3072, 1280, 1920, 3840, 5120])
 def test_level0_fft_matches_numpy(n):
     rng = np.random.default_rng(n)
     img = rng.standard_normal((n, n)).astype(np.float32)
@@ -210,13 +210,25 @@ def test_spectra_match_oracle_every_launch_shape(cfg1, r2c, chunk, tmp_path):
     assert err < 5e-7, err
 
 
-@pytest.mark.parametrize("geom", ["crop768", "pad1536"])
+@pytest.mark.parametrize("geom", ["crop768", "pad1536", "crop1280", "pad1920"])
 def test_three_times_power_of_two_sizes_match_oracle(ctx, o64, geom):
-    """Transform sizes 3 * 2^k (radix-3 stage): whole-movie parity with the
-    FP64 oracle, as for the power-of-two sizes: spectra within 5e-7 of the
-    maximum, box maps within 1e-5, shifts within 0.01 px (1/32 flips
-    counted), sum within 1e-4."""
-    if geom == "crop768":
+    """Transform sizes 2^k * 3, 5, 15 (radix-3 / radix-5 stages): whole-movie
+    parity with the FP64 oracle, as for the power-of-two sizes: spectra
+    within 5e-7 of the maximum, box maps within 1e-5, shifts within 0.01 px
+    (1/32 flips counted), sum within 1e-4."""
+    if geom == "crop1280":
+        frames, drift = make_movie(1300, 1290, 6, dose=2.0, step=0.8, seed=305)
+        op = dict(bin=1.0, crop_nsam=1280, fod=1, bfactor=150.0, box=96, full_c2r=1, iter=1,
+                  nthreads=8)
+        gp = mc.default_params(6, FrameDistOffset=1, iter=1, crop_nsam=1280)
+        n = 1280
+    elif geom == "pad1920":
+        frames, drift = make_movie(1800, 1700, 5, dose=2.0, step=0.8, seed=306)
+        op = dict(bin=1.0, pad=1920, fod=1, bfactor=150.0, box=96, full_c2r=1, iter=1,
+                  nthreads=8)
+        gp = mc.default_params(5, FrameDistOffset=1, iter=1, pad=1920)
+        n = 1920
+    elif geom == "crop768":
         frames, drift = make_movie(800, 780, 8, dose=2.0, step=0.8, seed=303)
         op = dict(bin=1.0, crop_nsam=768, fod=2, bfactor=150.0, box=96, full_c2r=1, iter=1,
                   nthreads=8)

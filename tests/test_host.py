@@ -84,3 +84,25 @@ def test_default_params_follow_verify_para():
     assert p["nEnd"] == 39 and p["nEndSum"] == 39 and p["nEndSum2"] == 39
     p = mc.default_params(40, nStartSum=5, nEndSum2=3)
     assert p["nEndSum2"] == 0 and p["nStartSum"] == 5
+
+
+def test_shipped_tuning_table_is_well_formed():
+    """Every line of the shipped autotuner table names this device type, a
+    supported geometry and launch shapes inside the ranges mc_tune_load
+    accepts (it skips anything else)."""
+    import json
+    import os
+
+    path = os.path.join(os.path.dirname(mc.__file__), "tune_mi355x.jsonl")
+    sizes = {1 << k for k in range(6, 14)} | {768, 1280, 1536, 1920, 2560, 3072, 3840, 5120, 6144}
+    lines = open(path).read().splitlines()
+    assert lines
+    for line in lines:
+        e = json.loads(line)
+        assert e["arch"] == "gfx950" and e["cus"] > 0
+        assert e["nu"] in sizes and e["n"] in sizes and e["n"] <= e["nu"]
+        assert e["dtype"] in mc.MC_DTYPE.values() and 16 <= e["box"] <= 256
+        assert e["precision"] in (0, 1)
+        assert 1 <= (e["r2c"] & 7) <= 3 and (e["r2c"] & ~15) == 0
+        assert e["chunk"] in (1, 2, 4, 8)
+        assert 0 <= e["cols"] <= 3 and 0 <= e["rows"] <= 2
