@@ -131,7 +131,29 @@ typedef struct {
                           (DFAlign.cu:1133); 0: nz                        */
   int first_frame;     /* absolute index of frames[0]; frame indices in
                           mc_params are absolute                          */
+  const struct mc_corrections_s *corr; /* dark/gain/stretch applied on the
+                          device while the frames are loaded, or NULL     */
 } mc_movie_desc;
+
+/* Dark/gain correction and stretch of the raw frames, applied on the device
+ * in the load of the row FFT (reference DFAlign.cu:614-766, done there on
+ * the host).  Per frame pixel (X, Y) of the nx x ny frame:
+ *   v1 = (raw - dark) * gain    (a missing reference is skipped; the float
+ *   v2 = (raw - dark2) * gain2   operations are the reference's, in order)
+ * then, with a stretch (xscale < 1 or yscale < 1), pixel (X, Y), X, Y >= 1,
+ * takes the value at (int(X*xscale + 0.5), int(Y*yscale + 0.5)) (nearest
+ * neighbour; row 0 and column 0 keep their own values).  xscale, yscale are
+ * 1/-scx and 1/-scy.  v1 feeds every spectrum; with `second`, v2 feeds a
+ * second spectrum set used as the conjugated operand of the pair stage (the
+ * hbuf2 stack, DFAlign.cu:889-890; with single-section references the
+ * reference sets dark2 = 0 and gain2 = 1).  References are device buffers
+ * of nx*ny floats, row pitch nx.  frames2 must be NULL when corr is used. */
+typedef struct mc_corrections_s {
+  const float *dark, *gain;    /* section 1 (NULL: none)                  */
+  const float *dark2, *gain2;  /* section 2 (NULL: none)                  */
+  float xscale, yscale;        /* stretch factors in (0, 1]; 1: none      */
+  int second;                  /* build the section-2 spectra             */
+} mc_corrections;
 
 /* Alignment parameters: APARA (DFAlign.h:16-63) and CDFAlign::m_iter /
  * m_iterthres (DFAlign.h:75-76), plus the new geometry and form selectors.

@@ -23,12 +23,33 @@ class MCError(RuntimeError):
     pass
 
 
+class Corrections(ctypes.Structure):
+    _fields_ = [("dark", ctypes.c_void_p), ("gain", ctypes.c_void_p),
+                ("dark2", ctypes.c_void_p), ("gain2", ctypes.c_void_p),
+                ("xscale", ctypes.c_float), ("yscale", ctypes.c_float),
+                ("second", ctypes.c_int)]
+
+
 class MovieDesc(ctypes.Structure):
     _fields_ = [("frames", ctypes.c_void_p), ("frames2", ctypes.c_void_p),
                 ("dtype", ctypes.c_int), ("nx", ctypes.c_int), ("ny", ctypes.c_int),
                 ("nz", ctypes.c_int), ("row_pitch", ctypes.c_size_t),
                 ("frame_pitch", ctypes.c_size_t), ("nz_file", ctypes.c_int),
-                ("first_frame", ctypes.c_int)]
+                ("first_frame", ctypes.c_int), ("corr", ctypes.POINTER(Corrections))]
+
+
+def corrections(dark=None, gain=None, dark2=None, gain2=None, xscale=1.0, yscale=1.0,
+                second=False):
+    """mc_corrections from DeviceArrays of nx*ny float32 references (None:
+    absent); xscale, yscale are 1/-scx and 1/-scy."""
+    c = Corrections()
+    c.dark = dark.ptr if dark is not None else None
+    c.gain = gain.ptr if gain is not None else None
+    c.dark2 = dark2.ptr if dark2 is not None else None
+    c.gain2 = gain2.ptr if gain2 is not None else None
+    c.xscale, c.yscale = xscale, yscale
+    c.second = 1 if second else 0
+    return c
 
 
 class Params(ctypes.Structure):
@@ -255,9 +276,11 @@ class Context:
             pass
 
     @staticmethod
-    def movie_desc(dev_frames, shape, dtype, dev_frames2=None, nz_file=0):
+    def movie_desc(dev_frames, shape, dtype, dev_frames2=None, nz_file=0, corr=None):
         nz, ny, nx = shape
         m = MovieDesc()
+        if corr is not None:
+            m.corr = ctypes.pointer(corr)
         m.frames = dev_frames.ptr
         m.frames2 = dev_frames2.ptr if dev_frames2 is not None else None
         m.dtype = MC_DTYPE[np.dtype(dtype)]
@@ -309,10 +332,11 @@ class Context:
         return res
 
     def align_movie(self, dev_frames, shape, dtype, params, dev_frames2=None, want=(),
-                    nz_file=0, sum_dev=None):
-        """Runs mc_align_movie.  `want` names optional outputs to return."""
+                    nz_file=0, sum_dev=None, corr=None):
+        """Runs mc_align_movie.  `want` names optional outputs to return;
+        corr: an mc_corrections (see corrections())."""
         p = to_params(params)
-        m = self.movie_desc(dev_frames, shape, dtype, dev_frames2, nz_file)
+        m = self.movie_desc(dev_frames, shape, dtype, dev_frames2, nz_file, corr)
         o, res = self._outputs(shape, p, want, sum_dev)
         r = Report()
         check(self.L.mc_align_movie(self.h, ctypes.byref(m), ctypes.byref(p), ctypes.byref(o),
@@ -321,7 +345,7 @@ class Context:
 
     def align_batch(self, movies):
         """Runs mc_align_batch over a list of dicts with keys dev, shape,
-        dtype, params and optionally dev2, want, nz_file, sum_dev.  Returns
+        dtype, params and optionally dev2, want, nz_file, sum_dev, corr.  Returns
         (per-movie results as align_movie returns them, batch report)."""
         n = len(movies)
         items = (BatchItem * n)()
@@ -329,7 +353,7 @@ class Context:
         for i, mv in enumerate(movies):
             p = to_params(mv["params"])
             m = self.movie_desc(mv["dev"], mv["shape"], mv["dtype"], mv.get("dev2"),
-                                mv.get("nz_file", 0))
+                                mv.get("nz_file", 0), mv.get("corr"))
             o, res = self._outputs(mv["shape"], p, mv.get("want", ()), mv.get("sum_dev"))
             r = Report()
             keep.append((p, m, o, r))
@@ -361,9 +385,9 @@ class Context:
         check(self.L.mc_tune_load(self.h, cache_path.encode(), ctypes.byref(n)))
         return n.value
 
-    def load_frames(self, dev_frames, shape, dtype, params, dev_frames2=None):
+    def load_frames(self, dev_frames, shape, dtype, params, dev_frames2=None, corr=None):
         p = to_params(params)
-        m = self.movie_desc(dev_frames, shape, dtype, dev_frames2)
+        m = self.movie_desc(dev_frames, shape, dtype, dev_frames2, corr=corr)
         r = Report()
         check(self.L.mc_load_frames(self.h, ctypes.byref(m), ctypes.byref(p), ctypes.byref(r)))
         return {k: getattr(r, k) for k, _ in Report._fields_}

@@ -442,7 +442,7 @@ static const mc_context_s::Tune* pair_tune(const mc_context_s* c, int n, int box
 // variant / chunk < 0: the tuned shape for this geometry, else the default.
 static int run_stage1(mc_context_s* c, const void* frames, const mc_movie_desc* m,
                       const mc_params* p, const Geom& g, int first, int count, int s0,
-                      int variant = -1, int chunk = -1) {
+                      int variant = -1, int chunk = -1, int section = 1) {
   const int nkx = g.n / 2 + 1;
   const size_t rt_per = mc::rt_frame_elems(g.n, g.nsu);
   const mc_context_s::Tune* tn = stage1_tune(c, g.nsu, g.n, m->dtype);
@@ -471,6 +471,13 @@ static int run_stage1(mc_context_s* c, const void* frames, const mc_movie_desc* 
   rg.nu = g.nsu;
   rg.n = g.n;
   rg.pad_mode = g.pad_mode;
+  if (m->corr) {
+    rg.dark = section == 1 ? m->corr->dark : m->corr->dark2;
+    rg.gain = section == 1 ? m->corr->gain : m->corr->gain2;
+    rg.xs = m->corr->xscale;
+    rg.ys = m->corr->yscale;
+    rg.nx = m->nx;
+  }
   for (int f = 0; f < count; f += chunk) {
     const int cnt = std::min(chunk, count - f);
     HIP_TRY(mc::launch_frames_to_spectra(rg, f, cnt, slot(c, s0 + f), c->spec_stride,
@@ -478,6 +485,15 @@ static int run_stage1(mc_context_s* c, const void* frames, const mc_movie_desc* 
                                          c->fill.as<float>(), c->padP.as<float2>(), variant,
                                          c->st));
   }
+  return MC_OK;
+}
+
+static int check_corrections(const mc_movie_desc* m) {
+  if (!m->corr) return MC_OK;
+  const mc_corrections* k = m->corr;
+  if (m->frames2) return set_err(MC_ERR_ARG, "frames2 and device corrections are exclusive");
+  if (!(k->xscale > 0.f && k->xscale <= 1.f && k->yscale > 0.f && k->yscale <= 1.f))
+    return set_err(MC_ERR_ARG, "stretch factors must be in (0, 1] (1/-scx, 1/-scy)");
   return MC_OK;
 }
 
@@ -489,12 +505,15 @@ static int load_frames(mc_context_s* c, const mc_movie_desc* m, const mc_params*
     return set_err(MC_ERR_ARG, "frame range [%d, %d] invalid for frames [%d, %d)", p->nStart,
                    p->nEnd, m->first_frame, m->first_frame + m->nz);
   if (!m->frames) return set_err(MC_ERR_ARG, "frames is NULL");
-  MC_TRY(ensure_spectra(c, g->n, nframe, m->frames2 != nullptr));
+  MC_TRY(check_corrections(m));
+  const bool two = m->frames2 != nullptr || (m->corr && m->corr->second);
+  MC_TRY(ensure_spectra(c, g->n, nframe, two));
   {
     KScope ks(c, MC_KT_SPECTRA);
     const int first = p->nStart - m->first_frame;
     MC_TRY(run_stage1(c, m->frames, m, p, *g, first, nframe, 0));
     if (m->frames2) MC_TRY(run_stage1(c, m->frames2, m, p, *g, first, nframe, c->off2));
+    else if (two) MC_TRY(run_stage1(c, m->frames, m, p, *g, first, nframe, c->off2, -1, -1, 2));
   }
   *nframe_out = nframe;
   return MC_OK;
@@ -837,7 +856,8 @@ static int prepare_run(MovieRun& r) {
   r.P = mc_pair_list(r.F, p->FrameDistOffset, nullptr, 0);
   r.pairs.assign(2 * (size_t)r.P, 0);
   mc_pair_list(r.F, p->FrameDistOffset, r.pairs.data(), r.P);
-  r.nslot = r.F * (m->frames2 ? 2 : 1) + 1;
+  MC_TRY(check_corrections(m));
+  r.nslot = r.F * ((m->frames2 || (m->corr && m->corr->second)) ? 2 : 1) + 1;
   return MC_OK;
 }
 
@@ -1080,7 +1100,7 @@ static int align_group(mc_context_s* c, std::vector<MovieRun*>& rs, bool use_gra
   int nslots = 0, ftot = 0;
   for (MovieRun* r : rs) {
     r->base = nslots;
-    r->off2 = r->m->frames2 ? r->F : 0;
+    r->off2 = (r->m->frames2 || (r->m->corr && r->m->corr->second)) ? r->F : 0;
     r->sum = r->base + r->nslot - 1;
     nslots += r->nslot;
     ftot += r->F;
@@ -1108,6 +1128,9 @@ static int align_group(mc_context_s* c, std::vector<MovieRun*>& rs, bool use_gra
       MC_TRY(run_stage1(c, r->m->frames, r->m, r->p, r->g, first, r->F, r->base));
       if (r->m->frames2)
         MC_TRY(run_stage1(c, r->m->frames2, r->m, r->p, r->g, first, r->F, r->base + r->off2));
+      else if (r->off2)  // section-2 corrections of the same frames
+        MC_TRY(run_stage1(c, r->m->frames, r->m, r->p, r->g, first, r->F, r->base + r->off2, -1,
+                          -1, 2));
     }
   }
   {

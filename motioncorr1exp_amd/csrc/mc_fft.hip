@@ -68,6 +68,25 @@ template <> struct Elem<DT_FLOAT32> { typedef float T; };
 // two-level twiddle split for a table of size ntw
 __host__ __device__ constexpr int tw_lbits(int ntw) { return ilog2c(ntw) / 2; }
 
+// Corrected value of frame pixel (X, Y): dark/gain at the source pixel of
+// the nearest-neighbour stretch (DFAlign.cu:718-766).  The float operations
+// are the reference's: (raw - dark) * gain, raw * gain or raw - dark; the
+// source index is int(X * xs + 0.5) with the product in float and the
+// rounding addition in double.
+template <class T>
+MC_DEV float corrected_px(const R2CGeom& g, const char* fbase, int X, int Y) {
+  if ((g.xs < 1.f || g.ys < 1.f) && X > 0 && Y > 0) {
+    X = (int)((double)((float)X * g.xs) + 0.5);
+    Y = (int)((double)((float)Y * g.ys) + 0.5);
+  }
+  float v = (float)reinterpret_cast<const T*>(fbase + (size_t)Y * g.row_pitch)[X];
+  const size_t k = (size_t)Y * g.nx + X;
+  if (g.dark && g.gain) v = (v - g.dark[k]) * g.gain[k];
+  else if (g.gain) v = v * g.gain[k];
+  else if (g.dark) v = v - g.dark[k];
+  return v;
+}
+
 // ------------------------------------------------------------- row pass
 template <int NU, int ROWS, int NT, int DT>
 __global__ __launch_bounds__(NT) void k_r2c_rows(R2CGeom g, int f0, int nkx, float2* rt,
@@ -88,12 +107,21 @@ __global__ __launch_bounds__(NT) void k_r2c_rows(R2CGeom g, int f0, int nkx, flo
   struct alignas(2 * sizeof(T)) Pair {
     T a, b;
   };
+  const bool corr = g.dark || g.gain || g.xs < 1.f || g.ys < 1.f;
 #pragma unroll
   for (int r = 0; r < ROWS; r++) {
     const int y = y0 + r;
     const T* row = reinterpret_cast<const T*>(fbase + (size_t)(y + g.offy) * g.row_pitch) + g.offx;
     const bool yok = y < g.h;
-    if (g.vec && yok) {
+    if (corr) {
+      for (int m = threadIdx.x; m < N; m += NT) {
+        const int x = 2 * m;
+        float2 z = make_float2(0.f, 0.f);
+        if (yok && x < g.w) z.x = corrected_px<T>(g, fbase, g.offx + x, g.offy + y);
+        if (yok && x + 1 < g.w) z.y = corrected_px<T>(g, fbase, g.offx + x + 1, g.offy + y);
+        buf[r * TS + lix<PB>(m)] = z;
+      }
+    } else if (g.vec && yok) {
 #pragma unroll 4
       for (int m = threadIdx.x; m < N; m += NT) {
         const int x = 2 * m;

@@ -21,6 +21,11 @@ struct R2CGeom {
   int n;                // binned size (<= nu), even
   int pad_mode;         // 1: canvas outside the region holds the frame mean
   int vec = 0;          // set by the launcher: element pairs are 2-element aligned
+  // corrections of the raw frame (frame pixel (X, Y), see mc_corrections):
+  const float* dark = nullptr;  // nx*ny or nullptr
+  const float* gain = nullptr;  // nx*ny or nullptr
+  float xs = 1.f, ys = 1.f;     // nearest-neighbour stretch when either < 1
+  int nx = 0;                   // frame width (reference row pitch)
 };
 
 // Row-pass scratch of one frame (float2 elements): the kept bins kx in

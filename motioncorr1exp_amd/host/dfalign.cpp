@@ -130,7 +130,8 @@ void CDFAlign::AlignWorker() {
     m_status = -1;
     return;
   }
-  const float xscale = 1.0f / para.xscale, yscale = 1.0f / para.yscale;
+  // DFAlign.cu:503-504: the quotient in double, stored as float
+  const float xscale = (float)(1.0 / para.xscale), yscale = (float)(1.0 / para.yscale);
   const int offx = para.crop_offsetx, offy = para.crop_offsety;
   // geometry (DFAlign.cu:508-522; func.cpp:80-103)
   int nsamUnbin;
@@ -200,93 +201,81 @@ void CDFAlign::AlignWorker() {
   }
   const bool stretch = xscale < 1.0f || yscale < 1.0f;
   const bool correct = hasDark || hasNorm || stretch;
-  // two different corrected copies only with two-section references
-  const bool two = (hasDark && dark2 != dark) || (hasNorm && norm2 != norm);
+  // The conjugated pair operand differs from the other spectra only with
+  // references whose second section differs from the first (a single-section
+  // reference has dark2 = 0, gain2 = 1: the raw frame, DFAlign.cu:640-690).
+  const bool dark_two = hasDark && dark2 != dark, norm_two = hasNorm && norm2 != norm;
+  const bool two = dark_two || norm_two;
 
-  // read stack (DFAlign.cu:696-820).  Without corrections the raw sections
-  // are uploaded and decoded on the device.
+  // read stack (DFAlign.cu:696-820): raw sections are uploaded and decoded,
+  // dark/gain corrected and stretched on the device (mc_corrections)
   TextOutput("\nRead stack:\n");
   int dtype = MC_DTYPE_FLOAT32;
   size_t esize = 4;
   const int mode = stack->getMode();
-  const bool raw = !correct && (mode == 0 || mode == 1 || mode == 2 || mode == 6);
+  const bool raw = mode == 0 || mode == 1 || mode == 2 || mode == 6;
   if (raw) {
     dtype = mode == 0 ? MC_DTYPE_INT8 : mode == 1 ? MC_DTYPE_INT16 : mode == 6 ? MC_DTYPE_UINT16 : MC_DTYPE_FLOAT32;
     esize = mode == 0 ? 1 : mode == 2 ? 4 : 2;
   }
   const size_t fbytes = (size_t)nx * ny * esize;
-  std::vector<unsigned char> host((size_t)nframe * fbytes), host2;
-  if (two) host2.resize((size_t)nframe * fbytes);
+  std::vector<unsigned char> host((size_t)nframe * fbytes);
   std::vector<std::string> corr_lines(nframe);
-  std::vector<float> b1((size_t)nx * ny), b2((size_t)nx * ny);
   for (int j = para.nStart; j <= para.nEnd; j++) {
     const int pos = j - para.nStart;
-    if (raw) {
-      if (stack->read2DIm(host.data() + (size_t)pos * fbytes, j) != fbytes)
-        TextOutput(fmt("Error when reading #%03d\n", j).c_str());
-      continue;
-    }
-    if (stack->read2DIm_32bit(b1.data(), j) != stack->getImSize())
+    if (hasNorm && hasDark) corr_lines[pos] = "Correcting dark/gain...\n";
+    else if (hasNorm) corr_lines[pos] = "Correcting gain...\n";
+    else if (hasDark) corr_lines[pos] = "Correcting dark...\n";
+    unsigned char* dst = host.data() + (size_t)pos * fbytes;
+    const size_t got = raw ? stack->read2DIm(dst, j) : stack->read2DIm_32bit(reinterpret_cast<float*>(dst), j) * sizeof(float);
+    if (got != (raw ? fbytes : (size_t)stack->getImSize() * sizeof(float)))
       TextOutput(fmt("Error when reading #%03d\n", j).c_str());
-    const size_t np = (size_t)nx * ny;
-    if (hasNorm && hasDark) {
-      corr_lines[pos] = "Correcting dark/gain...\n";
-      for (size_t k = 0; k < np; k++) {
-        b2[k] = (b1[k] - dark2[k]) * norm2[k];
-        b1[k] = (b1[k] - dark[k]) * norm[k];
-      }
-    } else if (hasNorm) {
-      corr_lines[pos] = "Correcting gain...\n";
-      for (size_t k = 0; k < np; k++) {
-        b2[k] = b1[k] * norm2[k];
-        b1[k] = b1[k] * norm[k];
-      }
-    } else if (hasDark) {
-      corr_lines[pos] = "Correcting dark...\n";
-      for (size_t k = 0; k < np; k++) {
-        b2[k] = b1[k] - dark2[k];
-        b1[k] = b1[k] - dark[k];
-      }
-    } else {
-      memcpy(b2.data(), b1.data(), np * sizeof(float));
-    }
-    if (stretch) {  // nearest neighbour, row 0 and column 0 untouched (DFAlign.cu:752-766)
-      for (int ky = ny - 1; ky > 0; ky--)
-        for (int kx = nx - 1; kx > 0; kx--) {
-          const size_t o = (size_t)(int)(ky * yscale + 0.5) * nx + (int)(kx * xscale + 0.5);
-          const size_t d = (size_t)ky * nx + kx;
-          b1[d] = b1[o];
-          b2[d] = b2[o];
-        }
-    }
-    memcpy(host.data() + (size_t)pos * fbytes, b1.data(), fbytes);
-    if (two) memcpy(host2.data() + (size_t)pos * fbytes, b2.data(), fbytes);
   }
   delete stack;
 
   // upload: the device frames hold [nStart, nEnd]; indices are made relative
   void* dframes = nullptr;
-  void* dframes2 = nullptr;
-  if (mc_mem_alloc(&dframes, host.size()) != MC_OK ||
-      mc_mem_h2d(dframes, host.data(), host.size()) != MC_OK ||
-      (two && (mc_mem_alloc(&dframes2, host2.size()) != MC_OK ||
-               mc_mem_h2d(dframes2, host2.data(), host2.size()) != MC_OK))) {
+  void* dref[4] = {nullptr, nullptr, nullptr, nullptr};  // dark, gain, dark2, gain2
+  const std::vector<float>* refs[4] = {hasDark ? &dark : nullptr, hasNorm ? &norm : nullptr,
+                                       dark_two ? &dark2 : nullptr, norm_two ? &norm2 : nullptr};
+  bool ok = mc_mem_alloc(&dframes, host.size()) == MC_OK &&
+            mc_mem_h2d(dframes, host.data(), host.size()) == MC_OK;
+  for (int i = 0; ok && i < 4; i++)
+    if (refs[i])
+      ok = mc_mem_alloc(&dref[i], refs[i]->size() * sizeof(float)) == MC_OK &&
+           mc_mem_h2d(dref[i], refs[i]->data(), refs[i]->size() * sizeof(float)) == MC_OK;
+  auto free_dev = [&]() {
+    if (dframes) mc_mem_free(dframes);
+    for (void* d : dref)
+      if (d) mc_mem_free(d);
+  };
+  if (!ok) {
     Message(mc_last_error());
     m_status = MC_ERR_NOMEM;
-    if (dframes) mc_mem_free(dframes);
+    free_dev();
     m_bRun = false;
     return;
   }
+  mc_corrections corr;
+  memset(&corr, 0, sizeof corr);
+  corr.dark = static_cast<const float*>(dref[0]);
+  corr.gain = static_cast<const float*>(dref[1]);
+  corr.dark2 = static_cast<const float*>(dref[2]);
+  corr.gain2 = static_cast<const float*>(dref[3]);
+  corr.xscale = stretch ? xscale : 1.0f;
+  corr.yscale = stretch ? yscale : 1.0f;
+  corr.second = two ? 1 : 0;
   mc_movie_desc md;
   memset(&md, 0, sizeof md);
   md.frames = dframes;
-  md.frames2 = dframes2;
+  md.frames2 = nullptr;
   md.dtype = dtype;
   md.nx = nx;
   md.ny = ny;
   md.nz = nframe;
   md.nz_file = nz;
   md.first_frame = para.nStart;
+  md.corr = correct ? &corr : nullptr;
   mc_params mp;
   memset(&mp, 0, sizeof mp);
   mp.crop_offsetx = offx;
@@ -352,8 +341,7 @@ void CDFAlign::AlignWorker() {
     Message(mc_last_error());
     m_status = rc;
     mc_context_destroy(ctx);
-    mc_mem_free(dframes);
-    if (dframes2) mc_mem_free(dframes2);
+    free_dev();
     m_bRun = false;
     return;
   }
@@ -532,7 +520,6 @@ void CDFAlign::AlignWorker() {
   if (para.bLogFSC) Message("FSC table (-fsc) is not produced by this build.");
   TextOutput("Done.\n");
   mc_context_destroy(ctx);
-  mc_mem_free(dframes);
-  if (dframes2) mc_mem_free(dframes2);
+  free_dev();
   m_bRun = false;
 }
