@@ -210,6 +210,8 @@ typedef struct {
   float *sum2_image;   /* [n*n] second sum (-2es) (host)                     */
   float *rawsum_image; /* [n*n] unaligned sum (host)                         */
   void *sum_image_dev; /* [n*n] aligned sum left in device memory            */
+  double *fsc;         /* [mc_fsc_bins(n)*4] FSC table (-fsc): per shell
+                          D_Raw, U_Raw, D_Corr, U_Corr (DFAlign.cu:1314-1318) */
 } mc_outputs;
 
 typedef struct {
@@ -290,6 +292,21 @@ typedef struct {
 int mc_autotune(mc_context ctx, const mc_movie_desc *movie, const mc_params *p,
                 const char *cache_path, mc_tune_report *rep);
 int mc_tune_load(mc_context ctx, const char *cache_path, int *loaded);
+
+/* FSC table (DFAlign.cu:212-326, CDFAlign::PlotFSC).  Shell count for size
+ * n: (n/2+1)/step + 1 with step = n/400, at least 1 (the reference divides
+ * by zero for n < 400).  Returns the count or an error code (n odd, < 4 or
+ * > 8192). */
+int mc_fsc_bins(int n);
+/* FSC of two device half spectra a, b in the reference's layout (n rows of
+ * n/2+1 complex, i.e. the hFSCRaw/hFSCCorr arrays of DFAlign.cu:821-822,
+ * 1265-1266) in shells of width step, split into the elements parallel
+ * (out[2i]) and perpendicular (out[2i+1]) to the direction (dx, dy) (the
+ * total shift; normalised when its length exceeds 0.1, else (1, 0)).  Sums
+ * are taken in FP64 (the reference sums in float).  out: host,
+ * 2*mc_fsc_bins(n) doubles. */
+int mc_fsc_table(mc_context ctx, const void *a, const void *b, int n, double dx, double dy,
+                 double *out);
 
 /* Stage entry points on the context's resident state ------------------- */
 /* Stage 1 only: frames -> resident spectra (sizes written to rep). */

@@ -79,7 +79,8 @@ class Outputs(ctypes.Structure):
                 ("rmsd", _P(ctypes.c_double)), ("final_shift", _P(ctypes.c_double)),
                 ("frame_shift", _P(ctypes.c_double)), ("frame_mean", _P(ctypes.c_double)),
                 ("sum_image", _P(ctypes.c_float)), ("sum2_image", _P(ctypes.c_float)),
-                ("rawsum_image", _P(ctypes.c_float)), ("sum_image_dev", ctypes.c_void_p)]
+                ("rawsum_image", _P(ctypes.c_float)), ("sum_image_dev", ctypes.c_void_p),
+                ("fsc", _P(ctypes.c_double))]
 
 
 class Report(ctypes.Structure):
@@ -126,7 +127,8 @@ ABI_SYMBOLS = [
     "mc_context_create", "mc_context_destroy", "mc_align_movie", "mc_load_frames",
     "mc_set_spectra", "mc_get_spectra", "mc_pair_shifts", "mc_find_peaks", "mc_sum_image",
     "mc_pair_list", "mc_lsq", "mc_threshold", "mc_kernel_timing", "mc_kernel_timing_get",
-    "mc_sharpness", "mc_align_batch", "mc_autotune", "mc_tune_load",
+    "mc_sharpness", "mc_align_batch", "mc_autotune", "mc_tune_load", "mc_fsc_bins",
+    "mc_fsc_table",
 ]
 
 _lib = None
@@ -181,6 +183,8 @@ def lib():
         L.mc_align_batch.argtypes = [vp, _P(BatchItem), i, _P(BatchReport)]
         L.mc_autotune.argtypes = [vp, _P(MovieDesc), _P(Params), ctypes.c_char_p, _P(TuneReport)]
         L.mc_tune_load.argtypes = [vp, ctypes.c_char_p, _P(i)]
+        L.mc_fsc_bins.argtypes = [i]
+        L.mc_fsc_table.argtypes = [vp, vp, vp, i, d, d, vp]
         _lib = L
     return _lib
 
@@ -188,6 +192,13 @@ def lib():
 def check(rc):
     if rc != MC_OK:
         raise MCError("libmc error %d: %s" % (rc, lib().mc_last_error().decode(errors="replace")))
+    return rc
+
+
+def check_count(rc):
+    """A count returned by libmc, or MCError for a negative error code."""
+    if rc < 0:
+        check(rc)
     return rc
 
 
@@ -308,7 +319,8 @@ class Context:
                      rmsd=((max(p.iter, 1),), np.float64), final_shift=((K, 2), np.float64),
                      frame_shift=((F, 2), np.float64), frame_mean=((F,), np.float64),
                      sum_image=((n, n), np.float32), sum2_image=((n, n), np.float32),
-                     rawsum_image=((n, n), np.float32))
+                     rawsum_image=((n, n), np.float32),
+                     fsc=((max(self.L.mc_fsc_bins(n), 0), 4), np.float64))
         o = Outputs()
         for k in want:
             shp, dt = sizes[k]
@@ -434,6 +446,16 @@ class Context:
         g = np.zeros_like(fs)
         check(self.L.mc_sharpness(self.h, fs.ctypes.data, bfactor, ctypes.byref(J), g.ctypes.data))
         return J.value, g
+
+    def fsc_table(self, dev_a, dev_b, n, direction):
+        """mc_fsc_table on two device half spectra in the row-major layout
+        (n rows of n/2+1 complex).  Returns (nbox, 2): parallel (D) and
+        perpendicular (U) columns."""
+        nb = check_count(self.L.mc_fsc_bins(n))
+        out = np.zeros((nb, 2))
+        check(self.L.mc_fsc_table(self.h, dev_a.ptr, dev_b.ptr, n, float(direction[0]),
+                                  float(direction[1]), out.ctypes.data))
+        return out
 
     def kernel_timing(self, enable):
         check(self.L.mc_kernel_timing(self.h, 1 if enable else 0))

@@ -99,7 +99,7 @@ struct mc_context_s {
   int device = 0;
   hipStream_t st = nullptr;
   DevBuf spec, rt, rowsum, fill, padP, pairs_d, tbuf, box, peaks, shifts, image, tmp, means,
-      sum2img, wk, gpart, gout, jobs, tmax;
+      sum2img, wk, gpart, gout, jobs, tmax, fsc;
   int graph_captures = 0, graph_replays = 0;
   // autotuned launch shapes (mc_autotune / mc_tune_load)
   struct Tune {
@@ -829,6 +829,8 @@ struct MovieRun {
   std::vector<int> pairs;
   std::vector<double> b, s, oldx, newx, fs, sh, sh2;
   int ng = 0, it = 0, converged = 0, rawsum_written = 0, sum2_written = 0, gacc = 0;
+  int nss = 0, nes = 0;       // first and last summed frame positions
+  double tot_x = 0, tot_y = 0;  // total shift of frame nes (the FSC direction)
   double rmsd = 0, sj0 = 0, sj1 = 0;
   bool iterating() const { return it < p->iter && !converged; }
 };
@@ -979,7 +981,79 @@ static void final_shifts(MovieRun& r, int* nes2_out) {
     r.fs[2 * j] = -tx + midx;
     r.fs[2 * j + 1] = -ty + midy;
   }
+  r.nss = nss;
+  r.nes = nes;
+  r.tot_x = tx;
+  r.tot_y = ty;
   *nes2_out = nes2;
+}
+
+// Shell sums of one pair of half spectra into sums (nbox*6 doubles, device),
+// direction normalised as DFAlign.cu:236-237 does.  The caller sizes c->fsc
+// to (fsc_grid(n) + 2) * nbox * 6 doubles.
+static int fsc_sums(mc_context_s* c, const float2* a, const float2* b, int n, int padded,
+                    double dx, double dy, double* sums) {
+  const double dn = std::hypot(dx, dy);  // abs(complex<double>)
+  if (dn > 0.1) {
+    dx /= dn;
+    dy /= dn;
+  } else {
+    dx = 1.0;
+    dy = 0.0;
+  }
+  // c->fsc holds two tables of sums, then the partial sums
+  const size_t nk = (size_t)mc::fsc_bins(n, nullptr) * 6;
+  KScope ks(c, MC_KT_SUM);
+  HIP_TRY(mc::launch_fsc_sums(a, b, n, padded, dx, dy, c->fsc.as<double>() + 2 * nk, sums, c->st));
+  return MC_OK;
+}
+
+// FSC = sum Re(a conj b) / sqrt(sum|a|^2 sum|b|^2), 0 when the product is
+// <= 1e-5 (DFAlign.cu:283-298); column stride `ld`, side 0 then side 1.
+static void fsc_ratios(const double* sums, int nbox, double* out, int ld) {
+  for (int i = 0; i < nbox; i++)
+    for (int side = 0; side < 2; side++) {
+      const double* v = sums + (2 * i + side) * 3;
+      const double t = v[1] * v[2];
+      out[(size_t)i * ld + side] = t > 0.00001 ? v[0] / std::sqrt(t) : 0.0;
+    }
+}
+
+// FSC table of one movie (DFAlign.cu:808-822, 1143-1266, 1314-1318): raw
+// halves are the unshifted frames at alternating positions of
+// [nStartSum, nEndSum]; aligned halves are the first summed frame plus every
+// second frame after it against the others, with the shifts of the final
+// sum (still in the context's shift buffer).  Rows: D_Raw, U_Raw, D_Corr,
+// U_Corr.
+static int fsc_table(mc_context_s* c, MovieRun& r) {
+  const mc_params* p = r.p;
+  const int n = c->n;
+  const int nbox = mc::fsc_bins(n, nullptr);
+  const size_t nk = (size_t)nbox * 6;
+  HIP_TRY(c->tmp.ensure(sizeof(float2) * c->spec_stride * 2));
+  float2* h0 = c->tmp.as<float2>();
+  float2* h1 = h0 + c->spec_stride;
+  HIP_TRY(c->fsc.ensure(sizeof(double) * nk * (mc::fsc_grid(n) + 2)));
+  double* sums = c->fsc.as<double>();
+  const int j0 = std::max(p->nStartSum, p->nStart), j1 = std::min(p->nEndSum, p->nEnd);
+  {
+    KScope ks(c, MC_KT_SUM);
+    HIP_TRY(mc::launch_half_sums(slot(c, r.base + std::max(0, j0 - p->nStart)), c->spec_stride,
+                                 std::max(0, j1 - j0 + 1), n, nullptr, h0, h1, c->st));
+  }
+  MC_TRY(fsc_sums(c, h0, h1, n, 0, r.tot_x, r.tot_y, sums));
+  {
+    KScope ks(c, MC_KT_SUM);
+    HIP_TRY(mc::launch_half_sums(slot(c, r.base + r.nss), c->spec_stride, r.nes - r.nss + 1, n,
+                                 c->shifts.as<double2>() + r.nss, h0, h1, c->st));
+  }
+  MC_TRY(fsc_sums(c, h0, h1, n, 0, r.tot_x, r.tot_y, sums + nk));
+  std::vector<double> h(2 * nk);
+  HIP_TRY(hipMemcpyAsync(h.data(), sums, sizeof(double) * 2 * nk, hipMemcpyDeviceToHost, c->st));
+  HIP_TRY(hipStreamSynchronize(c->st));
+  fsc_ratios(h.data(), nbox, r.out->fsc, 4);
+  fsc_ratios(h.data() + nk, nbox, r.out->fsc + 2, 4);
+  return MC_OK;
 }
 
 // Final phase-shift-and-sum and images of one movie (DFAlign.cu:1143-1292).
@@ -1019,6 +1093,7 @@ static int final_sum(mc_context_s* c, MovieRun& r) {
                            hipMemcpyDeviceToHost, c->st));
     r.sum2_written = 1;
   }
+  if (out->fsc) MC_TRY(fsc_table(c, r));
   // the host copies above read c->image / sum2img before the next movie's
   // launches on the same stream overwrite them
   HIP_TRY(hipStreamSynchronize(c->st));
@@ -1290,6 +1365,29 @@ static void fill_report(const MovieRun& r, const GroupTimes& tm) {
   rep->ms_iter = tm.iter;
   rep->ms_sum = tm.sum;
   rep->ms_total = tm.total;
+}
+
+extern "C" int mc_fsc_bins(int n) {
+  if (n < 4 || n > 8192 || n % 2) return set_err(MC_ERR_ARG, "FSC size %d must be even, in [4, 8192]", n);
+  return mc::fsc_bins(n, nullptr);
+}
+
+extern "C" int mc_fsc_table(mc_context ctx, const void* a, const void* b, int n, double dx,
+                            double dy, double* out) {
+  if (!ctx || !a || !b || !out) return set_err(MC_ERR_ARG, "NULL argument");
+  const int nbox = mc_fsc_bins(n);
+  if (nbox < 0) return nbox;
+  HIP_TRY(hipSetDevice(ctx->device));
+  const size_t nk = (size_t)nbox * 6;
+  HIP_TRY(ctx->fsc.ensure(sizeof(double) * nk * (mc::fsc_grid(n) + 2)));
+  MC_TRY(fsc_sums(ctx, static_cast<const float2*>(a), static_cast<const float2*>(b), n, 1, dx, dy,
+                  ctx->fsc.as<double>()));
+  std::vector<double> h(nk);
+  HIP_TRY(hipMemcpyAsync(h.data(), ctx->fsc.p, sizeof(double) * nk, hipMemcpyDeviceToHost,
+                         ctx->st));
+  HIP_TRY(hipStreamSynchronize(ctx->st));
+  fsc_ratios(h.data(), nbox, out, 2);
+  return MC_OK;
 }
 
 extern "C" int mc_align_movie(mc_context ctx, const mc_movie_desc* m, const mc_params* p,

@@ -98,6 +98,22 @@ hipError_t launch_shift_sum(const float2* spectra, size_t spec_stride, int nfram
 hipError_t launch_shift_sum_multi(float2* spectra, size_t spec_stride, int n, const int4* jobs,
                                   int njobs, const double2* shifts, hipStream_t st);
 
+// Even/odd half sums of `count` consecutive spectra: frame f goes to h0
+// (f even) or h1 (f odd) after its phase shift (shifts == nullptr: none;
+// NaN: frame skipped).
+hipError_t launch_half_sums(const float2* spectra, size_t spec_stride, int count, int n,
+                            const double2* shifts, float2* h0, float2* h1, hipStream_t st);
+
+// FSC shells (DFAlign.cu:212-297): shell width and count for size n, the grid
+// of the partial-sum pass, and the shell sums of the half spectra a, b
+// (padded = 1: row-major [ky][kx], else the resident [kx][ky] layout) split
+// by the unit direction (dx, dy).  part: fsc_grid(n) * nbox * 6 doubles;
+// sums: nbox * 6 doubles, [shell][side][Re(a conj b), |a|^2, |b|^2].
+int fsc_bins(int n, int* step);
+int fsc_grid(int n);
+hipError_t launch_fsc_sums(const float2* a, const float2* b, int n, int padded, double dx,
+                           double dy, double* part, double* sums, hipStream_t st);
+
 // Sharpness J of the shifted sum S (already formed by launch_shift_sum with
 // the same shifts) and dJ/ds_f: out[f] = (dJ/dsx_f, dJ/dsy_f), out[F] = (J, 0).
 // wk: [n/2+1] column weights (incl. the half-plane factor) then [n] row

@@ -9,6 +9,8 @@
 // so its error does not grow with the shift or the frequency.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "mc_device.h"
 #include "mc_kernels.h"
 
@@ -111,6 +113,223 @@ hipError_t launch_shift_sum_multi(float2* spectra, size_t spec_stride, int n, co
   const size_t total = (size_t)(n / 2 + 1) * n;
   hipLaunchKernelGGL(k_shift_sum_multi, dim3((unsigned)((total + 255) / 256), (unsigned)njobs),
                      dim3(256), 0, st, spectra, spec_stride, n, jobs, shifts);
+  return hipGetLastError();
+}
+
+// Even/odd half sums of `count` consecutive spectra in one pass (the halves
+// of the FSC table): frame f adds to h0 when f is even, else to h1, after its
+// phase shift (shifts == nullptr: unshifted; a NaN shift skips the frame).
+// Each half accumulates in frame order (DFAlign.cu:808-812 raw halves,
+// 1143-1243 aligned halves).  count == 0 writes zeros.
+__global__ __launch_bounds__(256) void k_half_sums(const float2* __restrict__ spectra,
+                                                   size_t spec_stride, int count, int n,
+                                                   const double2* __restrict__ shifts,
+                                                   float2* __restrict__ h0,
+                                                   float2* __restrict__ h1) {
+  const size_t total = (size_t)(n / 2 + 1) * n;
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= total) return;
+  const int kx = (int)(e / n), ky = (int)(e % n);
+  const int yy = ky < n / 2 ? ky : ky - n;
+  const double inv_n = 1.0 / n;
+  float2 acc[2] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+  constexpr int U = 8;
+  for (int f0 = 0; f0 < count; f0 += U) {
+    float2 v[U];
+    double2 sh[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      sh[u] = f0 + u >= count ? make_double2(NAN, NAN)
+                              : (shifts ? shifts[f0 + u] : make_double2(0.0, 0.0));
+      v[u] = sh[u].x == sh[u].x ? spectra[(size_t)(f0 + u) * spec_stride + e]
+                                : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const double2 s = sh[u];
+      if (s.x != s.x) continue;
+      float2 t = v[u];
+      if (shifts) {
+        double w = (s.x * kx + s.y * yy) * inv_n;
+        w -= floor(w);
+        float sn, cs;
+        sincospif(2.0f * (float)w, &sn, &cs);
+        t = cmul(t, make_float2(cs, sn));
+      }
+      acc[(f0 + u) & 1] = cadd(acc[(f0 + u) & 1], t);
+    }
+  }
+  h0[e] = acc[0];
+  h1[e] = acc[1];
+}
+
+hipError_t launch_half_sums(const float2* spectra, size_t spec_stride, int count, int n,
+                            const double2* shifts, float2* h0, float2* h1, hipStream_t st) {
+  const size_t total = (size_t)(n / 2 + 1) * n;
+  hipLaunchKernelGGL(k_half_sums, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+                     spectra, spec_stride, count, n, shifts, h0, h1);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------- FSC shells (DFAlign.cu:212-297)
+// Element (kx, y) of the half spectrum (y = ky for ky < n/2, else ky - n,
+// cufunc.cu:309-340) with r = sqrtf(kx^2 + y^2) in float belongs to shell
+// int(r / step) unless int(r) <= 0 or r >= n/2+1, on side 0 (parallel to the
+// drift direction d) when float(|kx dx + y dy| / r) > float(cos(pi/4)), else
+// side 1.  Per (shell, side) the sums of Re(a conj b), |a|^2 and |b|^2 are
+// taken in FP64.
+//
+// The float square root and division are rounded to nearest with an exact
+// check against the neighbouring midpoints (their squares and their
+// products with step are exact in FP64; sqrt and quotients of these operands
+// never fall on a midpoint), so shell membership is the reference's whatever
+// the compiler's lowering of sqrtf and '/'.
+__device__ inline float f32_next(float x, int d) { return __int_as_float(__float_as_int(x) + d); }
+
+__device__ inline float sqrt_rn(float q) {  // q >= 1
+  float r = sqrtf(q);
+#pragma unroll
+  for (int it = 0; it < 2; it++) {
+    const double hi = 0.5 * ((double)r + (double)f32_next(r, 1));
+    const double lo = 0.5 * ((double)r + (double)f32_next(r, -1));
+    if ((double)q > hi * hi)
+      r = f32_next(r, 1);
+    else if ((double)q < lo * lo)
+      r = f32_next(r, -1);
+  }
+  return r;
+}
+
+__device__ inline float div_rn(float a, int step) {  // a >= 1, step >= 1
+  float d = (float)((double)a / (double)step);
+#pragma unroll
+  for (int it = 0; it < 2; it++) {
+    const double hi = 0.5 * ((double)d + (double)f32_next(d, 1));
+    const double lo = 0.5 * ((double)d + (double)f32_next(d, -1));
+    if ((double)a > hi * step)
+      d = f32_next(d, 1);
+    else if ((double)a < lo * step)
+      d = f32_next(d, -1);
+  }
+  return d;
+}
+
+// Side of (kx, y): FP64 products, sum and quotient each rounded once as in
+// the reference's host code (no fused multiply-add).
+__device__ inline int fsc_side(int kx, int y, float r, double dx, double dy) {
+#pragma clang fp contract(off)
+  const double p = (double)kx * dx;
+  const double q = (double)y * dy;
+  const float angle = (float)(fabs(p + q) / (double)r);
+  return angle > (float)0.70710678118654752440 ? 0 : 1;
+}
+
+// Block-partial shell sums.  Each wave takes chunks of 64 consecutive
+// elements (chunk c goes to wave c mod (waves of the grid)), groups the
+// lanes of a chunk by (shell, side) and adds each group's wave-reduced sums
+// into its own LDS table in program order; the block's tables are then added
+// in wave order.  The result depends only on n and the grid, not on timing.
+// padded = 1: row-major [ky][kx] layout (pitch n/2+1), else [kx][ky].
+constexpr int FSC_NW = 4;
+
+__global__ __launch_bounds__(FSC_NW * 64) void k_fsc_partial(const float2* __restrict__ a,
+                                                            const float2* __restrict__ b, int n,
+                                                            int padded, int step, int nbox,
+                                                            double dx, double dy,
+                                                            double* __restrict__ part) {
+  extern __shared__ double fsc_acc[];  // [FSC_NW][nbox][2][3]
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int H = n / 2 + 1, total = H * n, nchunk = (total + 63) / 64;
+  const int nk = nbox * 6;
+  for (int i = threadIdx.x; i < FSC_NW * nk; i += FSC_NW * 64) fsc_acc[i] = 0.0;
+  __syncthreads();
+  double* my = fsc_acc + w * nk;
+  for (int c = blockIdx.x * FSC_NW + w; c < nchunk; c += gridDim.x * FSC_NW) {
+    const int e = c * 64 + lane;
+    int key = -1;
+    double sab = 0.0, saa = 0.0, sbb = 0.0;
+    if (e < total) {
+      int kx, ky;
+      if (padded) {
+        ky = e / H;
+        kx = e - ky * H;
+      } else {
+        kx = e / n;
+        ky = e - kx * n;
+      }
+      const int y = ky < n / 2 ? ky : ky - n;
+      const int q = kx * kx + y * y;
+      if (q > 0) {
+        const float r = sqrt_rn((float)q);
+        if ((int)r > 0 && r < (float)H) {
+          const int id = min((int)div_rn(r, step), nbox - 1);  // r < H keeps id < nbox
+          key = 2 * id + fsc_side(kx, y, r, dx, dy);
+          const float2 va = a[e], vb = b[e];
+          sab = (double)va.x * vb.x + (double)va.y * vb.y;
+          saa = (double)va.x * va.x + (double)va.y * va.y;
+          sbb = (double)vb.x * vb.x + (double)vb.y * vb.y;
+        }
+      }
+    }
+    unsigned long long rem = __ballot(key >= 0);
+    while (rem) {
+      const int k = __shfl(key, __ffsll((unsigned long long)rem) - 1);
+      const bool mine = key == k;
+      double t0 = mine ? sab : 0.0, t1 = mine ? saa : 0.0, t2 = mine ? sbb : 0.0;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        t0 += __shfl_xor(t0, o);
+        t1 += __shfl_xor(t1, o);
+        t2 += __shfl_xor(t2, o);
+      }
+      if (lane == 0) {
+        my[3 * k] += t0;
+        my[3 * k + 1] += t1;
+        my[3 * k + 2] += t2;
+      }
+      rem &= ~__ballot(mine);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nk; i += FSC_NW * 64) {
+    double s = 0.0;
+#pragma unroll
+    for (int v = 0; v < FSC_NW; v++) s += fsc_acc[v * nk + i];
+    part[(size_t)blockIdx.x * nk + i] = s;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fsc_final(const double* __restrict__ part, int nblocks,
+                                                   int nk, double* __restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nk) return;
+  double s = 0.0;
+  for (int b = 0; b < nblocks; b++) s += part[(size_t)b * nk + i];
+  out[i] = s;
+}
+
+int fsc_bins(int n, int* step) {
+  int st = n / 400;  // DFAlign.cu:214 (n < 400 would divide by zero there)
+  if (st < 1) st = 1;
+  if (step) *step = st;
+  return (n / 2 + 1) / st + 1;  // DFAlign.cu:218
+}
+
+int fsc_grid(int n) {
+  const long long nchunk = ((long long)(n / 2 + 1) * n + 63) / 64;
+  return (int)std::max(1LL, std::min(512LL, nchunk / 64));
+}
+
+hipError_t launch_fsc_sums(const float2* a, const float2* b, int n, int padded, double dx,
+                           double dy, double* part, double* sums, hipStream_t st) {
+  int step;
+  const int nbox = fsc_bins(n, &step);
+  const int nk = nbox * 6, G = fsc_grid(n);
+  const size_t lds = sizeof(double) * FSC_NW * nk;
+  if (lds > 160 * 1024) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_fsc_partial, dim3(G), dim3(FSC_NW * 64), lds, st, a, b, n, padded, step,
+                     nbox, dx, dy, part);
+  hipLaunchKernelGGL(k_fsc_final, dim3((nk + 255) / 256), dim3(256), 0, st, part, G, nk, sums);
   return hipGetLastError();
 }
 

@@ -308,6 +308,11 @@ void CDFAlign::AlignWorker() {
   std::vector<float> sum((size_t)nsam * nsam), sum2, rawsum, boxes, iboxes;
   if (para.bSaveRawSum) rawsum.resize((size_t)nsam * nsam);
   if (para.nEndSum2 > 0 && para.nEndSum2 < para.nEndSum) sum2.resize((size_t)nsam * nsam);
+  std::vector<double> fsc;  // DFAlign.cu:1314-1318
+  if (para.bLogFSC) {
+    const int nb = mc_fsc_bins(nsam);
+    if (nb > 0) fsc.resize((size_t)nb * 4);
+  }
   if (para.bSaveCCmap) {
     boxes.resize((size_t)P * box * box);
     iboxes.resize((size_t)it_cap * nframe * box * box);
@@ -333,6 +338,7 @@ void CDFAlign::AlignWorker() {
   out.sum_image = sum.data();
   out.sum2_image = sum2.empty() ? nullptr : sum2.data();
   out.rawsum_image = rawsum.empty() ? nullptr : rawsum.data();
+  out.fsc = fsc.empty() ? nullptr : fsc.data();
   mc_report rep;
   mc_context ctx = nullptr;
   int rc = mc_context_create(para.GPUNum, &ctx);
@@ -517,7 +523,16 @@ void CDFAlign::AlignWorker() {
   };
   if (!sum2.empty() && rep.sum2_written) write_sum(m_fnAlignsum2, sum2);
   write_sum(m_fnAlignsum, sum);
-  if (para.bLogFSC) Message("FSC table (-fsc) is not produced by this build.");
+  if (!fsc.empty()) {
+    // DFAlign.cu:311-320
+    const int nb = (int)(fsc.size() / 4);
+    TextOutput("\nFSC parallel(D) and perpendicular(U) to drift direction:\n");
+    TextOutput("   Nq%       D_Raw       U_Raw       D_Corr       U_Corr\n");
+    for (int i = 0; i < nb; i++)
+      TextOutput(fmt("%7.2f %12.4f %12.4f %12.4f %12.4f\n", i * 100.0 / nb, fsc[4 * i],
+                     fsc[4 * i + 1], fsc[4 * i + 2], fsc[4 * i + 3]).c_str());
+    TextOutput("\n");
+  }
   TextOutput("Done.\n");
   mc_context_destroy(ctx);
   free_dev();

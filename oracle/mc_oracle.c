@@ -770,8 +770,16 @@ int or_align_movie(const float *frames, const float *frames2,
     }
   }
   if (r->spectra) memcpy(r->spectra, hbuf, sizeof(R) * sb * nframe);
+  /* DFAlign.cu:821-822: the raw halves are kept for the FSC table */
+  R *fsc_raw0 = NULL, *fsc_raw1 = NULL;
+  if (r->fsc) {
+    fsc_raw0 = (R *)malloc(sizeof(R) * sb);
+    memcpy(fsc_raw0, rs0, sizeof(R) * sb);
+    fsc_raw1 = rs1;
+    rs1 = NULL;
+  }
   if (r->rawsum_image) {
-    add_spec(rs0, rs1, sb); /* DFAlign.cu:823 */
+    add_spec(rs0, fsc_raw1 ? fsc_raw1 : rs1, sb); /* DFAlign.cu:823 */
     c2r_to_image(rs0, n, r->rawsum_image);
     r->rawsum_written = 1;
   }
@@ -938,6 +946,21 @@ int or_align_movie(const float *frames, const float *frames2,
         r->sum2_written = 1;
       }
     }
+    if (r->fsc) {
+      /* DFAlign.cu:1265-1266,1314-1318: Corr0 = dsumcorr, Corr1 = dsum,
+       * direction = total shift of the last summed frame */
+      const int nb = or_fsc_bins(n, NULL);
+      double *t = (double *)malloc(sizeof(double) * 4 * (size_t)nb);
+      or_fsc(fsc_raw0, fsc_raw1, n, tx, ty, t);
+      or_fsc(dsumcorr, dsum, n, tx, ty, t + 2 * (size_t)nb);
+      for (int i = 0; i < nb; i++) {
+        r->fsc[4 * i] = t[2 * i];
+        r->fsc[4 * i + 1] = t[2 * i + 1];
+        r->fsc[4 * i + 2] = t[2 * nb + 2 * i];
+        r->fsc[4 * i + 3] = t[2 * nb + 2 * i + 1];
+      }
+      free(t);
+    }
     add_spec(dsumcorr, dsum, sb);
     if (r->sum_image) c2r_to_image(dsumcorr, n, r->sum_image);
     free(fr);
@@ -950,10 +973,14 @@ int or_align_movie(const float *frames, const float *frames2,
   free(boxes);
   free(pairs);
   free(work);
+  free(fsc_raw0);
+  free(fsc_raw1);
   if (hbuf2 != hbuf) free(hbuf2);
   free(hbuf);
   return 0;
 fail:
+  free(fsc_raw0);
+  free(fsc_raw1);
   free(s);
   free(ki);
   free(b);
@@ -963,6 +990,56 @@ fail:
   if (hbuf2 != hbuf) free(hbuf2);
   free(hbuf);
   return -10;
+}
+
+/* DFAlign.cu:214-218 */
+int or_fsc_bins(int n, int *step) {
+  int st = n / 400;
+  if (st < 1) st = 1;
+  if (step) *step = st;
+  return (n / 2 + 1) / st + 1;
+}
+
+/* DFAlign.cu:212-297.  The element order of the reference loop is the
+ * padded layout's (row ky, column kx); MkPosList (cufunc.cu:309-340) gives
+ * row ky the frequency y = ky for ky < n/2, else ky - n. */
+int or_fsc(const R *a, const R *b, int n, double dx, double dy, double *out) {
+  int step;
+  const int nbox = or_fsc_bins(n, &step);
+  const int nsamc = n / 2 + 1;
+  const double dn = hypot(dx, dy); /* abs(complex<double>) */
+  if (dn > 0.1) {
+    dx /= dn;
+    dy /= dn;
+  } else {
+    dx = 1.0;
+    dy = 0.0;
+  }
+  const float edge = (float)cos(OR_PI / 4);
+  double *acc = (double *)calloc((size_t)nbox * 6, sizeof(double));
+  if (!acc) return -1;
+  for (int ky = 0; ky < n; ky++) {
+    const int y = ky < n / 2 ? ky : ky - n;
+    const cpx *A = (const cpx *)(a + (size_t)ky * (n + 2));
+    const cpx *B = (const cpx *)(b + (size_t)ky * (n + 2));
+    for (int x = 0; x < nsamc; x++) {
+      const float r = sqrtf((float)(x * x + y * y));
+      if ((int)r <= 0 || r >= (float)nsamc) continue;
+      const int id = (int)(r / (float)step);
+      const float angle = (float)(fabs(x * dx + y * dy) / (double)r);
+      double *s = acc + ((size_t)id * 2 + (angle > edge ? 0 : 1)) * 3;
+      s[0] += (double)A[x].re * B[x].re + (double)A[x].im * B[x].im;
+      s[1] += (double)A[x].re * A[x].re + (double)A[x].im * A[x].im;
+      s[2] += (double)B[x].re * B[x].re + (double)B[x].im * B[x].im;
+    }
+  }
+  for (int i = 0; i < 2 * nbox; i++) {
+    const double *s = acc + (size_t)i * 3;
+    const double t = s[1] * s[2];
+    out[i] = t > 0.00001 ? s[0] / sqrt(t) : 0.0;
+  }
+  free(acc);
+  return 0;
 }
 
 /* Sharpness objective and gradient (no reference counterpart). */

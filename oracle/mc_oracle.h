@@ -147,6 +147,8 @@ typedef struct {
   or_real *rawsum_image; /* [n*n]                                           */
   or_real *spectra;      /* [nframe*n*(n+2)] frame spectra (padded layout)  */
   int sum2_written, rawsum_written;
+  double *fsc;           /* [or_fsc_bins(n)*4] FSC table: per shell D_Raw,
+                            U_Raw, D_Corr, U_Corr (DFAlign.cu:1314-1318)   */
 } or_result;
 
 /* frames: nz x ny x nx float (decoded, dark/gain corrected = hbuf source);
@@ -154,6 +156,21 @@ typedef struct {
  * or NULL when it equals frames.  Returns 0 on success. */
 int or_align_movie(const float *frames, const float *frames2,
                    const or_params *p, or_result *r);
+
+/* ---- FSC table (DFAlign.cu:212-326) ------------------------------------ */
+/* Shell width step = n/400 (at least 1: the reference divides by zero for
+ * n < 400) and shell count (n/2+1)/step + 1 (DFAlign.cu:214-218). */
+int or_fsc_bins(int n, int *step);
+/* FSC of two half spectra a, b (padded layout) in shells of width step,
+ * split into elements parallel (|k.d|/|k| > cos(pi/4), out[2i]) and
+ * perpendicular (out[2i+1]) to the direction d = (dx, dy) (normalised when
+ * |d| > 0.1, else (1, 0)).  Element (kx, y) with r = sqrtf(kx^2+y^2) in
+ * float goes to shell int(r/step); DC and r >= n/2+1 are skipped.  Sums of
+ * Re(a conj b), |a|^2, |b|^2 per shell in FP64 (the reference sums in
+ * float); FSC = sum Re / sqrt(sum|a|^2 sum|b|^2), 0 when that product is
+ * <= 1e-5.  DFAlign.cu:212-297. */
+int or_fsc(const or_real *a, const or_real *b, int n, double dx, double dy,
+           double *out);
 
 /* Sharpness objective of the phase-shift-and-sum and its gradient with
  * respect to per-frame shifts (no reference counterpart):

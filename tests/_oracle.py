@@ -42,6 +42,7 @@ def _result_struct(real_t):
             ("iter_meas", dp), ("rmsd", dp), ("final_shift", dp), ("frame_shift", dp),
             ("sum_image", rp), ("sum2_image", rp), ("rawsum_image", rp), ("spectra", rp),
             ("sum2_written", ctypes.c_int), ("rawsum_written", ctypes.c_int),
+            ("fsc", dp),
         ]
     return OrResult
 
@@ -83,6 +84,8 @@ class Oracle:
         L.or_align_movie.argtypes = [vp, vp, ctypes.POINTER(OrParams), ctypes.POINTER(self.Result)]
         L.or_sharpness.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, vp, vp]
         L.or_sharpness.restype = ctypes.c_double
+        L.or_fsc_bins.argtypes = [ctypes.c_int, vp]
+        L.or_fsc.argtypes = [vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, vp]
 
     # -- FFT -------------------------------------------------------------
     def fft_c2c(self, x, sign=-1):
@@ -189,6 +192,25 @@ class Oracle:
                                   g.ctypes.data if want_grad else None)
         return J, g.reshape(nf, 2)
 
+    def fsc_bins(self, n):
+        st = ctypes.c_int()
+        nb = self.lib.or_fsc_bins(n, ctypes.addressof(st))
+        return nb, st.value
+
+    def fsc(self, a, b, direction):
+        """a, b: complex (n, n/2+1) half spectra.  Returns (nbox, 2): columns
+        parallel (D) and perpendicular (U) to `direction`."""
+        n = a.shape[0]
+        A = self.padded(a)
+        B = self.padded(b)
+        nb, _ = self.fsc_bins(n)
+        out = np.zeros((nb, 2))
+        rc = self.lib.or_fsc(A.ctypes.data, B.ctypes.data, n, float(direction[0]),
+                             float(direction[1]), out.ctypes.data)
+        if rc:
+            raise RuntimeError("or_fsc rc=%d" % rc)
+        return out
+
     # -- whole movie -------------------------------------------------------
     def align_movie(self, frames, params, frames2=None, want_spectra=False, want_boxes=True):
         """frames: (nz, ny, nx) float32.  params: dict of OrParams fields."""
@@ -220,6 +242,7 @@ class Oracle:
             final_shift=np.zeros((K, 2)), frame_shift=np.zeros((nframe, 2)),
             sum_image=np.zeros((n, n), self.real), sum2_image=np.zeros((n, n), self.real),
             rawsum_image=np.zeros((n, n), self.real),
+            fsc=np.zeros((self.lib.or_fsc_bins(n, None), 4)),
         )
         if want_boxes:
             out["boxmaps"] = np.zeros((npairs, box, box), self.real)

@@ -396,3 +396,100 @@ def test_sharpness_gradient_central_differences(o):
             sm[f, ax] -= h
             fd = (o.sharpness(spectra, n, 150.0, sp, False)[0] - o.sharpness(spectra, n, 150.0, sm, False)[0]) / (2 * h)
             assert abs(fd - g[f, ax]) <= 1e-6 * max(1.0, abs(J))
+
+
+# ------------------------------------------------------------ FSC table
+@pytest.mark.parametrize("n,nbox,step", [(4096, 205, 10), (8192, 205, 20), (1024, 257, 2),
+                                         (768, 386, 1), (512, 258, 1), (256, 130, 1)])
+def test_fsc_bins_closed_form(o, n, nbox, step):
+    """DFAlign.cu:214-218: step = n/400 (clamped to 1), nbox = (n/2+1)/step + 1."""
+    assert o.fsc_bins(n) == (nbox, step)
+
+
+def _rand_half(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal((n, n // 2 + 1)) + 1j * rng.standard_normal((n, n // 2 + 1))
+
+
+def test_fsc_closed_forms(o):
+    """b = c a (c > 0) gives 1, b = -a gives -1, b = i a gives 0, in every shell
+    that holds elements; shells beyond n/2+1 stay 0."""
+    n = 512
+    a = _rand_half(n, 40)
+    nb, _ = o.fsc_bins(n)
+    d = (0.3, -1.7)
+    one = o.fsc(a, 2.5 * a, d)
+    filled = o.fsc(a, a, d) != 0
+    # step 1: r >= 1 leaves shell 0 empty; r < 257 leaves shell 257 empty
+    assert filled[1: nb - 1].all() and not filled[0].any() and not filled[nb - 1].any()
+    assert np.abs(one[filled] - 1).max() < 1e-12
+    assert np.abs(o.fsc(a, -a, d)[filled] + 1).max() < 1e-12
+    assert np.abs(o.fsc(a, 1j * a, d)).max() < 1e-12
+
+
+def _delta(n, kx, y):
+    a = np.zeros((n, n // 2 + 1), complex)
+    a[y % n, kx] = 3.0 - 2.0j
+    return a
+
+
+@pytest.mark.parametrize("kx,y,d,shell,col", [
+    (100, -30, (1.0, 0.0), 52, 0),     # r = 104.40, |k.d|/r = 0.958
+    (100, -30, (0.0, 5.0), 52, 1),     # 0.287
+    (100, -30, (0.05, 0.0), 52, 0),    # |d| <= 0.1: direction (1, 0)
+    (100, -30, (-3.0, 4.0), 52, 0),    # |-60 - 24| / 104.4 = 0.805
+    (100, 30, (-3.0, 4.0), 52, 1),     # |-60 + 24| / 104.4 = 0.345: sign of y counts
+    (0, 1, (1.0, 0.0), 0, 1),          # r = 1: first shell
+    (512, 0, (1.0, 0.0), 256, 0),      # r = 512 < n/2+1: last shell
+    (0, -512, (0.0, 1.0), 256, 0),     # row n/2 is y = -n/2
+    (3, 3, (1.0, 0.0), 2, 0),          # float angle 0.7071068 > cos(pi/4)
+    (7, 7, (1.0, 0.0), 4, 1),          # float angle rounds to cos(pi/4): not greater
+])
+def test_fsc_single_element_shell_and_side(o, kx, y, d, shell, col):
+    """One nonzero element lands in shell int(sqrtf(kx^2+y^2)/step) on the side
+    fixed by the float angle test of DFAlign.cu:243-249 (n = 1024, step 2)."""
+    n = 1024
+    a = _delta(n, kx, y)
+    f = o.fsc(a, a, d)
+    expect = np.zeros_like(f)
+    expect[shell, col] = 1.0
+    assert np.array_equal(f, expect)
+
+
+@pytest.mark.parametrize("kx,y", [(0, 0), (512, -512), (400, 400)])
+def test_fsc_skips_dc_and_corners(o, kx, y):
+    n = 1024
+    a = _delta(n, kx, y)
+    assert not o.fsc(a, a, (1.0, 0.0)).any()
+
+
+def test_fsc_table_halves_of_a_movie(o):
+    """Raw halves are the unshifted frames at alternating positions of
+    [nStartSum, nEndSum]; aligned halves are the first summed frame plus every
+    second frame after it versus the rest, each with its final shift; the
+    direction is the total shift of the last summed frame (DFAlign.cu:808-822,
+    1143-1266, 1314-1318).  Frames alternate between two different images so a
+    wrong partition changes the table."""
+    F, n = 7, 128
+    rng = np.random.default_rng(41)
+    base = _rolled_movie(2, n, np.zeros((2, 2)), 41)
+    P = base[0]
+    Q = np.roll(P, 1, axis=0) + 0.3 * rng.standard_normal((n, n)).astype(np.float32)
+    drift = np.cumsum(rng.integers(-2, 3, size=(F, 2)), axis=0).astype(float)
+    frames = np.stack([np.roll(P if k % 2 == 0 else Q, (int(drift[k, 1]), int(drift[k, 0])), axis=(0, 1))
+                       for k in range(F)]).astype(np.float32)
+    prm = dict(fod=1, atm=0, box=32, nStartSum=1, nEndSum=5, nthreads=4)
+    r = o.align_movie(frames, prm, want_spectra=True, want_boxes=False)
+    spec = r["spectra"].view(np.complex128)  # (F, n, n/2+1)
+    fs = r["frame_shift"]
+    tot = -fs[5]  # atm 0: frame_shift = -total shift
+    raw = o.fsc(spec[1] + spec[3] + spec[5], spec[2] + spec[4], tot)
+
+    def sh(k):
+        return o.phase_shift(spec[k], fs[k, 0], fs[k, 1])
+    corr = o.fsc(sh(1) + sh(3) + sh(5), sh(2) + sh(4), tot)
+    assert np.abs(r["fsc"][:, :2] - raw).max() < 1e-10
+    assert np.abs(r["fsc"][:, 2:] - corr).max() < 1e-10
+    # the partition matters: a first-half / second-half split differs
+    other = o.fsc(spec[1] + spec[2], spec[3] + spec[4] + spec[5], tot)
+    assert np.abs(other - raw).max() > 1e-3
