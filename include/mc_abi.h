@@ -178,6 +178,8 @@ typedef struct {
   int precision;  /* MC_PREC_*                                          */
   int grad_steps; /* new: adjoint sharpness refinement steps (0 = off)  */
   float grad_step_px; /* new: trust-region step of the refinement (px)  */
+  float fftscale; /* APARA fftscale of the quick-look log-modulus images
+                     (0 = 0.0001, the reference default)                  */
 } mc_params;
 
 #define MC_PREC_FP32 0      /* FP32 on the vector ALUs (pair row pass: FP32 MFMA) */
@@ -212,7 +214,15 @@ typedef struct {
   void *sum_image_dev; /* [n*n] aligned sum left in device memory            */
   double *fsc;         /* [mc_fsc_bins(n)*4] FSC table (-fsc): per shell
                           D_Raw, U_Raw, D_Corr, U_Corr (DFAlign.cu:1314-1318) */
+  /* Quick-look images (-dsp 1, dosef_quick/; DFAlign.cu:73-179, 842-848,
+   * 908-911, 1270-1285; func.cpp:372-555): MC_DISPDIM^2 8-bit values each,
+   * written as MRC mode 1.  Need n >= MC_DISPDIM.                          */
+  short *disp_raw_fft;  /* _RawFFT: log-modulus display of the unaligned sum */
+  short *disp_corr_sum; /* _CorrSum: the aligned sum binned to 400 x 400     */
+  short *disp_corr_fft; /* _CorrFFT: log-modulus display of the aligned sum  */
 } mc_outputs;
+
+#define MC_DISPDIM 400 /* DFAlign.h:13 */
 
 typedef struct {
   int n, nsamUnbin, nframe, npairs, ngood, niter, converged;
@@ -307,6 +317,12 @@ int mc_fsc_bins(int n);
  * 2*mc_fsc_bins(n) doubles. */
 int mc_fsc_table(mc_context ctx, const void *a, const void *b, int n, double dx, double dy,
                  double *out);
+
+/* 8-bit display of `size` device floats into host shorts: histMinMax with
+ * 400 bins and a 0.5% threshold, then buf2DispShort (func.cpp:372-443), in
+ * the reference's float arithmetic.  A constant buffer gives zeros (the
+ * reference leaves its output unwritten). */
+int mc_disp_short(mc_context ctx, const float *buf, int size, short *out);
 
 /* Stage entry points on the context's resident state ------------------- */
 /* Stage 1 only: frames -> resident spectra (sizes written to rep). */

@@ -63,7 +63,7 @@ class Params(ctypes.Structure):
                 ("bAlignToMid", ctypes.c_int), ("iter", ctypes.c_int),
                 ("iterthres", ctypes.c_float), ("pad", ctypes.c_int),
                 ("precision", ctypes.c_int), ("grad_steps", ctypes.c_int),
-                ("grad_step_px", ctypes.c_float)]
+                ("grad_step_px", ctypes.c_float), ("fftscale", ctypes.c_float)]
 
 
 _P = ctypes.POINTER
@@ -80,7 +80,10 @@ class Outputs(ctypes.Structure):
                 ("frame_shift", _P(ctypes.c_double)), ("frame_mean", _P(ctypes.c_double)),
                 ("sum_image", _P(ctypes.c_float)), ("sum2_image", _P(ctypes.c_float)),
                 ("rawsum_image", _P(ctypes.c_float)), ("sum_image_dev", ctypes.c_void_p),
-                ("fsc", _P(ctypes.c_double))]
+                ("fsc", _P(ctypes.c_double)), ("disp_raw_fft", _P(ctypes.c_short)),
+                ("disp_corr_sum", _P(ctypes.c_short)), ("disp_corr_fft", _P(ctypes.c_short))]
+
+DISPDIM = 400  # MC_DISPDIM
 
 
 class Report(ctypes.Structure):
@@ -128,7 +131,7 @@ ABI_SYMBOLS = [
     "mc_set_spectra", "mc_get_spectra", "mc_pair_shifts", "mc_find_peaks", "mc_sum_image",
     "mc_pair_list", "mc_lsq", "mc_threshold", "mc_kernel_timing", "mc_kernel_timing_get",
     "mc_sharpness", "mc_align_batch", "mc_autotune", "mc_tune_load", "mc_fsc_bins",
-    "mc_fsc_table",
+    "mc_fsc_table", "mc_disp_short",
 ]
 
 _lib = None
@@ -185,6 +188,7 @@ def lib():
         L.mc_tune_load.argtypes = [vp, ctypes.c_char_p, _P(i)]
         L.mc_fsc_bins.argtypes = [i]
         L.mc_fsc_table.argtypes = [vp, vp, vp, i, d, d, vp]
+        L.mc_disp_short.argtypes = [vp, vp, i, vp]
         _lib = L
     return _lib
 
@@ -242,7 +246,8 @@ def default_params(nframe, **kw):
     p = dict(crop_offsetx=0, crop_offsety=0, crop_nsam=0, bin=1.0, nStart=0, nEnd=0,
              nStartSum=0, nEndSum=0, nEndSum2=0, bfactor=150.0, CCPeakSearchDim=96,
              FrameDistOffset=2, NoisePeakSize=0, kiThresh=1.0, bAlignToMid=1, iter=0,
-             iterthres=0.0, pad=0, precision=0, grad_steps=0, grad_step_px=0.0)
+             iterthres=0.0, pad=0, precision=0, grad_steps=0, grad_step_px=0.0,
+             fftscale=0.0001)
     p.update(kw)
     if p["nStart"] < 0:
         p["nStart"] = 0
@@ -320,12 +325,16 @@ class Context:
                      frame_shift=((F, 2), np.float64), frame_mean=((F,), np.float64),
                      sum_image=((n, n), np.float32), sum2_image=((n, n), np.float32),
                      rawsum_image=((n, n), np.float32),
-                     fsc=((max(self.L.mc_fsc_bins(n), 0), 4), np.float64))
+                     fsc=((max(self.L.mc_fsc_bins(n), 0), 4), np.float64),
+                     disp_raw_fft=((DISPDIM, DISPDIM), np.int16),
+                     disp_corr_sum=((DISPDIM, DISPDIM), np.int16),
+                     disp_corr_fft=((DISPDIM, DISPDIM), np.int16))
         o = Outputs()
         for k in want:
             shp, dt = sizes[k]
             res[k] = np.zeros(shp, dt)
-            ct = ctypes.c_int if dt == np.int32 else (ctypes.c_float if dt == np.float32 else ctypes.c_double)
+            ct = {np.int32: ctypes.c_int, np.int16: ctypes.c_short,
+                  np.float32: ctypes.c_float}.get(dt, ctypes.c_double)
             setattr(o, k, _ptr(res[k], ct))
         if sum_dev is not None:
             o.sum_image_dev = sum_dev.ptr
@@ -455,6 +464,12 @@ class Context:
         out = np.zeros((nb, 2))
         check(self.L.mc_fsc_table(self.h, dev_a.ptr, dev_b.ptr, n, float(direction[0]),
                                   float(direction[1]), out.ctypes.data))
+        return out
+
+    def disp_short(self, dev_buf, size):
+        """mc_disp_short: 8-bit display (int16) of `size` device floats."""
+        out = np.zeros(size, np.int16)
+        check(self.L.mc_disp_short(self.h, dev_buf.ptr, size, out.ctypes.data))
         return out
 
     def kernel_timing(self, enable):

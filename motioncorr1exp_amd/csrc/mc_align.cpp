@@ -99,7 +99,7 @@ struct mc_context_s {
   int device = 0;
   hipStream_t st = nullptr;
   DevBuf spec, rt, rowsum, fill, padP, pairs_d, tbuf, box, peaks, shifts, image, tmp, means,
-      sum2img, wk, gpart, gout, jobs, tmax, fsc;
+      sum2img, wk, gpart, gout, jobs, tmax, fsc, disp;
   int graph_captures = 0, graph_replays = 0;
   // autotuned launch shapes (mc_autotune / mc_tune_load)
   struct Tune {
@@ -859,6 +859,10 @@ static int prepare_run(MovieRun& r) {
   r.pairs.assign(2 * (size_t)r.P, 0);
   mc_pair_list(r.F, p->FrameDistOffset, r.pairs.data(), r.P);
   MC_TRY(check_corrections(m));
+  if (r.out && (r.out->disp_raw_fft || r.out->disp_corr_sum || r.out->disp_corr_fft) &&
+      (r.g.n < MC_DISPDIM || r.g.n % 128))
+    return set_err(MC_ERR_ARG, "quick-look images need a transform size >= %d (n %d)",
+                   MC_DISPDIM, r.g.n);
   r.nslot = r.F * ((m->frames2 || (m->corr && m->corr->second)) ? 2 : 1) + 1;
   return MC_OK;
 }
@@ -939,6 +943,55 @@ static void iter_update(MovieRun& r, const double* res) {
   if (r.rmsd < (double)r.p->iterthres * (double)r.p->iterthres) r.converged = 1;
   r.it++;
 }
+
+// Quick-look images of one sum spectrum ([kx][ky], size n >= 400, n % 128
+// == 0) into host buffers of MC_DISPDIM^2 shorts (DFAlign.cu:73-179,
+// 842-848, 908-911, 1270-1285; func.cpp:372-555).  sum_img: crop to 400,
+// C2R, 8-bit mapping.  fft_img: centred log-modulus image, its R2C binned
+// to 400 by the stage-1 transform, C2R, / 400^2, 8-bit mapping.
+static int quicklook(mc_context_s* c, const float2* spec, int n, float fftscale, short* fft_img,
+                     short* sum_img) {
+  constexpr int D = MC_DISPDIM;
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t b_img = al(sizeof(float) * (size_t)n * n);
+  const size_t b_rt = al(sizeof(float2) * mc::rt_frame_elems(D, n));
+  const size_t b_sp = al(sizeof(float2) * (size_t)(D / 2 + 1) * D);
+  const size_t b_out = al(sizeof(float) * (size_t)D * D);
+  const size_t b_sh = al(sizeof(short) * (size_t)D * D);
+  HIP_TRY(c->disp.ensure(b_img + b_rt + 2 * b_sp + b_out + b_sh));
+  char* base = static_cast<char*>(c->disp.p);
+  float* img = reinterpret_cast<float*>(base);
+  float2* rt = reinterpret_cast<float2*>(base + b_img);
+  float2* dsp = reinterpret_cast<float2*>(base + b_img + b_rt);
+  float2* tmp = reinterpret_cast<float2*>(base + b_img + b_rt + b_sp);
+  float* outf = reinterpret_cast<float*>(base + b_img + b_rt + 2 * b_sp);
+  short* sh = reinterpret_cast<short*>(base + b_img + b_rt + 2 * b_sp + b_out);
+  KScope ks(c, MC_KT_IMAGE);
+  if (sum_img) {
+    HIP_TRY(mc::launch_small_c2r(spec, n, D, 0.f, tmp, outf, c->st));
+    HIP_TRY(mc::launch_disp_short(outf, D * D, sh, c->st));
+    HIP_TRY(hipMemcpyAsync(sum_img, sh, sizeof(short) * D * D, hipMemcpyDeviceToHost, c->st));
+  }
+  if (fft_img) {
+    HIP_TRY(mc::launch_logmod_image(spec, n, fftscale, img, c->st));
+    mc::R2CGeom g{};
+    g.src = reinterpret_cast<const char*>(img);
+    g.dtype = mc::DT_FLOAT32;
+    g.row_pitch = sizeof(float) * (size_t)n;
+    g.frame_pitch = sizeof(float) * (size_t)n * n;
+    g.w = g.h = g.nu = n;
+    g.n = D;
+    g.nx = n;
+    HIP_TRY(mc::launch_frames_to_spectra(g, 0, 1, dsp, (size_t)(D / 2 + 1) * D, rt, nullptr,
+                                         nullptr, nullptr, mc::R2C_AUTO, c->st));
+    HIP_TRY(mc::launch_small_c2r(dsp, D, D, (float)(D * D), tmp, outf, c->st));
+    HIP_TRY(mc::launch_disp_short(outf, D * D, sh, c->st));
+    HIP_TRY(hipMemcpyAsync(fft_img, sh, sizeof(short) * D * D, hipMemcpyDeviceToHost, c->st));
+  }
+  return MC_OK;
+}
+
+static float fft_scale(const mc_params* p) { return p->fftscale > 0.f ? p->fftscale : 0.0001f; }
 
 // Increments after the iteration and the shift of every summed frame
 // (DFAlign.cu:1114-1148); frames outside [nss, nes] get NaN (not summed).
@@ -1083,6 +1136,8 @@ static int final_sum(mc_context_s* c, MovieRun& r) {
   if (out->sum_image)
     HIP_TRY(hipMemcpyAsync(out->sum_image, img, sizeof(float) * (size_t)n * n,
                            hipMemcpyDeviceToHost, c->st));
+  if (out->disp_corr_fft || out->disp_corr_sum)  // DFAlign.cu:1270-1285
+    MC_TRY(quicklook(c, slot(c, r.sum), n, fft_scale(p), out->disp_corr_fft, out->disp_corr_sum));
   if (want2) {
     // the second sum shares tmp: copy it into a spectrum slot first
     HIP_TRY(hipMemcpyAsync(slot(c, r.sum), s2, sizeof(float2) * c->spec_stride,
@@ -1221,7 +1276,7 @@ static int align_group(mc_context_s* c, std::vector<MovieRun*>& rs, bool use_gra
   }
   HIP_TRY(c->image.ensure(sizeof(float) * (size_t)n * n));
   for (MovieRun* r : rs) {
-    if (!r->out->rawsum_image) continue;
+    if (!r->out->rawsum_image && !r->out->disp_raw_fft) continue;
     // DFAlign.cu:808-823,853-857: unaligned sum of frames [nStartSum, nEndSum]
     r->sh.assign(2 * (size_t)r->F, NAN);
     for (int f = 0; f < r->F; f++) {
@@ -1229,11 +1284,15 @@ static int align_group(mc_context_s* c, std::vector<MovieRun*>& rs, bool use_gra
       if (j >= r->p->nStartSum && j <= r->p->nEndSum) r->sh[2 * f] = r->sh[2 * f + 1] = 0.0;
     }
     MC_TRY(shift_sum(c, r->base, r->F, r->sh, slot(c, r->sum), nullptr, -1));
-    MC_TRY(to_image(c, slot(c, r->sum), c->image.as<float>()));
-    HIP_TRY(hipMemcpyAsync(r->out->rawsum_image, c->image.p, sizeof(float) * (size_t)n * n,
-                           hipMemcpyDeviceToHost, c->st));
+    if (r->out->disp_raw_fft)  // DFAlign.cu:842-848, 908-911
+      MC_TRY(quicklook(c, slot(c, r->sum), n, fft_scale(r->p), r->out->disp_raw_fft, nullptr));
+    if (r->out->rawsum_image) {
+      MC_TRY(to_image(c, slot(c, r->sum), c->image.as<float>()));
+      HIP_TRY(hipMemcpyAsync(r->out->rawsum_image, c->image.p, sizeof(float) * (size_t)n * n,
+                             hipMemcpyDeviceToHost, c->st));
+      r->rawsum_written = 1;
+    }
     HIP_TRY(hipStreamSynchronize(c->st));
-    r->rawsum_written = 1;
   }
   HIP_TRY(hipEventRecord(ev[1], c->st));
 
@@ -1365,6 +1424,17 @@ static void fill_report(const MovieRun& r, const GroupTimes& tm) {
   rep->ms_iter = tm.iter;
   rep->ms_sum = tm.sum;
   rep->ms_total = tm.total;
+}
+
+extern "C" int mc_disp_short(mc_context ctx, const float* buf, int size, short* out) {
+  if (!ctx || !buf || !out || size <= 0) return set_err(MC_ERR_ARG, "bad argument");
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(ctx->disp.ensure(sizeof(short) * (size_t)size));
+  short* d = ctx->disp.as<short>();
+  HIP_TRY(mc::launch_disp_short(buf, size, d, ctx->st));
+  HIP_TRY(hipMemcpyAsync(out, d, sizeof(short) * (size_t)size, hipMemcpyDeviceToHost, ctx->st));
+  HIP_TRY(hipStreamSynchronize(ctx->st));
+  return MC_OK;
 }
 
 extern "C" int mc_fsc_bins(int n) {

@@ -380,6 +380,45 @@ MC_DEV void fft_lds_rw(float2* buf, const FftTw<N, NT, S, NB>& tw) {
   if constexpr (stage_count(N) > 3) fft_lds_stage_rw<N, NT, S, NB, 3, TS, PB>(buf, tw.s3);
 }
 
+// -------------------------------------------- float results rounded exactly
+// Host code of the reference computes some float square roots and quotients
+// whose rounding decides a discrete outcome (a shell index, a histogram
+// bin).  These return the correctly rounded float result whatever the
+// compiler's lowering of sqrtf and '/': the first guess is corrected against
+// the midpoints to its neighbours, whose squares (products with b) are exact
+// in FP64 (25-bit by 24-bit significands); such square roots and quotients
+// never fall on a midpoint.  Operands are positive and finite.
+MC_DEV float f32_next(float x, int d) { return __int_as_float(__float_as_int(x) + d); }
+
+MC_DEV float sqrt_rn(float q) {  // q > 0
+  float r = sqrtf(q);
+#pragma unroll
+  for (int it = 0; it < 2; it++) {
+    const double hi = 0.5 * ((double)r + (double)f32_next(r, 1));
+    const double lo = 0.5 * ((double)r + (double)f32_next(r, -1));
+    if ((double)q > hi * hi)
+      r = f32_next(r, 1);
+    else if ((double)q < lo * lo)
+      r = f32_next(r, -1);
+  }
+  return r;
+}
+
+MC_DEV float div_rn(float a, float b) {  // a >= 0, b > 0
+  if (a == 0.f) return 0.f;
+  float d = (float)((double)a / (double)b);
+#pragma unroll
+  for (int it = 0; it < 2; it++) {
+    const double hi = 0.5 * ((double)d + (double)f32_next(d, 1));
+    const double lo = 0.5 * ((double)d + (double)f32_next(d, -1));
+    if ((double)a > hi * (double)b)
+      d = f32_next(d, 1);
+    else if ((double)a < lo * (double)b)
+      d = f32_next(d, -1);
+  }
+  return d;
+}
+
 // --------------------------------------------------------- XCD remapping
 // gfx950 dispatches consecutive workgroups round-robin over 8 XCDs, each
 // with its own L2.  Remap so that a contiguous range of logical block ids

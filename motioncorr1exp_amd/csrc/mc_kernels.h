@@ -114,6 +114,17 @@ int fsc_grid(int n);
 hipError_t launch_fsc_sums(const float2* a, const float2* b, int n, int padded, double dx,
                            double dy, double* part, double* sums, hipStream_t st);
 
+// Quick-look images (mc_disp.hip).  Centred n x n log-modulus image
+// log(1 + |X| scale) of a [kx][ky] spectrum (n % 128 == 0); d-point C2R of
+// the d-row crop of a [kx][ky] spectrum of column length ld (divided by div
+// when div > 0; tmp: (d/2+1)*d float2); the 8-bit display mapping of `size`
+// floats (one workgroup).
+hipError_t launch_logmod_image(const float2* spec, int n, float scale, float* img,
+                               hipStream_t st);
+hipError_t launch_small_c2r(const float2* X, int ld, int d, float div, float2* tmp, float* out,
+                            hipStream_t st);
+hipError_t launch_disp_short(const float* buf, int size, short* out, hipStream_t st);
+
 // Sharpness J of the shifted sum S (already formed by launch_shift_sum with
 // the same shifts) and dJ/ds_f: out[f] = (dJ/dsx_f, dJ/dsy_f), out[F] = (J, 0).
 // wk: [n/2+1] column weights (incl. the half-plane factor) then [n] row

@@ -179,40 +179,9 @@ hipError_t launch_half_sums(const float2* spectra, size_t spec_stride, int count
 // side 1.  Per (shell, side) the sums of Re(a conj b), |a|^2 and |b|^2 are
 // taken in FP64.
 //
-// The float square root and division are rounded to nearest with an exact
-// check against the neighbouring midpoints (their squares and their
-// products with step are exact in FP64; sqrt and quotients of these operands
-// never fall on a midpoint), so shell membership is the reference's whatever
-// the compiler's lowering of sqrtf and '/'.
-__device__ inline float f32_next(float x, int d) { return __int_as_float(__float_as_int(x) + d); }
-
-__device__ inline float sqrt_rn(float q) {  // q >= 1
-  float r = sqrtf(q);
-#pragma unroll
-  for (int it = 0; it < 2; it++) {
-    const double hi = 0.5 * ((double)r + (double)f32_next(r, 1));
-    const double lo = 0.5 * ((double)r + (double)f32_next(r, -1));
-    if ((double)q > hi * hi)
-      r = f32_next(r, 1);
-    else if ((double)q < lo * lo)
-      r = f32_next(r, -1);
-  }
-  return r;
-}
-
-__device__ inline float div_rn(float a, int step) {  // a >= 1, step >= 1
-  float d = (float)((double)a / (double)step);
-#pragma unroll
-  for (int it = 0; it < 2; it++) {
-    const double hi = 0.5 * ((double)d + (double)f32_next(d, 1));
-    const double lo = 0.5 * ((double)d + (double)f32_next(d, -1));
-    if ((double)a > hi * step)
-      d = f32_next(d, 1);
-    else if ((double)a < lo * step)
-      d = f32_next(d, -1);
-  }
-  return d;
-}
+// The float square root and division are rounded to nearest exactly
+// (sqrt_rn, div_rn), so shell membership is the reference's whatever the
+// compiler's lowering of sqrtf and '/'.
 
 // Side of (kx, y): FP64 products, sum and quotient each rounded once as in
 // the reference's host code (no fused multiply-add).
@@ -262,7 +231,7 @@ __global__ __launch_bounds__(FSC_NW * 64) void k_fsc_partial(const float2* __res
       if (q > 0) {
         const float r = sqrt_rn((float)q);
         if ((int)r > 0 && r < (float)H) {
-          const int id = min((int)div_rn(r, step), nbox - 1);  // r < H keeps id < nbox
+          const int id = min((int)div_rn(r, (float)step), nbox - 1);  // r < H keeps id < nbox
           key = 2 * id + fsc_side(kx, y, r, dx, dy);
           const float2 va = a[e], vb = b[e];
           sab = (double)va.x * vb.x + (double)va.y * vb.y;

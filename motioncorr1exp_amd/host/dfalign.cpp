@@ -296,6 +296,7 @@ void CDFAlign::AlignWorker() {
   mp.iter = m_iter;
   mp.iterthres = m_iterthres;
   mp.pad = para.pad;
+  mp.fftscale = para.fftscale;
 
   const int box = para.CCPeakSearchDim;
   const int P = mc_pair_list(nframe, para.FrameDistOffset, nullptr, 0);
@@ -308,6 +309,17 @@ void CDFAlign::AlignWorker() {
   std::vector<float> sum((size_t)nsam * nsam), sum2, rawsum, boxes, iboxes;
   if (para.bSaveRawSum) rawsum.resize((size_t)nsam * nsam);
   if (para.nEndSum2 > 0 && para.nEndSum2 < para.nEndSum) sum2.resize((size_t)nsam * nsam);
+  // quick-look images (dosef_quick/, DFAlign.cu:73-179, 842-848, 908-911, 1270-1285)
+  std::vector<short> disp_raw, disp_sum, disp_fft;
+  if (m_bSaveDisp && nsam >= MC_DISPDIM) {
+    const size_t dd = (size_t)MC_DISPDIM * MC_DISPDIM;
+    if (para.bDispFFTRaw) disp_raw.resize(dd);
+    if (para.bDispSumCorr) disp_sum.resize(dd);
+    if (para.bDispFFTCorr) disp_fft.resize(dd);
+  } else if (m_bSaveDisp) {
+    Message(fmt("Quick-look images need a transform size of at least %d (here %d): "
+                "none written.", MC_DISPDIM, nsam).c_str());
+  }
   std::vector<double> fsc;  // DFAlign.cu:1314-1318
   if (para.bLogFSC) {
     const int nb = mc_fsc_bins(nsam);
@@ -339,6 +351,17 @@ void CDFAlign::AlignWorker() {
   out.sum2_image = sum2.empty() ? nullptr : sum2.data();
   out.rawsum_image = rawsum.empty() ? nullptr : rawsum.data();
   out.fsc = fsc.empty() ? nullptr : fsc.data();
+  out.disp_raw_fft = disp_raw.empty() ? nullptr : disp_raw.data();
+  out.disp_corr_sum = disp_sum.empty() ? nullptr : disp_sum.data();
+  out.disp_corr_fft = disp_fft.empty() ? nullptr : disp_fft.data();
+  auto write_disp = [&](const char* fn, const std::vector<short>& im) {
+    if (im.empty() || !fn[0]) return;
+    MRC m;
+    if (m.open(fn, "wb") > 0) {
+      m.createMRC(im.data(), MC_DISPDIM, MC_DISPDIM, 1);
+      m.close();
+    }
+  };
   mc_report rep;
   mc_context ctx = nullptr;
   int rc = mc_context_create(para.GPUNum, &ctx);
@@ -375,6 +398,7 @@ void CDFAlign::AlignWorker() {
                    pairs[2 * q] + para.nStart, pairs[2 * q + 1] + para.nStart, (float)pb[2 * q],
                    (float)pb[2 * q + 1], (float)pcc[q])
                    .c_str());
+  write_disp(m_dispRawFFT, disp_raw);  // DFAlign.cu:907-911
   TextOutput("\n......ki: First round \n");
   double kis = 0;
   for (int q = 0; q < P; q++) {
@@ -523,6 +547,8 @@ void CDFAlign::AlignWorker() {
   };
   if (!sum2.empty() && rep.sum2_written) write_sum(m_fnAlignsum2, sum2);
   write_sum(m_fnAlignsum, sum);
+  write_disp(m_dispCorrFFT, disp_fft);  // DFAlign.cu:1271-1285
+  write_disp(m_dispCorrSum, disp_sum);
   if (!fsc.empty()) {
     // DFAlign.cu:311-320
     const int nb = (int)(fsc.size() / 4);

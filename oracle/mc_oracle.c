@@ -692,6 +692,8 @@ static void c2r_to_image(R *spec, int n, R *img) {
     for (int x = 0; x < n; x++) img[(size_t)y * n + x] = spec[(size_t)y * (n + 2) + x] * sc;
 }
 
+static int disp_image(const R *spec, int n, float scale, int fft, short *out);
+
 static void add_spec(R *dst, const R *src, size_t len) {
   for (size_t i = 0; i < len; i++) dst[i] += src[i];
 }
@@ -778,10 +780,17 @@ int or_align_movie(const float *frames, const float *frames2,
     fsc_raw1 = rs1;
     rs1 = NULL;
   }
-  if (r->rawsum_image) {
+  const float fftscale = p->fftscale > 0 ? p->fftscale : 0.0001f;
+  r->disp_written = 0;
+  if (r->rawsum_image || (r->disp_raw_fft && n >= OR_DISPDIM)) {
     add_spec(rs0, fsc_raw1 ? fsc_raw1 : rs1, sb); /* DFAlign.cu:823 */
-    c2r_to_image(rs0, n, r->rawsum_image);
-    r->rawsum_written = 1;
+    /* DFAlign.cu:842-848, 908-911: log-modulus display of the raw sum */
+    if (r->disp_raw_fft && n >= OR_DISPDIM && disp_image(rs0, n, fftscale, 1, r->disp_raw_fft))
+      r->disp_written |= 1;
+    if (r->rawsum_image) {
+      c2r_to_image(rs0, n, r->rawsum_image);
+      r->rawsum_written = 1;
+    }
   }
   free(rs0);
   free(rs1);
@@ -962,6 +971,11 @@ int or_align_movie(const float *frames, const float *frames2,
       free(t);
     }
     add_spec(dsumcorr, dsum, sb);
+    /* DFAlign.cu:1270-1285 quick-look images of the aligned sum */
+    if (r->disp_corr_fft && n >= OR_DISPDIM && disp_image(dsumcorr, n, fftscale, 1, r->disp_corr_fft))
+      r->disp_written |= 4;
+    if (r->disp_corr_sum && n >= OR_DISPDIM && disp_image(dsumcorr, n, fftscale, 0, r->disp_corr_sum))
+      r->disp_written |= 2;
     if (r->sum_image) c2r_to_image(dsumcorr, n, r->sum_image);
     free(fr);
     free(dsum);
@@ -1040,6 +1054,125 @@ int or_fsc(const R *a, const R *b, int n, double dx, double dy, double *out) {
   }
   free(acc);
   return 0;
+}
+
+/* cufunc.cu:178-188 and func.cpp:413-447 */
+int or_logmod_image(const R *spec, int n, float scale, R *img) {
+  if (n < 4 || (n & 1)) return -1;
+  const int hn = n / 2;
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) {
+      int kx, ky;
+      if (i >= hn) { /* right half: rows fftshifted */
+        kx = i - hn;
+        ky = (j + hn) % n;
+      } else if (j >= 1) { /* left half: point mirror of the right half */
+        kx = hn - i;
+        ky = (hn - j + n) % n;
+      } else { /* row 0 copies row 1 */
+        kx = hn - i;
+        ky = hn - 1;
+      }
+      const cpx v = ((const cpx *)(spec + (size_t)ky * (n + 2)))[kx];
+      img[(size_t)j * n + i] = (R)log(1.0 + sqrt((double)v.re * v.re + (double)v.im * v.im) * scale);
+    }
+  return 0;
+}
+
+/* func.cpp:487-523 up to buf2DispShort */
+int or_fft_display(const R *spec, int n, float scale, int d, R *out) {
+  if (n < d || d < 2 || (d & 1)) return -1;
+  R *buf = (R *)calloc((size_t)n * (n + 2), sizeof(R));
+  R *img = (R *)malloc(sizeof(R) * (size_t)n * n);
+  R *c = (R *)calloc((size_t)d * (d + 2), sizeof(R));
+  or_logmod_image(spec, n, scale, img);
+  for (int j = 0; j < n; j++) memcpy(buf + (size_t)j * (n + 2), img + (size_t)j * n, sizeof(R) * n);
+  or_r2c2d(buf, n);
+  or_bin_crop(c, buf, d, n);
+  or_c2r2d(c, d);
+  const R dd = (R)((double)d * d);
+  for (int y = 0; y < d; y++)
+    for (int x = 0; x < d; x++) out[(size_t)y * d + x] = c[(size_t)y * (d + 2) + x] / dd;
+  free(buf);
+  free(img);
+  free(c);
+  return 0;
+}
+
+/* DFAlign.cu:1281-1284, 73-90 */
+int or_sum_display(const R *spec, int n, int d, R *out) {
+  if (n < d || d < 2 || (d & 1)) return -1;
+  R *c = (R *)calloc((size_t)d * (d + 2), sizeof(R));
+  or_bin_crop(c, spec, d, n);
+  or_c2r2d(c, d);
+  for (int y = 0; y < d; y++)
+    for (int x = 0; x < d; x++) out[(size_t)y * d + x] = c[(size_t)y * (d + 2) + x];
+  free(c);
+  return 0;
+}
+
+/* func.cpp:372-411 (histMinMax, threshold 0.005) and 429-443 (buf2DispShort) */
+int or_disp_short(const float *buf, int size, short *out) {
+  const int nbin = 400;
+  const float threshold = 0.005f;
+  float mn = 1e20f, mx = -1e20f;
+  for (int i = 0; i < size; i++) {
+    if (buf[i] < mn) mn = buf[i];
+    if (buf[i] > mx) mx = buf[i];
+  }
+  if (mn >= mx) return 1;
+  int *hist = (int *)calloc(nbin, sizeof(int));
+  const float step = (mx - mn) / nbin;
+  for (int i = 0; i < size; i++) {
+    int id = (int)((buf[i] - mn) / step);
+    if (id >= nbin) id = nbin - 1;
+    hist[id]++;
+  }
+  const int percent = (int)(size * threshold);
+  int count = 0;
+  for (int i = 0; i < nbin; i++) {
+    count += hist[i];
+    if (count > percent) {
+      mn += i * step;
+      break;
+    }
+  }
+  count = 0;
+  for (int i = nbin - 1; i > 0; i--) {
+    count += hist[i];
+    if (count > percent) {
+      mx -= (nbin - 1 - i) * step;
+      break;
+    }
+  }
+  free(hist);
+  if (mn >= mx) return 1;
+  const float sc = 255.0 / (mx - mn);
+  for (int i = 0; i < size; i++) {
+    if (buf[i] <= mn)
+      out[i] = 0;
+    else if (buf[i] >= mx)
+      out[i] = 255;
+    else
+      out[i] = (short)((buf[i] - mn) * sc);
+  }
+  return 0;
+}
+
+/* One quick-look image: the float display buffer, then the 8-bit mapping. */
+static int disp_image(const R *spec, int n, float scale, int fft, short *out) {
+  const int d = OR_DISPDIM;
+  R *f = (R *)malloc(sizeof(R) * (size_t)d * d);
+  float *ff = (float *)malloc(sizeof(float) * (size_t)d * d);
+  if (fft)
+    or_fft_display(spec, n, scale, d, f);
+  else
+    or_sum_display(spec, n, d, f);
+  for (size_t i = 0; i < (size_t)d * d; i++) ff[i] = (float)f[i];
+  const int rc = or_disp_short(ff, d * d, out);
+  free(f);
+  free(ff);
+  return rc == 0;
 }
 
 /* Sharpness objective and gradient (no reference counterpart). */

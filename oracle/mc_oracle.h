@@ -123,6 +123,7 @@ typedef struct {
   int full_c2r;                   /* 1: full n x n inverse per pair as the
                                      reference does; 0: pruned box inverse */
   int nthreads;                   /* OpenMP threads (0 = default)           */
+  float fftscale;                 /* APARA fftscale (0: 0.0001, the default) */
 } or_params;
 
 typedef struct {
@@ -149,6 +150,10 @@ typedef struct {
   int sum2_written, rawsum_written;
   double *fsc;           /* [or_fsc_bins(n)*4] FSC table: per shell D_Raw,
                             U_Raw, D_Corr, U_Corr (DFAlign.cu:1314-1318)   */
+  short *disp_raw_fft;   /* [400*400] dosef_quick/ _RawFFT (n >= 400)       */
+  short *disp_corr_sum;  /* [400*400] _CorrSum                              */
+  short *disp_corr_fft;  /* [400*400] _CorrFFT                              */
+  int disp_written;      /* bit 0 raw FFT, bit 1 sum, bit 2 corrected FFT   */
 } or_result;
 
 /* frames: nz x ny x nx float (decoded, dark/gain corrected = hbuf source);
@@ -171,6 +176,24 @@ int or_fsc_bins(int n, int *step);
  * <= 1e-5.  DFAlign.cu:212-297. */
 int or_fsc(const or_real *a, const or_real *b, int n, double dx, double dy,
            double *out);
+
+/* ---- quick-look images (dosef_quick/) ------------------------------------ */
+#define OR_DISPDIM 400 /* DFAlign.h:13 */
+/* Log-modulus log(1 + |X| scale) of the half spectrum spec (padded layout;
+ * cufunc.cu:178-188) laid out as the full n x n image (pitch n) with DC at
+ * row n/2, column n/2 (func.cpp:413-447 FFTModulusToDispBuf). */
+int or_logmod_image(const or_real *spec, int n, float scale, or_real *img);
+/* func.cpp:487-523 (BinFFTDispBufToChar up to the display mapping): R2C of
+ * the log-modulus image, Fourier crop to d x d (the row pattern of
+ * cufunc.cu:147-161), C2R, / d^2.  out: d x d.  n >= d. */
+int or_fft_display(const or_real *spec, int n, float scale, int d, or_real *out);
+/* DFAlign.cu:1281-1284 with ImageOutputThread (DFAlign.cu:73-90): Fourier
+ * crop of the sum spectrum to d x d, C2R, no scaling.  n >= d. */
+int or_sum_display(const or_real *spec, int n, int d, or_real *out);
+/* func.cpp:372-411 histMinMax (400 bins, threshold 0.005) and
+ * func.cpp:429-443 buf2DispShort, in the reference's float arithmetic.
+ * Returns 1 (out untouched) when the clipped range is empty, else 0. */
+int or_disp_short(const float *buf, int size, short *out);
 
 /* Sharpness objective of the phase-shift-and-sum and its gradient with
  * respect to per-frame shifts (no reference counterpart):

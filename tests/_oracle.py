@@ -22,7 +22,7 @@ class OrParams(ctypes.Structure):
         ("bfactor", ctypes.c_float), ("box", ctypes.c_int), ("fod", ctypes.c_int),
         ("nps", ctypes.c_int), ("kit", ctypes.c_float), ("atm", ctypes.c_int),
         ("iter", ctypes.c_int), ("iterthres", ctypes.c_float),
-        ("full_c2r", ctypes.c_int), ("nthreads", ctypes.c_int),
+        ("full_c2r", ctypes.c_int), ("nthreads", ctypes.c_int), ("fftscale", ctypes.c_float),
     ]
 
 
@@ -42,7 +42,9 @@ def _result_struct(real_t):
             ("iter_meas", dp), ("rmsd", dp), ("final_shift", dp), ("frame_shift", dp),
             ("sum_image", rp), ("sum2_image", rp), ("rawsum_image", rp), ("spectra", rp),
             ("sum2_written", ctypes.c_int), ("rawsum_written", ctypes.c_int),
-            ("fsc", dp),
+            ("fsc", dp), ("disp_raw_fft", P(ctypes.c_short)),
+            ("disp_corr_sum", P(ctypes.c_short)), ("disp_corr_fft", P(ctypes.c_short)),
+            ("disp_written", ctypes.c_int),
         ]
     return OrResult
 
@@ -86,6 +88,10 @@ class Oracle:
         L.or_sharpness.restype = ctypes.c_double
         L.or_fsc_bins.argtypes = [ctypes.c_int, vp]
         L.or_fsc.argtypes = [vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, vp]
+        L.or_logmod_image.argtypes = [vp, ctypes.c_int, ctypes.c_float, vp]
+        L.or_fft_display.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_int, vp]
+        L.or_sum_display.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+        L.or_disp_short.argtypes = [vp, ctypes.c_int, vp]
 
     # -- FFT -------------------------------------------------------------
     def fft_c2c(self, x, sign=-1):
@@ -211,8 +217,42 @@ class Oracle:
             raise RuntimeError("or_fsc rc=%d" % rc)
         return out
 
+    # -- quick-look images --------------------------------------------------
+    def logmod_image(self, spec, scale=1e-4):
+        n = spec.shape[0]
+        S = self.padded(spec)
+        img = np.zeros((n, n), self.real)
+        if self.lib.or_logmod_image(S.ctypes.data, n, scale, img.ctypes.data):
+            raise RuntimeError("or_logmod_image")
+        return img
+
+    def fft_display(self, spec, scale=1e-4, d=400):
+        n = spec.shape[0]
+        S = self.padded(spec)
+        out = np.zeros((d, d), self.real)
+        if self.lib.or_fft_display(S.ctypes.data, n, scale, d, out.ctypes.data):
+            raise RuntimeError("or_fft_display")
+        return out
+
+    def sum_display(self, spec, d=400):
+        n = spec.shape[0]
+        S = self.padded(spec)
+        out = np.zeros((d, d), self.real)
+        if self.lib.or_sum_display(S.ctypes.data, n, d, out.ctypes.data):
+            raise RuntimeError("or_sum_display")
+        return out
+
+    def disp_short(self, buf):
+        """Returns the 8-bit display (int16) or None when the clipped range is
+        empty (the reference leaves the image unwritten)."""
+        b = np.ascontiguousarray(buf, np.float32)
+        out = np.zeros(b.shape, np.int16)
+        rc = self.lib.or_disp_short(b.ctypes.data, b.size, out.ctypes.data)
+        return None if rc else out
+
     # -- whole movie -------------------------------------------------------
-    def align_movie(self, frames, params, frames2=None, want_spectra=False, want_boxes=True):
+    def align_movie(self, frames, params, frames2=None, want_spectra=False, want_boxes=True,
+                    want_disp=False):
         """frames: (nz, ny, nx) float32.  params: dict of OrParams fields."""
         frames = np.ascontiguousarray(frames, np.float32)
         nz, ny, nx = frames.shape
@@ -220,7 +260,7 @@ class Oracle:
         defaults = dict(nx=nx, ny=ny, nz=nz, crop_offsetx=0, crop_offsety=0, crop_nsam=0,
                         bin=1.0, pad=0, nStart=0, nEnd=nz - 1, nStartSum=0, nEndSum=nz - 1,
                         nEndSum2=0, bfactor=150.0, box=96, fod=2, nps=0, kit=1.0, atm=1,
-                        iter=0, iterthres=0.0, full_c2r=1, nthreads=0)
+                        iter=0, iterthres=0.0, full_c2r=1, nthreads=0, fftscale=0.0)
         defaults.update(params)
         for k, v in defaults.items():
             setattr(p, k, v)
@@ -248,6 +288,9 @@ class Oracle:
             out["boxmaps"] = np.zeros((npairs, box, box), self.real)
         if want_spectra:
             out["spectra"] = np.zeros((nframe, n, n + 2), self.real)
+        if want_disp:
+            for k in ("disp_raw_fft", "disp_corr_sum", "disp_corr_fft"):
+                out[k] = np.zeros((400, 400), np.int16)
         r = self.Result()
         for k, v in out.items():
             setattr(r, k, v.ctypes.data_as(type(getattr(r, k))))
@@ -270,4 +313,5 @@ class Oracle:
         out["rmsd"] = out["rmsd"][: r.niter]
         out["sum2_written"] = r.sum2_written
         out["rawsum_written"] = r.rawsum_written
+        out["disp_written"] = r.disp_written
         return out

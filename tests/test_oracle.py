@@ -493,3 +493,101 @@ def test_fsc_table_halves_of_a_movie(o):
     # the partition matters: a first-half / second-half split differs
     other = o.fsc(spec[1] + spec[2], spec[3] + spec[4] + spec[5], tot)
     assert np.abs(other - raw).max() > 1e-3
+
+
+# ------------------------------------------------------- quick-look images
+def test_logmod_image_positions(o):
+    """func.cpp:413-447: the right half holds kx = i - n/2 of row ky = (j + n/2)
+    mod n, the left half its point mirror, and row 0 copies row 1; each
+    pixel is log(1 + |X| scale) (cufunc.cu:178-188)."""
+    n, hn, sc = 64, 32, 1.0 / 64  # scale exact in float
+    cases = [
+        (5, n - 3, [(hn - 3, hn + 5), (hn + 3, hn - 5)]),
+        (0, 0, [(hn, hn)]),
+        (hn, 4, [(hn - 4, 0)]),        # Nyquist column: left edge only
+        (7, hn - 1, [(n - 1, hn + 7), (1, hn - 7), (0, hn - 7)]),
+    ]
+    for kx, ky, pix in cases:
+        X = np.zeros((n, hn + 1), complex)
+        X[ky, kx] = 300.0 + 400.0j
+        img = o.logmod_image(X, sc)
+        expect = np.zeros((n, n))
+        for j, i in pix:
+            expect[j, i] = np.log(1 + 500.0 * sc)
+        assert np.abs(img - expect).max() < 1e-12, (kx, ky)
+
+
+def test_sum_display_is_cosine_on_the_display_grid(o):
+    """A cosine below the crop frequency: crop to 400 and the unnormalised
+    C2R give n^2 cos(2 pi (3X -/+ 5Y)/400) (DFAlign.cu:1281-1284, 73-90)."""
+    n, d = 512, 400
+    y, x = np.mgrid[0:n, 0:n]
+    Y, X = np.mgrid[0:d, 0:d]
+    for sy in (5, -5):
+        img = np.cos(2 * np.pi * (3 * x + sy * y) / n)
+        got = o.sum_display(np.fft.rfft2(img))
+        assert np.abs(got - n * n * np.cos(2 * np.pi * (3 * X + sy * Y) / d)).max() < 1e-6 * n * n
+
+
+def test_fft_display_matches_numpy(o):
+    """R2C of the log-modulus image, Fourier crop, C2R and / d^2 equal numpy's
+    normalised irfft2 of the cropped rfft2 (func.cpp:487-523)."""
+    n, d = 512, 400
+    rng = np.random.default_rng(50)
+    spec = np.fft.rfft2(rng.poisson(3.0, (n, n)).astype(float))
+    img = o.logmod_image(spec, 1e-4)
+    F = np.fft.rfft2(img)
+    crop = np.concatenate([F[: d // 2, : d // 2 + 1], F[n - d // 2:, : d // 2 + 1]])
+    ref = np.fft.irfft2(crop, s=(d, d))
+    got = o.fft_display(spec, 1e-4, d)
+    assert np.abs(got - ref).max() < 1e-9 * np.abs(ref).max()
+
+
+def test_disp_short_two_levels_and_constant(o):
+    b = np.zeros(1000, np.float32)
+    b[500:] = 1.0
+    out = o.disp_short(b)
+    assert (out[:500] == 0).all() and (out[500:] == 255).all()
+    assert o.disp_short(np.full(1000, 3.0, np.float32)) is None
+
+
+def test_disp_short_ramp_clips_half_a_percent(o):
+    """Uniform ramp of 160000 values: 400 values per histogram bin and a 0.5%
+    threshold of 800 values put the clip points two bins from either end; in
+    between the mapping is linear (func.cpp:372-443)."""
+    N = 160000
+    v = np.arange(N, dtype=np.float32)
+    out = o.disp_short(v)
+    step = (N - 1) / 400
+    lo, hi = 2 * step, (N - 1) - 2 * step
+    ref = np.clip(np.floor((v - lo) * 255.0 / (hi - lo)), 0, 255)
+    ref[v >= hi] = 255
+    assert np.abs(out - ref).max() <= 1
+    assert (np.diff(out.astype(int)) >= 0).all()
+    # a permutation of the pixels permutes the display
+    perm = np.random.default_rng(51).permutation(N)
+    assert np.array_equal(o.disp_short(v[perm]), out[perm])
+
+
+def test_movie_quicklook_inputs(o):
+    """_RawFFT is the display of the unaligned sum of [nStartSum, nEndSum],
+    _CorrSum and _CorrFFT those of the aligned sum (DFAlign.cu:842-848,
+    1270-1285)."""
+    from motioncorr1exp_amd.synth import make_movie
+    F, n = 6, 512
+    frames, _ = make_movie(n, n, F, dose=2.0, step=0.8, seed=52)
+    r = o.align_movie(frames, dict(fod=1, box=32, nStartSum=1, nEndSum=4, nthreads=8),
+                      want_spectra=True, want_boxes=False, want_disp=True)
+    assert r["disp_written"] == 7
+    spec = r["spectra"].view(np.complex128)
+    fs = r["frame_shift"]
+    raw = spec[1:5].sum(axis=0)
+    corr = sum(o.phase_shift(spec[k], fs[k, 0], fs[k, 1]) for k in range(1, 5))
+
+    def close(a, b):
+        return np.abs(a.astype(int) - b.astype(int)).max() <= 1 and (a == b).mean() > 0.999
+    assert close(r["disp_raw_fft"], o.disp_short(o.fft_display(raw).astype(np.float32)))
+    assert close(r["disp_corr_fft"], o.disp_short(o.fft_display(corr).astype(np.float32)))
+    assert close(r["disp_corr_sum"], o.disp_short(o.sum_display(corr).astype(np.float32)))
+    # the unaligned and aligned displays differ
+    assert (r["disp_raw_fft"] != r["disp_corr_fft"]).mean() > 0.01
