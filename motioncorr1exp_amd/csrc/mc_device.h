@@ -242,6 +242,21 @@ template <int PB> MC_DEV int lix(int i) { return PB ? i + (i >> PB) : i; }
 template <int PB> __host__ __device__ constexpr int lds_len(int n) {
   return PB ? n + ((n - 1) >> PB) + 1 : n;
 }
+// lix(i0 + r*STRIDE) for a compile-time r, given li0 = lix(i0).  When STRIDE
+// is a multiple of 2^PB, adding r*STRIDE leaves the low PB bits of i0 alone,
+// so the padded index is li0 plus a constant (an LDS instruction's immediate
+// offset instead of a shift-and-add per element).  With STRIDE == 1 the same
+// holds when i0 is a multiple of R and R divides 2^PB (r < R).
+template <int PB, int STRIDE, int R = 0> MC_DEV int lix_at(int li0, int i0, int r) {
+  if constexpr (PB == 0)
+    return i0 + r * STRIDE;
+  else if constexpr (STRIDE % (1 << PB) == 0)
+    return li0 + r * (STRIDE + (STRIDE >> PB));
+  else if constexpr (STRIDE == 1 && R > 0 && (1 << PB) % R == 0)
+    return li0 + r;
+  else
+    return lix<PB>(i0 + r * STRIDE);
+}
 
 // Full in-LDS FFT of NB transforms of length N (buf[b*TS + lix(i)]), NT
 // threads.  Data and result in natural order.  Caller synchronises before.
@@ -258,8 +273,10 @@ MC_DEV void fft_lds_stage(float2* buf, const TW& tw, int ntw) {
     const int q = tid + p * NT;
     if (NBF % NT == 0 || q < NBF) {
       const int b = q / (N / R), j = q % (N / R);
+      float2* src = buf + b * TS;
+      const int lj = lix<PB>(j);
 #pragma unroll
-      for (int r = 0; r < R; r++) v[p][r] = buf[b * TS + lix<PB>(j + r * (N / R))];
+      for (int r = 0; r < R; r++) v[p][r] = src[lix_at<PB, N / R>(lj, j, r)];
       stockham_bfly<N, R, Ns, S>(v[p], j, tw, ntw);
     }
   }
@@ -270,8 +287,10 @@ MC_DEV void fft_lds_stage(float2* buf, const TW& tw, int ntw) {
     if (NBF % NT == 0 || q < NBF) {
       const int b = q / (N / R), j = q % (N / R);
       const int o = stockham_out<R, Ns>(j);
+      float2* dst = buf + b * TS;
+      const int lo = lix<PB>(o);
 #pragma unroll
-      for (int r = 0; r < R; r++) buf[b * TS + lix<PB>(o + r * Ns)] = v[p][r];
+      for (int r = 0; r < R; r++) dst[lix_at<PB, Ns, R>(lo, o, r)] = v[p][r];
     }
   }
   __syncthreads();
@@ -334,8 +353,10 @@ MC_DEV void fft_lds_stage_rw(float2* buf, const WT& tw) {
     const int q = tid + p * NT;
     if (NBF % NT == 0 || q < NBF) {
       const int b = q / (N / R), j = q % (N / R);
+      float2* src = buf + b * TS;
+      const int lj = lix<PB>(j);
 #pragma unroll
-      for (int r = 0; r < R; r++) v[p][r] = buf[b * TS + lix<PB>(j + r * (N / R))];
+      for (int r = 0; r < R; r++) v[p][r] = src[lix_at<PB, N / R>(lj, j, r)];
       if constexpr (Ns > 1) {
 #pragma unroll
         for (int r = 1; r < R; r++) v[p][r] = cmul(v[p][r], tw.w[p][r]);
@@ -350,8 +371,10 @@ MC_DEV void fft_lds_stage_rw(float2* buf, const WT& tw) {
     if (NBF % NT == 0 || q < NBF) {
       const int b = q / (N / R), j = q % (N / R);
       const int o = stockham_out<R, Ns>(j);
+      float2* dst = buf + b * TS;
+      const int lo = lix<PB>(o);
 #pragma unroll
-      for (int r = 0; r < R; r++) buf[b * TS + lix<PB>(o + r * Ns)] = v[p][r];
+      for (int r = 0; r < R; r++) dst[lix_at<PB, Ns, R>(lo, o, r)] = v[p][r];
     }
   }
   __syncthreads();

@@ -155,6 +155,9 @@ __global__ __launch_bounds__(256, (N <= 4096 ? MINW : 1)) void k_pair_cols_m128(
   constexpr int P1 = (NB1 + NT - 1) / NT;
   constexpr int AS = A / SP;
   static_assert(NT % 16 == 0 && A % SP == 0, "shape");
+  // Under the 128-VGPR cap of MINW = 4 the stage-0 weights are re-read per
+  // item (L1-resident, coalesced) instead of held in 16 registers.
+  constexpr bool WYG = MINW >= 4;
   __shared__ float2 buf[A * TS];
   const int tid = threadIdx.x;
   // stage-0 element weights (fixed per thread)
@@ -164,12 +167,18 @@ __global__ __launch_bounds__(256, (N <= 4096 ? MINW : 1)) void k_pair_cols_m128(
     const int q = tid + p * NT;
     const int a = q % A, j = q / A;
 #pragma unroll
-    for (int r = 0; r < 16; r++) wy[p][r] = q < NB0 ? wxy[H + a + A * (j + 8 * r)] : 0.f;
+    for (int r = 0; r < 16; r++) wy[p][r] = (!WYG && q < NB0) ? wxy[H + a + A * (j + 8 * r)] : 0.f;
   }
-  // stage-1 twiddles W_128^{k r}, k = tid % 16 (inverse sign)
+  // stage-1 twiddles W_128^{k r}, k = tid % 16 (inverse sign): in registers,
+  // or (WYG) read from a 128-entry LDS table, which frees 16 more VGPRs
+  __shared__ float2 tw128[WYG ? M : 1];
   float2 t1[8];
+  if constexpr (WYG) {
+    for (int i = tid; i < M; i += NT) tw128[i] = tw[i * (N / M)];
+  } else {
 #pragma unroll
-  for (int r = 0; r < 8; r++) t1[r] = tw[((tid % 16) * r * (N / M)) % N];
+    for (int r = 0; r < 8; r++) t1[r] = tw[((tid % 16) * r * (N / M)) % N];
+  }
   // combine: lane group (SP lanes) per output row
   const int ori = (N - box) / 2;
   const int cr = tid / SP, ch = tid % SP;
@@ -219,10 +228,13 @@ __global__ __launch_bounds__(256, (N <= 4096 ? MINW : 1)) void k_pair_cols_m128(
         const int a = q % A, j = q / A;
         float2 v[16];
 #pragma unroll
-        for (int r = 0; r < 16; r++) v[r] = cscale(cmulc(ra[pp][r], rb[pp][r]), wx * wy[pp][r]);
+        for (int r = 0; r < 16; r++) {
+          const float w = WYG ? wxy[H + a + A * (j + 8 * r)] : wy[pp][r];
+          v[r] = cscale(cmulc(ra[pp][r], rb[pp][r]), wx * w);
+        }
         dft16<1>(v);
 #pragma unroll
-        for (int r = 0; r < 16; r++) buf[a * TS + lix<PB>(16 * j + r)] = v[r];
+        for (int r = 0; r < 16; r++) buf[a * TS + lix_at<PB, 1, 16>(lix<PB>(16 * j), 16 * j, r)] = v[r];
       }
     }
     if (PF && it + nper < i1) fetch(it + nper);
@@ -235,12 +247,13 @@ __global__ __launch_bounds__(256, (N <= 4096 ? MINW : 1)) void k_pair_cols_m128(
         const int a = q / 16, j = q % 16;
         float2 v[8];
 #pragma unroll
-        for (int r = 0; r < 8; r++) v[r] = buf[a * TS + lix<PB>(j + 16 * r)];
+        for (int r = 0; r < 8; r++) v[r] = buf[a * TS + lix_at<PB, 16>(lix<PB>(j), j, r)];
 #pragma unroll
-        for (int r = 1; r < 8; r++) v[r] = cmul(v[r], t1[r]);
+        for (int r = 1; r < 8; r++)
+          v[r] = cmul(v[r], WYG ? tw128[((tid % 16) * r) % M] : t1[r]);
         dft8<1>(v);
 #pragma unroll
-        for (int r = 0; r < 8; r++) buf[a * TS + lix<PB>(j + 16 * r)] = v[r];
+        for (int r = 0; r < 8; r++) buf[a * TS + lix_at<PB, 16>(lix<PB>(j), j, r)] = v[r];
       }
     }
     __syncthreads();
