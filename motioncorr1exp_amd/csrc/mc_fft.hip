@@ -108,41 +108,81 @@ __global__ __launch_bounds__(NT) void k_r2c_rows(R2CGeom g, int f0, int nkx, flo
     T a, b;
   };
   const bool corr = g.dark || g.gain || g.xs < 1.f || g.ys < 1.f;
+  // All loads of the workgroup's rows are issued before the first LDS write
+  // (a load-then-store loop waits out the memory latency once per element).
+  constexpr int PER = (N + NT - 1) / NT;
+  float2 zv[ROWS][PER];
+  // the path is uniform; inside it every load is predicated, not branched
+  if (corr) {
 #pragma unroll
-  for (int r = 0; r < ROWS; r++) {
-    const int y = y0 + r;
-    const T* row = reinterpret_cast<const T*>(fbase + (size_t)(y + g.offy) * g.row_pitch) + g.offx;
-    const bool yok = y < g.h;
-    if (corr) {
-      for (int m = threadIdx.x; m < N; m += NT) {
-        const int x = 2 * m;
+    for (int r = 0; r < ROWS; r++) {
+      const int y = y0 + r;
+#pragma unroll
+      for (int k = 0; k < PER; k++) {
+        const int m = threadIdx.x + k * NT, x = 2 * m;
+        const bool ok = (N % NT == 0 || m < N) && y < g.h;
         float2 z = make_float2(0.f, 0.f);
-        if (yok && x < g.w) z.x = corrected_px<T>(g, fbase, g.offx + x, g.offy + y);
-        if (yok && x + 1 < g.w) z.y = corrected_px<T>(g, fbase, g.offx + x + 1, g.offy + y);
-        buf[r * TS + lix<PB>(m)] = z;
+        if (ok && x < g.w) z.x = corrected_px<T>(g, fbase, g.offx + x, g.offy + y);
+        if (ok && x + 1 < g.w) z.y = corrected_px<T>(g, fbase, g.offx + x + 1, g.offy + y);
+        zv[r][k] = z;
       }
-    } else if (g.vec && yok) {
-#pragma unroll 4
-      for (int m = threadIdx.x; m < N; m += NT) {
-        const int x = 2 * m;
-        float2 z = make_float2(0.f, 0.f);
-        if (x + 1 < g.w) {
-          const Pair v = *reinterpret_cast<const Pair*>(row + x);
-          z = make_float2((float)v.a, (float)v.b);
-        } else if (x < g.w) {
-          z.x = (float)row[x];
+    }
+  } else if (g.vec && g.w >= 2 && g.h >= 1) {
+    // Unconditional pair loads at clamped (always valid) addresses, zeroed
+    // outside the region afterwards: no load sits in a branch, so all are in
+    // flight together and the element conversions follow the last wait.
+    const int xmax = (g.w - 2) & ~1;
+#pragma unroll
+    for (int r = 0; r < ROWS; r++) {
+      const int y = y0 + r;
+      const int yc = y < g.h ? y : g.h - 1;
+      const T* row = reinterpret_cast<const T*>(fbase + (size_t)(yc + g.offy) * g.row_pitch) + g.offx;
+#pragma unroll
+      for (int k = 0; k < PER; k++) {
+        const int m = threadIdx.x + k * NT, x = 2 * m;
+        const bool pv = (N % NT == 0 || m < N) && y < g.h && x + 1 < g.w;
+        const Pair v = *reinterpret_cast<const Pair*>(row + (x < xmax ? x : xmax));
+        if constexpr (DT == DT_FLOAT32) {
+          zv[r][k] = pv ? make_float2((float)v.a, (float)v.b) : make_float2(0.f, 0.f);
+        } else {
+          // integer counts are finite: a 0/1 factor instead of a select keeps
+          // the compiler from moving each load into a branch of its own
+          const float mk = pv ? 1.f : 0.f;
+          zv[r][k] = make_float2((float)v.a * mk, (float)v.b * mk);
         }
-        buf[r * TS + lix<PB>(m)] = z;
       }
-    } else {
-#pragma unroll 4
-      for (int m = threadIdx.x; m < N; m += NT) {
-        const int x = 2 * m;
-        float2 z;
-        z.x = (yok && x < g.w) ? (float)row[x] : 0.f;
-        z.y = (yok && x + 1 < g.w) ? (float)row[x + 1] : 0.f;
-        buf[r * TS + lix<PB>(m)] = z;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < ROWS; r++) {
+      const int y = y0 + r;
+      const T* row = reinterpret_cast<const T*>(fbase + (size_t)(y + g.offy) * g.row_pitch) + g.offx;
+#pragma unroll
+      for (int k = 0; k < PER; k++) {
+        const int m = threadIdx.x + k * NT, x = 2 * m;
+        const bool ok = (N % NT == 0 || m < N) && y < g.h;
+        T a = 0, b = 0;
+        if (ok && x < g.w) a = row[x];
+        if (ok && x + 1 < g.w) b = row[x + 1];
+        zv[r][k] = make_float2((float)a, (float)b);
       }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < ROWS; r++)
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int m = threadIdx.x + k * NT;
+      if (N % NT == 0 || m < N) buf[r * TS + lix<PB>(m)] = zv[r][k];
+    }
+  if (!corr && g.vec && g.w >= 2 && (g.w & 1) && (g.w - 1) / 2 % NT == (int)threadIdx.x) {
+    // odd width: the last column has no pair partner (same thread as above)
+    const int x = g.w - 1;
+    for (int r = 0; r < ROWS; r++) {
+      const int y = y0 + r;
+      if (y >= g.h) break;
+      const T* row = reinterpret_cast<const T*>(fbase + (size_t)(y + g.offy) * g.row_pitch) + g.offx;
+      buf[r * TS + lix<PB>(x / 2)] = make_float2((float)row[x], 0.f);
     }
   }
   __syncthreads();
@@ -211,6 +251,33 @@ __global__ __launch_bounds__(256) void k_fill(const float* rowsum, int nu, int h
 }
 
 // ---------------------------------------------------------- column pass
+// Loads column kx of the row-pass scratch (rows [0, h_data); the canvas rows
+// beyond are the fill) into LDS, adding fill x the spectrum P of the fill
+// mask in pad mode.  All NU/NT loads are issued before the LDS writes.
+template <int NU, int NT, int PB>
+MC_DEV void col_load(const float2* src, int h_data, bool fill, float fv, float2 P, int kx,
+                     float2* buf) {
+  constexpr int PER = (NU + NT - 1) / NT;
+  float2 vv[PER];
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int i = threadIdx.x + k * NT;
+    vv[k] = (NU % NT == 0 || i < NU) && i < h_data ? src[(size_t)i * 4] : make_float2(0.f, 0.f);
+  }
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int i = threadIdx.x + k * NT;
+    if (NU % NT != 0 && i >= NU) continue;
+    float2 v = vv[k];
+    if (i < h_data) {
+      if (fill) v = cadd(v, cscale(P, fv));
+    } else if (fill && kx == 0) {
+      v.x = fv * (float)NU;
+    }
+    buf[lix<PB>(i)] = v;
+  }
+}
+
 template <int NU, int NT>
 __global__ __launch_bounds__(NT) void k_c2c_cols(const float2* rt, size_t rt_fstride, int nkx,
                                                  int h_data, const float* fill,
@@ -231,16 +298,7 @@ __global__ __launch_bounds__(NT) void k_c2c_cols(const float2* rt, size_t rt_fst
     fv = fill[f];
     P = padP[kx];
   }
-  for (int i = threadIdx.x; i < NU; i += NT) {
-    float2 v = make_float2(0.f, 0.f);
-    if (i < h_data) {
-      v = src[(size_t)i * 4];
-      if (fill) v = cadd(v, cscale(P, fv));
-    } else if (fill && kx == 0) {
-      v.x = fv * (float)NU;
-    }
-    buf[lix<PB>(i)] = v;
-  }
+  col_load<NU, NT, PB>(src, h_data, fill != nullptr, fv, P, kx, buf);
   __syncthreads();
   fft_lds_t<NU, NT, -1, 1, NU, TwSplit, PB>(buf, TwSplit{tlo, thi, LB, NU}, NU);
   float2* dst = spec + (size_t)f * spec_stride + (size_t)kx * n;
@@ -274,16 +332,7 @@ __global__ __launch_bounds__(NT) void k_c2c_cols_p(const float2* rt, size_t rt_f
       fv = fill[f];
       P = padP[kx];
     }
-    for (int i = threadIdx.x; i < NU; i += NT) {
-      float2 v = make_float2(0.f, 0.f);
-      if (i < h_data) {
-        v = src[(size_t)i * 4];
-        if (fill) v = cadd(v, cscale(P, fv));
-      } else if (fill && kx == 0) {
-        v.x = fv * (float)NU;
-      }
-      buf[lix<PB>(i)] = v;
-    }
+    col_load<NU, NT, PB>(src, h_data, fill != nullptr, fv, P, kx, buf);
     __syncthreads();
     fft_lds_rw<NU, NT, -1, 1, NU, PB>(buf, ftw);
     float2* dst = spec + (size_t)f * spec_stride + (size_t)kx * n;
