@@ -685,7 +685,7 @@ extern "C" int mc_sharpness(mc_context ctx, const double* frame_shift, float bfa
 
 static int to_image(mc_context_s* c, const float2* spec, float* dev_img) {
   const int n = c->n;
-  HIP_TRY(c->tmp.ensure(sizeof(float2) * (size_t)(n / 2 + 1) * n));
+  HIP_TRY(c->tmp.ensure(sizeof(float2) * mc::rt_frame_elems(n, n)));
   KScope ks(c, MC_KT_IMAGE);
   // DFAlign.cu:1289-1290: C2R then x 1/size (float 1.0/size)
   HIP_TRY(mc::launch_spectrum_to_image(spec, n, (float)(1.0 / ((double)n * n)), dev_img, n,
@@ -1126,8 +1126,10 @@ static int final_sum(mc_context_s* c, MovieRun& r) {
   const bool want2 = out->sum2_image && p->nEndSum2 > 0 && p->nEndSum2 < p->nEndSum;
   float2* s2 = nullptr;
   if (want2) {
-    HIP_TRY(c->tmp.ensure(sizeof(float2) * c->spec_stride * 2));
-    s2 = c->tmp.as<float2>() + c->spec_stride;  // second half of tmp
+    // after the image scratch of to_image, which runs before s2 is copied out
+    const size_t off = std::max(c->spec_stride, mc::rt_frame_elems(n, n));
+    HIP_TRY(c->tmp.ensure(sizeof(float2) * (off + c->spec_stride)));
+    s2 = c->tmp.as<float2>() + off;
   }
   MC_TRY(shift_sum(c, r.base, r.F, r.fs, slot(c, r.sum), s2, nes2));
   HIP_TRY(c->image.ensure(sizeof(float) * (size_t)n * n));

@@ -451,19 +451,33 @@ hipError_t launch_frames_to_spectra(const R2CGeom& g, int f0, int nframes, float
 }
 
 // =================================================== inverse (sum images)
-// Column pass: inverse transform of S[kx][:] along ky, stored row-major
-// tmp[ky][kx] (pitch n/2+1).
+// Column pass: inverse transform of S[kx][:] along ky, stored in groups of
+// four bins, tmp[kx/4][ky][kx%4] (the row-pass scratch layout of the forward
+// transform): the four columns of a group run on one XCD (remapped block
+// order), so each 128-byte line of tmp is completed in that XCD's L2 rather
+// than written to HBM piecewise.
 template <int N, int NT>
 __global__ __launch_bounds__(NT) void k_cols_inv(const float2* spec, float2* tmp,
                                                  const float2* __restrict__ tw) {
   __shared__ float2 buf[N];
-  constexpr int H = N / 2 + 1;
-  const int kx = blockIdx.x;
+  constexpr int PER = (N + NT - 1) / NT;
+  const int kx = xcd_remap(blockIdx.x, gridDim.x);
   const float2* src = spec + (size_t)kx * N;
-  for (int i = threadIdx.x; i < N; i += NT) buf[i] = src[i];
+  float2 vv[PER];
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int i = threadIdx.x + k * NT;
+    if (N % NT == 0 || i < N) vv[k] = src[i];
+  }
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int i = threadIdx.x + k * NT;
+    if (N % NT == 0 || i < N) buf[i] = vv[k];
+  }
   __syncthreads();
   fft_lds<N, NT, 1, 1>(buf, tw, N);
-  for (int i = threadIdx.x; i < N; i += NT) tmp[(size_t)i * H + kx] = buf[i];
+  float2* dst = tmp + (size_t)(kx >> 2) * N * 4 + (kx & 3);
+  for (int i = threadIdx.x; i < N; i += NT) dst[(size_t)i * 4] = buf[i];
 }
 
 // Row pass: C2R of ROWS rows of tmp with the DC/Nyquist projection
@@ -476,9 +490,12 @@ __global__ __launch_bounds__(NT) void k_c2r_rows(const float2* tmp, float scale,
   constexpr int TS = NH + 1;       // LDS row stride
   __shared__ float2 buf[ROWS * TS];
   const int y0 = blockIdx.x * ROWS;
-  for (int i = threadIdx.x; i < ROWS * H; i += NT) {
-    const int r = i / H, k = i % H;
-    float2 v = tmp[(size_t)(y0 + r) * H + k];
+  // tmp[kx/4][y][kx%4]: 16 lanes read 4 bins x 4 rows = one 128-byte line
+  for (int i = threadIdx.x; i < ROWS * 4 * ((H + 3) / 4); i += NT) {
+    const int kk = i & 3, r = (i >> 2) % ROWS, kg = (i >> 2) / ROWS;
+    const int k = 4 * kg + kk;
+    if (k >= H) continue;
+    float2 v = tmp[((size_t)kg * N + y0 + r) * 4 + kk];
     if (k == 0 || k == NH) v.y = 0.f;
     buf[r * TS + k] = v;
   }

@@ -52,7 +52,7 @@ hipError_t launch_frames_to_spectra(const R2CGeom& g, int f0, int nframes, float
 
 // Inverse 2-D transform of one spectrum ([kx][ky] layout, size n) to an
 // n x n real image scaled by `scale` (C2R with the DC/Nyquist projection).
-// tmp: scratch of (n/2+1)*n float2.
+// tmp: scratch of rt_frame_elems(n, n) float2.
 hipError_t launch_spectrum_to_image(const float2* spec, int n, float scale, float* img,
                                     size_t img_pitch, float2* tmp, hipStream_t st);
 
