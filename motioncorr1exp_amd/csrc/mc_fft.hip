@@ -191,46 +191,58 @@ __global__ __launch_bounds__(NT) void k_r2c_rows(R2CGeom g, int f0, int nkx, flo
   // unpack in place, one thread per bin pair (k, N-k), k in [0, N/2]:
   // X[k] = E[k] + e^{-2 pi i k/NU} O[k], E = (Z[k] + conj Z[N-k]) / 2,
   // O = (Z[k] - conj Z[N-k]) / 2i; X[N] from k = 0.
+  // A thread takes a bin pair for all ROWS rows, so its two twiddles are
+  // looked up once.
   constexpr int NP = N / 2 + 1;
-  for (int i = threadIdx.x; i < ROWS * NP; i += NT) {
-    const int r = i / NP, k = i % NP;
-    float2* Z = buf + r * TS;
-    const float2 zk = Z[lix<PB>(k)], zm = Z[lix<PB>(k ? N - k : 0)];
-    // bin k
-    {
-      const float2 zc = cconj(zm);
-      const float2 E = cscale(cadd(zk, zc), 0.5f);
-      const float2 D = csub(zk, zc);
-      const float2 O = make_float2(0.5f * D.y, -0.5f * D.x);
-      const float2 X = cadd(E, cmul(twv<-1>(tw, k), O));
-      if (k == 0) {
-        // X[N] = E[0] - O[0]
-        Z[lix<PB>(N)] = csub(E, O);
+  for (int k = threadIdx.x; k < NP; k += NT) {
+    const float2 wk = twv<-1>(tw, k), wm = twv<-1>(tw, N - k);
+    const int lk = lix<PB>(k), lm = lix<PB>(k ? N - k : 0);
+#pragma unroll
+    for (int r = 0; r < ROWS; r++) {
+      float2* Z = buf + r * TS;
+      const float2 zk = Z[lk], zm = Z[lm];
+      // bin k
+      {
+        const float2 zc = cconj(zm);
+        const float2 E = cscale(cadd(zk, zc), 0.5f);
+        const float2 D = csub(zk, zc);
+        const float2 O = make_float2(0.5f * D.y, -0.5f * D.x);
+        const float2 X = cadd(E, cmul(wk, O));
+        if (k == 0) {
+          // X[N] = E[0] - O[0]
+          Z[lix<PB>(N)] = csub(E, O);
+        }
+        Z[lk] = X;
       }
-      Z[lix<PB>(k)] = X;
-    }
-    // bin N-k (k in (0, N/2))
-    if (k > 0 && k < N - k) {
-      const float2 zc = cconj(zk);
-      const float2 E = cscale(cadd(zm, zc), 0.5f);
-      const float2 D = csub(zm, zc);
-      const float2 O = make_float2(0.5f * D.y, -0.5f * D.x);
-      Z[lix<PB>(N - k)] = cadd(E, cmul(twv<-1>(tw, N - k), O));
+      // bin N-k (k in (0, N/2))
+      if (k > 0 && k < N - k) {
+        const float2 zc = cconj(zk);
+        const float2 E = cscale(cadd(zm, zc), 0.5f);
+        const float2 D = csub(zm, zc);
+        const float2 O = make_float2(0.5f * D.y, -0.5f * D.x);
+        Z[lm] = cadd(E, cmul(wm, O));
+      }
     }
   }
   __syncthreads();
   // transposed store, bins in groups of four: RT[kx/4][y][kx%4]; 16
-  // consecutive lanes write 4 bins x 4 rows = one 128-byte line
+  // consecutive lanes write 4 bins x 4 rows = one 128-byte line.  A thread
+  // keeps its bin-in-group and row; its group advances by NT/(4 ROWS).
+  static_assert(NT % (4 * ROWS) == 0, "store tiling");
   float2* out = rt + (size_t)f * rt_fstride;
   const int nkg = (nkx + 3) / 4;
-  for (int i = threadIdx.x; i < ROWS * 4 * nkg; i += NT) {
-    const int kk = i & 3, r = (i >> 2) % ROWS, kg = (i >> 2) / ROWS;
-    const int k = 4 * kg + kk;
+  {
+    const int kk = threadIdx.x & 3, r = (threadIdx.x >> 2) % ROWS;
     const int y = y0 + r;
-    if (y >= g.h || k >= nkx) continue;
-    const float2 X = buf[r * TS + lix<PB>(k)];
-    out[((size_t)kg * NU + y) * 4 + kk] = X;
-    if (k == 0 && rowsum) rowsum[(size_t)f * NU + y] = X.x;
+    if (y < g.h) {
+      for (int kg = (threadIdx.x >> 2) / ROWS; kg < nkg; kg += NT / (4 * ROWS)) {
+        const int k = 4 * kg + kk;
+        if (k >= nkx) break;
+        const float2 X = buf[r * TS + lix<PB>(k)];
+        out[((size_t)kg * NU + y) * 4 + kk] = X;
+        if (k == 0 && rowsum) rowsum[(size_t)f * NU + y] = X.x;
+      }
+    }
   }
 }
 
