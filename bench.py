@@ -30,6 +30,28 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
+FP32_PEAK_TFLOPS = 157.3
+
+
+def fft_flops(group, F, n, h, P, box):
+    """Algorithmic FP32 flops per movie of an FFT-bound kernel group.
+    spectra: per frame, h row R2Cs (an n/2-point complex FFT, 5 (n/2)
+    log2(n/2), plus the unpack, 10 per bin) and n/2+1 column FFTs
+    (5 n log2 n).  pair_cols: per (pair, kx) item, the weighted product (9
+    per element), n/128 inverse 128-point FFTs as radix 16 (168 flops per
+    butterfly) then radix 8 with twiddles (98), and the combine of box rows
+    from n/128 terms (8 each)."""
+    import math
+    if group == "spectra":
+        rows = h * (5 * (n // 2) * math.log2(n // 2) + 10 * (n // 2))
+        cols = (n // 2 + 1) * 5 * n * math.log2(n)
+        return F * (rows + cols)
+    if group == "pair_cols":
+        item = 9 * n + (n // 16) * 168 + (n // 8) * 98 + box * (n // 128) * 8
+        return P * (n // 2 + 1) * item
+    return None
+
+
 def compulsory_bytes(F, n, nx, ny, in_elem, P, box, itr):
     """Compulsory HBM bytes of one movie: every tensor read once per consuming
     stage, every output written once (stage 1: frames in, spectra out; pairs:
@@ -342,6 +364,14 @@ def main():
                 "traffic": committed_traffic(dom),
                 "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
                 "algorithmic_bytes_per_launch": alg / lpm}
+    # the FFT groups are bound by vector-ALU issue: their arithmetic against
+    # the FP32 peak as well
+    fl = fft_flops(dom, F, n, ny, P, 96)
+    if fl and per_launch_ms > 0:
+        tf = fl / lpm / (per_launch_ms * 1e-3) / 1e12
+        roofline["compute"] = {"flops_per_launch": fl / lpm, "achieved": tf,
+                               "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                               "frac": tf / FP32_PEAK_TFLOPS}
     comp = compulsory_bytes(F, n, nx, ny, 4, P, 96, 0)
     movies_s = args.steps / dt
     whole = {"compulsory_bytes_per_movie": comp,
