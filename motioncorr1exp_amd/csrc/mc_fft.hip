@@ -73,17 +73,17 @@ __host__ __device__ constexpr int tw_lbits(int ntw) { return ilog2c(ntw) / 2; }
 // are the reference's: (raw - dark) * gain, raw * gain or raw - dark; the
 // source index is int(X * xs + 0.5) with the product in float and the
 // rounding addition in double.
-template <class T>
-MC_DEV float corrected_px(const R2CGeom& g, const char* fbase, int X, int Y) {
+MC_DEV int2 stretch_src(const R2CGeom& g, int X, int Y) {
   if ((g.xs < 1.f || g.ys < 1.f) && X > 0 && Y > 0) {
     X = (int)((double)((float)X * g.xs) + 0.5);
     Y = (int)((double)((float)Y * g.ys) + 0.5);
   }
-  float v = (float)reinterpret_cast<const T*>(fbase + (size_t)Y * g.row_pitch)[X];
-  const size_t k = (size_t)Y * g.nx + X;
-  if (g.dark && g.gain) v = (v - g.dark[k]) * g.gain[k];
-  else if (g.gain) v = v * g.gain[k];
-  else if (g.dark) v = v - g.dark[k];
+  return make_int2(X, Y);
+}
+MC_DEV float correct(const R2CGeom& g, float v, float d, float gn) {
+  if (g.dark && g.gain) return (v - d) * gn;
+  if (g.gain) return v * gn;
+  if (g.dark) return v - d;
   return v;
 }
 
@@ -114,17 +114,40 @@ __global__ __launch_bounds__(NT) void k_r2c_rows(R2CGeom g, int f0, int nkx, flo
   float2 zv[ROWS][PER];
   // the path is uniform; inside it every load is predicated, not branched
   if (corr) {
+    // Row by row: every pixel's loads (frame, dark, gain at its stretch
+    // source) are issued unconditionally, at the region origin for pixels
+    // outside the region, whose loaded values are then replaced by zeros
+    // (which the corrections map to 0).
 #pragma unroll
     for (int r = 0; r < ROWS; r++) {
       const int y = y0 + r;
+      T raw[PER][2];
+      float dk[PER][2], gk[PER][2];
+      bool val[PER][2];
 #pragma unroll
       for (int k = 0; k < PER; k++) {
         const int m = threadIdx.x + k * NT, x = 2 * m;
         const bool ok = (N % NT == 0 || m < N) && y < g.h;
-        float2 z = make_float2(0.f, 0.f);
-        if (ok && x < g.w) z.x = corrected_px<T>(g, fbase, g.offx + x, g.offy + y);
-        if (ok && x + 1 < g.w) z.y = corrected_px<T>(g, fbase, g.offx + x + 1, g.offy + y);
-        zv[r][k] = z;
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+          val[k][e] = ok && x + e < g.w;
+          const int2 sp = val[k][e] ? stretch_src(g, g.offx + x + e, g.offy + y)
+                                    : make_int2(g.offx, g.offy);
+          raw[k][e] = reinterpret_cast<const T*>(fbase + (size_t)sp.y * g.row_pitch)[sp.x];
+          const size_t idx = (size_t)sp.y * g.nx + sp.x;
+          dk[k][e] = g.dark ? g.dark[idx] : 0.f;
+          gk[k][e] = g.gain ? g.gain[idx] : 1.f;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < PER; k++) {
+        float c[2];
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+          const T rv = val[k][e] ? raw[k][e] : (T)0;
+          c[e] = correct(g, (float)rv, val[k][e] ? dk[k][e] : 0.f, val[k][e] ? gk[k][e] : 0.f);
+        }
+        zv[r][k] = make_float2(c[0], c[1]);
       }
     }
   } else if (g.vec && g.w >= 2 && g.h >= 1) {
@@ -152,21 +175,31 @@ __global__ __launch_bounds__(NT) void k_r2c_rows(R2CGeom g, int f0, int nkx, flo
         }
       }
     }
-  } else {
+  } else if (g.w >= 1 && g.h >= 1) {
+    // element loads at clamped addresses, the loaded values then zeroed
+    // outside the region (a select on the load result, not a branch)
 #pragma unroll
     for (int r = 0; r < ROWS; r++) {
       const int y = y0 + r;
-      const T* row = reinterpret_cast<const T*>(fbase + (size_t)(y + g.offy) * g.row_pitch) + g.offx;
+      const int yc = y < g.h ? y : g.h - 1;
+      const T* row = reinterpret_cast<const T*>(fbase + (size_t)(yc + g.offy) * g.row_pitch) + g.offx;
 #pragma unroll
       for (int k = 0; k < PER; k++) {
         const int m = threadIdx.x + k * NT, x = 2 * m;
         const bool ok = (N % NT == 0 || m < N) && y < g.h;
-        T a = 0, b = 0;
-        if (ok && x < g.w) a = row[x];
-        if (ok && x + 1 < g.w) b = row[x + 1];
-        zv[r][k] = make_float2((float)a, (float)b);
+        const T a = row[x < g.w ? x : g.w - 1], b = row[x + 1 < g.w ? x + 1 : g.w - 1];
+        const bool va = ok && x < g.w, vb = ok && x + 1 < g.w;
+        if constexpr (DT == DT_FLOAT32)
+          zv[r][k] = make_float2(va ? (float)a : 0.f, vb ? (float)b : 0.f);
+        else  // finite counts: 0/1 factors (see the pair path above)
+          zv[r][k] = make_float2((float)a * (va ? 1.f : 0.f), (float)b * (vb ? 1.f : 0.f));
       }
     }
+  } else {
+#pragma unroll
+    for (int r = 0; r < ROWS; r++)
+#pragma unroll
+      for (int k = 0; k < PER; k++) zv[r][k] = make_float2(0.f, 0.f);
   }
 #pragma unroll
   for (int r = 0; r < ROWS; r++)
