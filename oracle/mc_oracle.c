@@ -1181,36 +1181,51 @@ double or_sharpness(const R *spectra, int nframe, int n, float bfactor,
   const int h = n / 2 + 1;
   const size_t sb = (size_t)n * (n + 2);
   double J = 0;
-  if (grad)
-    for (int f = 0; f < 2 * nframe; f++) grad[f] = 0;
   double *gl = (double *)calloc(2 * (size_t)nframe, sizeof(double));
-  for (int ky = 0; ky < n; ky++) {
-    int yy = ky < n / 2 ? ky : ky - n;
-    for (int kx = 0; kx < h; kx++) {
-      double w = exp(-0.5 * (double)bfactor * (double)(kx * kx + yy * yy) /
-                     ((double)n * (double)n)) *
-                 ((kx == 0 || kx == n / 2) ? 1.0 : 2.0);
-      double Sr = 0, Si = 0;
-      for (int f = 0; f < nframe; f++) {
-        const cpx *F = (const cpx *)(spectra + (size_t)f * sb + (size_t)ky * (n + 2));
-        double a = 2.0 * OR_PI * (kx * s[2 * f] + yy * s[2 * f + 1]) / n;
-        double c = cos(a), sn = sin(a);
-        Sr += F[kx].re * c - F[kx].im * sn;
-        Si += F[kx].re * sn + F[kx].im * c;
-      }
-      J += w * (Sr * Sr + Si * Si);
-      if (grad) {
+  /* rows in parallel; each thread sums its rows, the partials are added in
+   * thread order */
+#pragma omp parallel
+  {
+    double Jt = 0;
+    double *gt = (double *)calloc(2 * (size_t)nframe, sizeof(double));
+#pragma omp for schedule(static)
+    for (int ky = 0; ky < n; ky++) {
+      int yy = ky < n / 2 ? ky : ky - n;
+      for (int kx = 0; kx < h; kx++) {
+        double w = exp(-0.5 * (double)bfactor * (double)(kx * kx + yy * yy) /
+                       ((double)n * (double)n)) *
+                   ((kx == 0 || kx == n / 2) ? 1.0 : 2.0);
+        double Sr = 0, Si = 0;
         for (int f = 0; f < nframe; f++) {
           const cpx *F = (const cpx *)(spectra + (size_t)f * sb + (size_t)ky * (n + 2));
           double a = 2.0 * OR_PI * (kx * s[2 * f] + yy * s[2 * f + 1]) / n;
           double c = cos(a), sn = sin(a);
-          double Gr = F[kx].re * c - F[kx].im * sn, Gi = F[kx].re * sn + F[kx].im * c;
-          double im = Sr * Gi - Si * Gr; /* Im(conj(S) G) */
-          gl[2 * f] += -2.0 * w * (2.0 * OR_PI * kx / n) * im;
-          gl[2 * f + 1] += -2.0 * w * (2.0 * OR_PI * yy / n) * im;
+          Sr += F[kx].re * c - F[kx].im * sn;
+          Si += F[kx].re * sn + F[kx].im * c;
+        }
+        Jt += w * (Sr * Sr + Si * Si);
+        if (grad) {
+          for (int f = 0; f < nframe; f++) {
+            const cpx *F = (const cpx *)(spectra + (size_t)f * sb + (size_t)ky * (n + 2));
+            double a = 2.0 * OR_PI * (kx * s[2 * f] + yy * s[2 * f + 1]) / n;
+            double c = cos(a), sn = sin(a);
+            double Gr = F[kx].re * c - F[kx].im * sn, Gi = F[kx].re * sn + F[kx].im * c;
+            double im = Sr * Gi - Si * Gr; /* Im(conj(S) G) */
+            gt[2 * f] += -2.0 * w * (2.0 * OR_PI * kx / n) * im;
+            gt[2 * f + 1] += -2.0 * w * (2.0 * OR_PI * yy / n) * im;
+          }
         }
       }
     }
+#pragma omp for ordered schedule(static, 1)
+    for (int t = 0; t < omp_get_num_threads(); t++) {
+#pragma omp ordered
+      {
+        J += Jt;
+        for (int f = 0; f < 2 * nframe; f++) gl[f] += gt[f];
+      }
+    }
+    free(gt);
   }
   if (grad)
     for (int f = 0; f < 2 * nframe; f++) grad[f] = gl[f];

@@ -1,11 +1,12 @@
-"""GPU parity at a full benchmark size: 4096x4096x24 uint8 Poisson counts
+"""GPU parity at full benchmark sizes.  First, 4096x4096x24 uint8 Poisson counts
 (mean 1.0 over the random-field phantom, random-walk drift with N(0, 1.0 px)
 steps), 2x Fourier-crop binning, all 276 frame pairs, B-factor 150, peak box
 96 and 3 align-to-sum rounds.  The C ABI's results are compared with the FP64
 oracle run on the same frames (its pruned box inverse, which equals the
 reference's full inverse plus read-out): pair shifts and final shifts to the
 0.01 px bar with near-tie grid flips of 1/32 px counted, the aligned sum to
-1e-4 relative, and the recovered drift against the known drift."""
+1e-4 relative, and the recovered drift against the known drift.  Second,
+the differentiation form at 3838x3710x40 (below)."""
 import numpy as np
 import pytest
 
@@ -47,3 +48,49 @@ def test_4096x4096x24_uint8_bin2_itr3_matches_oracle():
     assert rms < 0.01
     assert rel < 1e-4
     assert vs_truth < 0.1
+
+
+def test_3838x3710x40_differentiation_form():
+    """The differentiation form at its full size: 3838x3710x40 float32 frames
+    (padded to 4096) with sub-pixel random-walk drift N(0, 0.4 px) plus a
+    quadratic slow drift.  The adjoint gradient of the sharpness on the
+    resident spectra is checked against central differences (h = 1e-3 px) and
+    against the FP64 oracle gradient on the same spectra; 20 refinement steps
+    raise the sharpness and do not move the shifts away from the known drift."""
+    F, n = 40, 4096
+    frames, drift = make_movie(3838, 3710, F, dose=0.8, step=0.4, seed=2402, quad_sigma=0.005)
+    print("movie ready", flush=True)
+    ctx = mc.Context(0)
+    p = mc.default_params(F, pad=n, FrameDistOffset=2, CCPeakSearchDim=96)
+    dev = mc.DeviceArray(frames)
+    r0 = ctx.align_movie(dev, frames.shape, frames.dtype, p, want=("frame_shift",))
+    r1 = ctx.align_movie(dev, frames.shape, frames.dtype, dict(p, grad_steps=20, grad_step_px=0.05),
+                         want=("frame_shift",))
+    rep = r1["report"]
+    truth = drift - drift[F // 2 + 1]
+    e0 = float(np.sqrt(np.mean((r0["frame_shift"] - truth) ** 2)))
+    e1 = float(np.sqrt(np.mean((r1["frame_shift"] - truth) ** 2)))
+    print("J0", rep["sharp_j0"], "J1", rep["sharp_j1"], "accepted", rep["grad_accepted"],
+          "vs truth RMS before", e0, "after", e1, flush=True)
+    assert rep["sharp_j1"] >= rep["sharp_j0"]
+    assert e1 <= e0 + 0.01
+    # gradient checks on the resident spectra, away from the optimum
+    ctx.load_frames(dev, frames.shape, frames.dtype, p)
+    s = r0["frame_shift"] + np.random.default_rng(7).normal(0, 0.4, size=(F, 2))
+    J, g = ctx.sharpness(s, 150.0)
+    h = 1e-3
+    for f, ax in [(0, 0), (1, 1), (5, 1), (10, 0), (20, 1), (25, 0), (30, 0), (39, 1)]:
+        sp, sm = s.copy(), s.copy()
+        sp[f, ax] += h
+        sm[f, ax] -= h
+        fd = (ctx.sharpness(sp, 150.0)[0] - ctx.sharpness(sm, 150.0)[0]) / (2 * h)
+        print("frame", f, "axis", ax, "adjoint", g[f, ax], "central difference", fd, flush=True)
+        assert abs(fd - g[f, ax]) <= 2e-3 * np.abs(g).max()
+    spec = ctx.get_spectra(F, n)
+    dev.free()
+    ctx.close()
+    oJ, og = Oracle("f64").sharpness(spec.astype(np.float64), n, 150.0, s)
+    print("J rel err", abs(J - oJ) / abs(oJ), "gradient max err / max",
+          np.abs(g - og).max() / np.abs(og).max(), flush=True)
+    assert abs(J - oJ) <= 2e-6 * abs(oJ)
+    assert np.abs(g - og).max() <= 1e-4 * np.abs(og).max()
