@@ -611,22 +611,26 @@ static int sharp_weights(mc_context_s* c, int n, float bfactor) {
 
 // J(s) and dJ/ds of frames in slots [base, base+F), sum formed in slot
 // `sum`; fs: 2F shifts (NaN: frame out).
+// J (and with grad, dJ/ds) at the shifts fs.  with_sum = false reuses the
+// sum already in the sum slot, formed with the same shifts (still in the
+// context's shift buffer).  Without grad only J is computed (same blocks and
+// summation order, so the same value).
 static int sharpness_eval(mc_context_s* c, int base, int F, int sum, const std::vector<double>& fs,
-                          double* J, std::vector<double>& grad) {
+                          double* J, std::vector<double>* grad, bool with_sum = true) {
   const int nchunks = 256;
-  MC_TRY(shift_sum(c, base, F, fs, slot(c, sum), nullptr, -1));
+  if (with_sum) MC_TRY(shift_sum(c, base, F, fs, slot(c, sum), nullptr, -1));
   HIP_TRY(c->gpart.ensure(sizeof(double2) * (size_t)nchunks * (F + 1)));
   HIP_TRY(c->gout.ensure(sizeof(double2) * (F + 1)));
   {
     KScope ks(c, MC_KT_SUM);
     HIP_TRY(mc::launch_sharpness(slot(c, base), c->spec_stride, F, c->n, c->shifts.as<double2>(),
                                  slot(c, sum), c->wk.as<float>(), c->gpart.as<double2>(), nchunks,
-                                 c->gout.as<double2>(), c->st));
+                                 c->gout.as<double2>(), grad ? 0 : 1, c->st));
   }
   std::vector<double> o(2 * (size_t)(F + 1));
   HIP_TRY(hipMemcpyAsync(o.data(), c->gout.p, sizeof(double) * o.size(), hipMemcpyDeviceToHost, c->st));
   HIP_TRY(hipStreamSynchronize(c->st));
-  grad.assign(o.begin(), o.begin() + 2 * F);
+  if (grad) grad->assign(o.begin(), o.begin() + 2 * F);
   *J = o[2 * F];
   return MC_OK;
 }
@@ -640,7 +644,7 @@ static int refine_shifts(mc_context_s* c, int base, int F, int sum, std::vector<
   MC_TRY(sharp_weights(c, c->n, bfactor));
   double J = 0;
   std::vector<double> g;
-  MC_TRY(sharpness_eval(c, base, F, sum, fs, &J, g));
+  MC_TRY(sharpness_eval(c, base, F, sum, fs, &J, &g));
   *j0 = J;
   double step = step_px > 0 ? step_px : 0.05;
   int acc = 0;
@@ -652,10 +656,14 @@ static int refine_shifts(mc_context_s* c, int base, int F, int sum, std::vector<
     std::vector<double> trial(fs);
     for (size_t k = 0; k < g.size(); k++)
       if (!std::isnan(fs[k])) trial[k] += step * g[k] / gmax;
+    // the trial's J alone; its gradient only once it is accepted (from the
+    // trial's sum, still resident)
     double Jt = 0;
-    std::vector<double> gt;
-    MC_TRY(sharpness_eval(c, base, F, sum, trial, &Jt, gt));
+    MC_TRY(sharpness_eval(c, base, F, sum, trial, &Jt, nullptr));
     if (Jt > J) {
+      std::vector<double> gt;
+      double Jg = 0;
+      MC_TRY(sharpness_eval(c, base, F, sum, trial, &Jg, &gt, false));
       fs.swap(trial);
       g.swap(gt);
       J = Jt;
@@ -677,7 +685,7 @@ extern "C" int mc_sharpness(mc_context ctx, const double* frame_shift, float bfa
   MC_TRY(sharp_weights(ctx, ctx->n, bfactor));
   std::vector<double> fs(frame_shift, frame_shift + 2 * (size_t)ctx->nframe), g;
   double j = 0;
-  MC_TRY(sharpness_eval(ctx, 0, ctx->nframe, ctx->sum_slot, fs, &j, g));
+  MC_TRY(sharpness_eval(ctx, 0, ctx->nframe, ctx->sum_slot, fs, &j, &g));
   if (J) *J = j;
   if (grad) memcpy(grad, g.data(), sizeof(double) * g.size());
   return MC_OK;

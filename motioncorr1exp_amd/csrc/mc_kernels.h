@@ -129,9 +129,12 @@ hipError_t launch_disp_short(const float* buf, int size, short* out, hipStream_t
 // the same shifts) and dJ/ds_f: out[f] = (dJ/dsx_f, dJ/dsy_f), out[F] = (J, 0).
 // wk: [n/2+1] column weights (incl. the half-plane factor) then [n] row
 // weights, w(kx, ky) = wk[kx] * wk[n/2+1+ky].  part: nchunks*(F+1) double2.
+// jonly = 1 computes only J (out[F]; out[0..F) are then not meaningful), with
+// the same blocks and summation order as the full pass.
 hipError_t launch_sharpness(const float2* spectra, size_t spec_stride, int nframes, int n,
                             const double2* shifts, const float2* S, const float* wk,
-                            double2* part, int nchunks, double2* out, hipStream_t st);
+                            double2* part, int nchunks, double2* out, int jonly,
+                            hipStream_t st);
 
 // Layout conversion between the resident [kx][ky] spectrum and the padded
 // row-major [ky][kx] layout (n rows of n/2+1 complex).

@@ -315,10 +315,11 @@ __global__ __launch_bounds__(256) void k_sharp_partial(const float2* __restrict_
                                                        const double2* __restrict__ shifts,
                                                        const float2* __restrict__ S,
                                                        const float* __restrict__ wk,
-                                                       int nchunks, double2* __restrict__ part) {
+                                                       int nchunks, int jonly,
+                                                       double2* __restrict__ part) {
   __shared__ double rx[256], ry[256];
   const int L = xcd_remap(blockIdx.x, gridDim.x);
-  const int c = L / (nframes + 1), f = L % (nframes + 1);
+  const int c = jonly ? L : L / (nframes + 1), f = jonly ? nframes : L % (nframes + 1);
   const int H = n / 2 + 1;
   const size_t total = (size_t)H * n;
   const size_t e0 = total * c / nchunks, e1 = total * (c + 1) / nchunks;
@@ -328,23 +329,49 @@ __global__ __launch_bounds__(256) void k_sharp_partial(const float2* __restrict_
   const float2* F = spectra + (size_t)(f < nframes ? f : 0) * spec_stride;
   const double inv_n = 1.0 / n;
   if (!skip) {
-    for (size_t e = e0 + threadIdx.x; e < e1; e += 256) {
-      const int kx = (int)(e / n), ky = (int)(e % n);
-      const int yy = ky < n / 2 ? ky : ky - n;
-      const float w = wk[kx] * wk[H + ky];
-      const float2 sv = S[e];
-      if (f == nframes) {
-        gx += (double)w * ((double)sv.x * sv.x + (double)sv.y * sv.y);
-      } else {
-        double u = (s.x * kx + s.y * yy) * inv_n;
-        u -= floor(u);
-        float sn, cs;
-        sincospif(2.0f * (float)u, &sn, &cs);
-        const float2 g = cmul(F[e], make_float2(cs, sn));
-        const double im = (double)sv.x * g.y - (double)sv.y * g.x;  // Im(conj(S) G)
-        const double t = -2.0 * w * im * (2.0 * M_PI * inv_n);
-        gx += t * kx;
-        gy += t * yy;
+    // U elements per round, all loads issued first (at valid addresses; the
+    // elements past the chunk get weight 0); (kx, ky) advance by the stride
+    // instead of being divided out of the element index.
+    constexpr int U = 4;
+    const int ie1 = (int)e1;
+    int e = (int)e0 + threadIdx.x;
+    int kx = e / n, ky = e - kx * n;
+    const int kadv = 256 / n, yadv = 256 % n;  // (kx, ky) step of e += 256
+    for (; e < ie1; e += 256 * U) {
+      float2 sv[U], fv[U];
+      float w[U];
+      int kxs[U], yys[U];
+      int ee = e;
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int ec = ee < ie1 ? ee : ie1 - 1;
+        sv[u] = S[ec];
+        fv[u] = F[ec];
+        w[u] = wk[kx] * wk[H + ky] * (ee < ie1 ? 1.f : 0.f);
+        kxs[u] = kx;
+        yys[u] = ky < n / 2 ? ky : ky - n;
+        ee += 256;
+        ky += yadv;
+        kx += kadv;
+        const bool wrap = ky >= n;
+        ky = wrap ? ky - n : ky;
+        kx = wrap ? kx + 1 : kx;
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (f == nframes) {
+          gx += (double)w[u] * ((double)sv[u].x * sv[u].x + (double)sv[u].y * sv[u].y);
+        } else {
+          double uu = (s.x * kxs[u] + s.y * yys[u]) * inv_n;
+          uu -= floor(uu);
+          float sn, cs;
+          sincospif(2.0f * (float)uu, &sn, &cs);
+          const float2 g = cmul(fv[u], make_float2(cs, sn));
+          const double im = (double)sv[u].x * g.y - (double)sv[u].y * g.x;  // Im(conj(S) G)
+          const double t = -2.0 * w[u] * im * (2.0 * M_PI * inv_n);
+          gx += t * kxs[u];
+          gy += t * yys[u];
+        }
       }
     }
   }
@@ -377,10 +404,11 @@ __global__ void k_sharp_final(const double2* __restrict__ part, int nchunks, int
 
 hipError_t launch_sharpness(const float2* spectra, size_t spec_stride, int nframes, int n,
                             const double2* shifts, const float2* S, const float* wk,
-                            double2* part, int nchunks, double2* out, hipStream_t st) {
-  const int nb = nchunks * (nframes + 1);
+                            double2* part, int nchunks, double2* out, int jonly,
+                            hipStream_t st) {
+  const int nb = nchunks * (jonly ? 1 : nframes + 1);
   hipLaunchKernelGGL(k_sharp_partial, dim3(nb), dim3(256), 0, st, spectra, spec_stride, nframes,
-                     n, shifts, S, wk, nchunks, part);
+                     n, shifts, S, wk, nchunks, jonly, part);
   hipLaunchKernelGGL(k_sharp_final, dim3((nframes + 1 + 63) / 64), dim3(64), 0, st, part,
                      nchunks, nframes, out);
   return hipGetLastError();
