@@ -284,19 +284,20 @@ int mc_align_batch(mc_context ctx, const mc_batch_item *items, int count,
  * "cols", "rows"}); mc_tune_load reads the lines of the same device type
  * back (later lines win).  Without tuning, the defaults are 4x512 (4x256
  * below 4096), 192 MB of row-pass scratch per launch, the M=128 column
- * kernel without prefetch and the FP32 MFMA row kernel. */
+ * kernel without prefetch and the streaming FP32 MFMA row kernel. */
 typedef struct {
   int r2c_variant;        /* 1: 4x256, 2: 2x256, 3: 4x512; +8: persistent
                              column pass with register twiddles           */
   int chunk_frames;       /* frames per stage-1 launch                      */
   int pair_cols_variant;  /* 0: M=128, 1: M=128 + prefetch, 2: M-generic,
                              3: M=128 at 4 waves per SIMD                 */
-  int pair_rows_variant;  /* 0: FP32 MFMA (4 waves), 1: vector ALU,
-                             2: FP32 MFMA (8 waves)                        */
+  int pair_rows_variant;  /* 0: FP32 MFMA streaming (box <= 96), 1: vector
+                             ALU, 2: FP32 MFMA through LDS (8 waves),
+                             3: FP32 MFMA through LDS (4 waves)             */
   float ms_stage1[2][4][4]; /* [column pass: per column, persistent]
                                [row shape][chunk 1, 2, 4, 8]; 0 = not run  */
   float ms_pair_cols[4];  /* 0 = not run                                    */
-  float ms_pair_rows[3];
+  float ms_pair_rows[4];
 } mc_tune_report;
 
 int mc_autotune(mc_context ctx, const mc_movie_desc *movie, const mc_params *p,

@@ -113,7 +113,7 @@ class TuneReport(ctypes.Structure):
     _fields_ = [("r2c_variant", ctypes.c_int), ("chunk_frames", ctypes.c_int),
                 ("pair_cols_variant", ctypes.c_int), ("pair_rows_variant", ctypes.c_int),
                 ("ms_stage1", ((ctypes.c_float * 4) * 4) * 2), ("ms_pair_cols", ctypes.c_float * 4),
-                ("ms_pair_rows", ctypes.c_float * 3)]
+                ("ms_pair_rows", ctypes.c_float * 4)]
 
 
 class MaskElem(ctypes.Structure):

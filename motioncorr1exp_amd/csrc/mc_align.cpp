@@ -98,7 +98,7 @@ struct KTimer {
 struct mc_context_s {
   int device = 0;
   hipStream_t st = nullptr;
-  DevBuf spec, rt, rowsum, fill, padP, pairs_d, tbuf, box, peaks, shifts, image, tmp, means,
+  DevBuf spec, rt, rowsum, fill, padP, pairs_d, tbuf, box, boxpart, peaks, shifts, image, tmp, means,
       sum2img, wk, gpart, gout, jobs, tmax, fsc, disp;
   int graph_captures = 0, graph_replays = 0;
   // autotuned launch shapes (mc_autotune / mc_tune_load)
@@ -172,9 +172,10 @@ extern "C" int mc_context_destroy(mc_context ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->st);
   DevBuf* bufs[] = {&ctx->spec, &ctx->rt, &ctx->rowsum, &ctx->fill, &ctx->padP,
-                    &ctx->pairs_d, &ctx->tbuf, &ctx->box, &ctx->peaks, &ctx->shifts,
-                    &ctx->image, &ctx->tmp, &ctx->means, &ctx->sum2img, &ctx->wk,
-                    &ctx->gpart, &ctx->gout, &ctx->jobs, &ctx->tmax};
+                    &ctx->pairs_d, &ctx->tbuf, &ctx->box, &ctx->boxpart, &ctx->peaks,
+                    &ctx->shifts, &ctx->image, &ctx->tmp, &ctx->means, &ctx->sum2img,
+                    &ctx->wk, &ctx->gpart, &ctx->gout, &ctx->jobs, &ctx->tmax, &ctx->fsc,
+                    &ctx->disp};
   for (DevBuf* b : bufs) b->release();
   for (hipEvent_t e : ctx->pool) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->st);
@@ -526,6 +527,7 @@ static int pair_reserve(mc_context_s* c, int np, int n, int box, int precision) 
   HIP_TRY(c->tbuf.ensure(sizeof(float2) * (size_t)np * H * box));
   if (precision == MC_PREC_MFMA_HILO) HIP_TRY(c->tmax.ensure(sizeof(float) * (size_t)np * H));
   HIP_TRY(c->box.ensure(sizeof(float) * (size_t)np * box * box));
+  HIP_TRY(c->boxpart.ensure(sizeof(float) * std::max<size_t>(1, mc::pair_rows_part_elems(np, n, box))));
   HIP_TRY(c->peaks.ensure(sizeof(double) * 3 * (size_t)np));
   return MC_OK;
 }
@@ -546,7 +548,8 @@ static int pair_launch(mc_context_s* c, int np, int n, float bfactor, int box, i
   {
     KScope ks(c, MC_KT_PAIR_ROWS);
     HIP_TRY(mc::launch_pair_rows(c->tbuf.as<float2>(), c->tmax.as<float>(), np, n, box,
-                                 c->box.as<float>(), precision, rows, c->st));
+                                 c->box.as<float>(), c->boxpart.as<float>(), precision, rows,
+                                 c->st));
   }
   {
     KScope ks(c, MC_KT_PEAKS);
@@ -1693,7 +1696,7 @@ extern "C" int mc_autotune(mc_context ctx, const mc_movie_desc* m, const mc_para
       MC_TRY(time_min(c, reps, &ms, [&] {
         const hipError_t e = mc::launch_pair_rows(
             c->tbuf.as<float2>(), c->tmax.as<float>(), r.P, n, box, c->box.as<float>(),
-            p->precision, v, c->st);
+            c->boxpart.as<float>(), p->precision, v, c->st);
         return e == hipSuccess ? MC_OK : set_err(MC_ERR_DEVICE, "pair rows: %s", hipGetErrorString(e));
       }));
       tr.ms_pair_rows[v] = ms;

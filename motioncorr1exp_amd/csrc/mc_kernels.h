@@ -68,17 +68,23 @@ hipError_t launch_spectrum_to_image(const float2* spec, int n, float scale, floa
 //   M128_PF = the same with next-item register prefetch (fewer resident
 //   workgroups), GENERIC = the M-generic persistent kernel, M128_W4 = M = 128
 //   compiled for 4 waves per SIMD (128 VGPRs); row pass: AUTO = FP32 MFMA
-//   with 4 waves per pair (box <= 96), VALU = register-tiled vector ALU
-//   kernel, MFMA8 = FP32 MFMA with 8 waves per pair.
+//   streaming each wave's kx range from memory (box <= 96; when pairs are
+//   few the kx range is also split over workgroups whose partial maps go to
+//   `part`, pair_rows_part_elems(npairs, n, box) floats, and are added in
+//   order), VALU = register-tiled vector ALU kernel, MFMA8 / MFMA4 = FP32
+//   MFMA staged through LDS with 8 / 4 waves per (pair, 32-row block).
 enum { PAIR_COLS_AUTO = 0, PAIR_COLS_M128_PF = 1, PAIR_COLS_GENERIC = 2, PAIR_COLS_M128_W4 = 3,
        PAIR_COLS_VARIANTS = 4 };
-enum { PAIR_ROWS_AUTO = 0, PAIR_ROWS_VALU = 1, PAIR_ROWS_MFMA8 = 2, PAIR_ROWS_VARIANTS = 3 };
+enum { PAIR_ROWS_AUTO = 0, PAIR_ROWS_VALU = 1, PAIR_ROWS_MFMA8 = 2, PAIR_ROWS_MFMA4 = 3,
+       PAIR_ROWS_VARIANTS = 4 };
 bool pair_form_supported(int precision, int n, int box);
 hipError_t launch_pair_cols(const float2* spectra, size_t spec_stride, const int2* pairs,
                             int npairs, int n, float bfactor, int box, float2* tbuf,
                             float* tmax, int precision, int variant, hipStream_t st);
+size_t pair_rows_part_elems(int npairs, int n, int box);
 hipError_t launch_pair_rows(const float2* tbuf, const float* tmax, int npairs, int n, int box,
-                            float* boxmaps, int precision, int variant, hipStream_t st);
+                            float* boxmaps, float* part, int precision, int variant,
+                            hipStream_t st);
 
 // Peaks: integer peak + FP64 16x16 -> 512x512 sub-pixel search per box map.
 // out: 3 doubles per pair (sx, sy, cc).

@@ -630,6 +630,110 @@ __global__ __launch_bounds__(NT) void k_pair_rows_mfma(const float2* __restrict_
     out[(size_t)(32 * rb + e / box) * box + e % box] = red[(e / box) * W + e % box];
 }
 
+// Streaming form of the same GEMM (box <= 96), with no LDS staging in the
+// main loop.  The K range of a pair (bins kx in [0, n/2]) is split over ks
+// workgroups and over the NW waves of each; a wave walks its contiguous kx
+// range, each lane loading just the float it feeds (A[l % 32][l / 32] =
+// (Re | Im) T[kx][r]; B[l / 32][l % 32] from a table that holds
+// a_kx (cos, -sin)(2 pi kx R_c/n), zero past the box), D bins ahead in
+// registers, and keeps all NB x NB accumulator blocks.  Rows of A past the
+// box repeat the last box row; their outputs are never stored.  The NW
+// partial maps are summed in LDS in wave order; with ks > 1 workgroup s
+// writes partial map s of its pair and k_sum_parts adds them in order.
+template <int NB, int NW, int D>
+__global__ __launch_bounds__(NW * 64, (8 / NW > 1 ? 8 / NW : 1)) void k_pair_rows_stream(const float* __restrict__ tf, int n,
+                                                              int box, int ks,
+                                                              float* __restrict__ out,
+                                                              const float* __restrict__ ef) {
+  constexpr int W = 32 * NB;
+  const int p = blockIdx.x, s = blockIdx.y;
+  const int H = n / 2 + 1;
+  const int wv = threadIdx.x / 64, l = threadIdx.x % 64, li = l % 32, lh = l / 32;
+  const int nsub = ks * NW, sub = s * NW + wv;
+  const int k0 = (int)((long long)H * sub / nsub), k1 = (int)((long long)H * (sub + 1) / nsub);
+  const float* T = tf + (size_t)p * H * box * 2;
+  int roff[NB];
+#pragma unroll
+  for (int i = 0; i < NB; i++) roff[i] = 2 * min(32 * i + li, box - 1) + lh;
+  const int eoff = 2 * li + lh;
+  v16f acc[NB][NB];
+#pragma unroll
+  for (int i = 0; i < NB; i++)
+#pragma unroll
+    for (int j = 0; j < NB; j++)
+#pragma unroll
+      for (int v = 0; v < 16; v++) acc[i][j][v] = 0.f;
+  float a[D][NB], b[D][NB];
+  // operands of bin min(kx, n/2) into slot d (bins past the wave's range
+  // are loaded but never multiplied)
+  auto load = [&](int d, int kx) {
+    const int kc = min(kx, H - 1);
+    const float* tr = T + (size_t)kc * box * 2;
+    const float* er = ef + (size_t)kc * W * 2;
+#pragma unroll
+    for (int i = 0; i < NB; i++) a[d][i] = tr[roff[i]];
+#pragma unroll
+    for (int j = 0; j < NB; j++) b[d][j] = er[eoff + 64 * j];
+  };
+  auto mul = [&](int d) {
+#pragma unroll
+    for (int i = 0; i < NB; i++)
+#pragma unroll
+      for (int j = 0; j < NB; j++)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[d][i], b[d][j], acc[i][j], 0, 0, 0);
+  };
+#pragma unroll
+  for (int d = 0; d < D; d++) load(d, k0 + d);
+  const int nfull = (k1 - k0) / D, rem = (k1 - k0) % D;
+  for (int it = 0; it < nfull; it++) {
+    const int kn = k0 + (it + 1) * D;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      mul(d);
+      load(d, kn + d);
+      // keeps each step's loads in that step, D bins ahead of their use: the
+      // memory clobber stops them sinking into the next iteration, the
+      // scheduling barrier stops the MFMAs of later steps moving above them
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < D; d++)
+    if (d < rem) mul(d);
+  // ordered sum of the NW wave maps
+  __shared__ float red[W * W];
+  for (int w = 0; w < NW; w++) {
+    if (wv == w) {
+#pragma unroll
+      for (int i = 0; i < NB; i++)
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+#pragma unroll
+          for (int v = 0; v < 16; v++) {
+            const int r = 32 * i + 8 * (v / 4) + 4 * lh + (v % 4), c = 32 * j + li;
+            if (w == 0) red[r * W + c] = acc[i][j][v];
+            else red[r * W + c] += acc[i][j][v];
+          }
+    }
+    __syncthreads();
+  }
+  float* o = out + ((size_t)p * ks + s) * box * box;
+  for (int e = threadIdx.x; e < box * box; e += NW * 64) o[e] = red[(e / box) * W + e % box];
+}
+
+// out[p][e] = sum_{s < ks} part[p][s][e] (e < m), added in order of s.
+__global__ __launch_bounds__(256) void k_sum_parts(const float* __restrict__ part, int ks, int m,
+                                                   long long total, float* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= total) return;
+  const long long p = i / m, e = i % m;
+  const float* q = part + p * ks * m + e;
+  float v = q[0];
+  for (int s = 1; s < ks; s++) v += q[(size_t)s * m];
+  out[i] = v;
+}
+
 // Row pass of the MFMA FP16 hi/lo form: the GEMM of k_pair_rows_mfma with
 // v_mfma_f32_32x32x8f16 (four kx per instruction: K index 2 kk + (Re|Im)).
 // T of a pair is scaled by the power of two that puts max|T| (the column
@@ -825,6 +929,82 @@ static const float2* row_twiddles(int n, int box) {
   return d;
 }
 
+// B operand table of k_pair_rows_stream: per bin kx, 32 ceil(box/32)
+// columns of (a_kx cos t, -a_kx sin t), t = 2 pi (kx R_c mod n)/n, with
+// a_0 = a_{n/2} = 1, a_kx = 2 otherwise, and zeros past the box.
+static const float* row_twiddles_stream(int n, int box) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int>, float*> cache;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(mu);
+  auto key = std::make_tuple(dev, n, box);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  const int H = n / 2 + 1, ori = (n - box) / 2, W = 32 * ((box + 31) / 32);
+  std::vector<float> h((size_t)H * W * 2, 0.f);
+  for (int kx = 0; kx < H; kx++) {
+    const float al = (kx == 0 || 2 * kx == n) ? 1.f : 2.f;
+    for (int c = 0; c < box; c++) {
+      const long long e = ((long long)kx * (ori + c)) % n;
+      const double a = 2.0 * M_PI * (double)e / (double)n;
+      h[((size_t)kx * W + c) * 2] = al * (float)cos(a);
+      h[((size_t)kx * W + c) * 2 + 1] = -al * (float)sin(a);
+    }
+  }
+  float* d = nullptr;
+  if (hipMalloc(&d, sizeof(float) * h.size()) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    return nullptr;
+  }
+  cache[key] = d;
+  return d;
+}
+
+// Compute units of the current device (queried once per device).
+static int device_cus() {
+  static int cache[64] = {0};
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (dev < 0 || dev >= 64) dev = 0;
+  if (cache[dev] <= 0) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+    cache[dev] = cus;
+  }
+  return cache[dev];
+}
+
+// Waves per workgroup of the streaming row pass; two workgroups share a CU.
+constexpr int ROWS_NW = 4;
+
+// K splits of the streaming row pass: the count ks in [1, 8] that minimises
+// (rounds of workgroups over 2 per CU) x (bins per wave + a fixed cost of
+// 16 bins for the operand prologue and the wave reduction).
+static int pair_rows_splits(int npairs, int n) {
+  const long long slots = 2LL * device_cus();
+  const double H = n / 2 + 1;
+  int best = 1;
+  double bc = 1e300;
+  for (int ks = 1; ks <= 8; ks++) {
+    const long long rounds = ((long long)npairs * ks + slots - 1) / slots;
+    const double cost = (double)rounds * (H / (ks * ROWS_NW) + 16.0);
+    if (cost < bc) {
+      bc = cost;
+      best = ks;
+    }
+  }
+  return best;
+}
+
+size_t pair_rows_part_elems(int npairs, int n, int box) {
+  const int ks = pair_rows_splits(std::max(1, npairs), n);
+  return ks > 1 ? (size_t)npairs * ks * box * box : 0;
+}
+
 // Persistent grid: every workgroup resident at once (the item ranges are
 // static), a multiple of 8 so that each XCD gets the same number.
 template <class K>
@@ -976,11 +1156,32 @@ hipError_t launch_pair_cols(const float2* spectra, size_t spec_stride, const int
 }
 
 hipError_t launch_pair_rows(const float2* tbuf, const float* tmax, int npairs, int n, int box,
-                            float* boxmaps, int precision, int variant, hipStream_t st) {
+                            float* boxmaps, float* part, int precision, int variant,
+                            hipStream_t st) {
   if (npairs <= 0) return hipSuccess;
   if (!pair_form_supported(precision, n, box)) return hipErrorInvalidValue;
   const float2* tw = twiddles(n);
   if (!tw) return hipErrorOutOfMemory;
+  if (precision == 0 && variant == PAIR_ROWS_AUTO && box <= 96) {
+    const float* ef = row_twiddles_stream(n, box);
+    if (!ef) return hipErrorOutOfMemory;
+    const int ks = pair_rows_splits(npairs, n);
+    if (ks > 1 && !part) return hipErrorInvalidValue;
+    float* dst = ks > 1 ? part : boxmaps;
+    const float* tf = reinterpret_cast<const float*>(tbuf);
+    constexpr int NT = ROWS_NW * 64;
+    switch ((box + 31) / 32) {
+      case 1: hipLaunchKernelGGL((k_pair_rows_stream<1, ROWS_NW, 16>), dim3(npairs, ks), dim3(NT), 0, st, tf, n, box, ks, dst, ef); break;
+      case 2: hipLaunchKernelGGL((k_pair_rows_stream<2, ROWS_NW, 12>), dim3(npairs, ks), dim3(NT), 0, st, tf, n, box, ks, dst, ef); break;
+      default: hipLaunchKernelGGL((k_pair_rows_stream<3, ROWS_NW, 8>), dim3(npairs, ks), dim3(NT), 0, st, tf, n, box, ks, dst, ef); break;
+    }
+    if (ks > 1) {
+      const long long total = (long long)npairs * box * box;
+      hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, part,
+                         ks, box * box, total, boxmaps);
+    }
+    return hipGetLastError();
+  }
   const float2* etab = (precision == 1 || variant != PAIR_ROWS_VALU) ? row_twiddles(n, box) : nullptr;
   if ((precision == 1 || variant != PAIR_ROWS_VALU) && !etab) return hipErrorOutOfMemory;
   if (precision == 1) {

@@ -150,7 +150,7 @@ def test_autotune_picks_valid_shapes_and_keeps_results(ctx, tmp_path):
     rep = ctx.autotune(d, frames.shape, frames.dtype, prm, cache_path=cache)
     shape, cols_p = rep["r2c_variant"] & 7, rep["r2c_variant"] >> 3
     assert shape in (1, 2) and cols_p in (0, 1) and rep["chunk_frames"] in (1, 2, 4, 8)
-    assert rep["pair_cols_variant"] in (0, 1, 2, 3) and rep["pair_rows_variant"] in (0, 1, 2)
+    assert rep["pair_cols_variant"] in (0, 1, 2, 3) and rep["pair_rows_variant"] in (0, 1, 2, 3)
     best = min(v for blk in rep["ms_stage1"] for row in blk for v in row if v > 0)
     assert rep["ms_stage1"][cols_p][shape][[1, 2, 4, 8].index(rep["chunk_frames"])] == best
     tuned = ctx.align_movie(d, frames.shape, frames.dtype, prm, want=("frame_shift", "sum_image"))

@@ -1,0 +1,64 @@
+"""GPU tests of the pair stage's box maps over the launch shapes of the row
+pass (the C2R row transform restricted to the box, reference DFAlign.cu:897
+with FindShift's read-back, cufunc.cu:447-456): pair counts from 1 to 300
+(with fewer pairs than CUs the kx range of a pair is split over up to 8
+workgroups whose partial maps are added in order) and boxes of one, two and
+three 32-row blocks, whole and partial.  Spectra of random real images are
+set directly, so the only difference from the FP64 oracle's cross power
+(cufunc.cu:392-410) and pruned box inverse is the FP32 arithmetic."""
+import numpy as np
+import pytest
+
+import motioncorr1exp_amd as mc
+from _oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+N, F = 2048, 25
+
+
+@pytest.fixture(scope="module")
+def loaded():
+    rng = np.random.default_rng(4242)
+    spec = np.fft.rfft2(rng.standard_normal((F, N, N))).astype(np.complex64)
+    ctx = mc.Context(0)
+    ctx.set_spectra(spec.view(np.float32))
+    yield ctx, spec
+    ctx.close()
+
+
+@pytest.fixture(scope="module")
+def o():
+    return Oracle("f64")
+
+
+def _pairs(npairs):
+    allp = np.array([(a, b) for a in range(F) for b in range(a + 1, F)], np.int32)
+    order = np.random.default_rng(7).permutation(len(allp))
+    return allp[order[:npairs]]
+
+
+@pytest.mark.parametrize("npairs", [1, 7, 40, 129, 300])
+@pytest.mark.parametrize("box", [32, 48, 64, 80, 96])
+def test_box_maps_match_oracle(loaded, o, npairs, box):
+    ctx, spec = loaded
+    pairs = _pairs(npairs)
+    _, _, boxes = ctx.pair_shifts(pairs, 150.0, box, want_boxes=True)
+    check = sorted({0, npairs - 1, npairs // 2, npairs // 3})
+    worst = 0.0
+    for q in check:
+        a, b = pairs[q]
+        ref = o.pruned_box(o.cross_power(spec[a], spec[b], N, 150.0), box)
+        worst = max(worst, float(np.abs(boxes[q] - ref).max() / np.abs(ref).max()))
+    print("pairs", npairs, "box", box, "max error / max", worst, flush=True)
+    assert worst < 1e-5
+
+
+def test_split_and_unsplit_maps_agree(loaded):
+    """The same pair alone (kx range split over 8 workgroups) and among 300
+    (one workgroup per pair) gives the same map up to FP32 summation order."""
+    ctx, _ = loaded
+    pairs = _pairs(300)
+    _, _, many = ctx.pair_shifts(pairs, 150.0, 96, want_boxes=True)
+    for q in (0, 150, 299):
+        _, _, one = ctx.pair_shifts(pairs[q:q + 1], 150.0, 96, want_boxes=True)
+        assert np.abs(one[0] - many[q]).max() <= 5e-6 * np.abs(many[q]).max()
