@@ -109,38 +109,60 @@ __global__ __launch_bounds__(NT) void k_find_peaks(const float* __restrict__ box
     G[ky][kx] = make_double2(re * m, im * m);
   }
   __syncthreads();
-  // 3-4. exact evaluation of the 512 x 512 interpolated map, first maximum
+  // 3-4. exact evaluation of the 512 x 512 interpolated map, first maximum.
+  // Thread t owns the columns c0 + 128 j (c0 = t % 128, j < 4) in the rows
+  // q = t / 128 mod 4: the eight twiddles of each column stay in registers, and
+  // the row sums Hq[kx] of QC rows at a time are formed in LDS, where every
+  // lane reads the same address (a broadcast).  Each point keeps the
+  // arithmetic of a per-point evaluation, so the map does not depend on the
+  // thread layout; the first maximum in row-major order is the reduction's
+  // (value, smallest index).
+  static_assert(NT == 512, "four columns and one row class per thread");
+  constexpr int QC = 128;
+  __shared__ double2 Hs[QC][9];
   const int scale = 32;
   const int wns = scale * wnoise;
   const int ox = (box / 2 - offx) * scale, oy = (box / 2 - offy) * scale;
+  constexpr int NC = 4;
+  const int c0 = tid % (512 / NC), par = tid / (512 / NC);
+  double2 wc[NC][8];
+#pragma unroll
+  for (int j = 0; j < NC; j++)
+#pragma unroll
+    for (int kx = 1; kx < 9; kx++) wc[j][kx - 1] = t512[(kx * (c0 + (512 / NC) * j)) & 511];
   ValIdx bb = {-1e300, 0x7fffffff};
   double u0 = 0.0;
-  for (int qq = 0; qq < 512 / NT; qq++) {
-    const int q = tid + NT * qq;
-    double2 Hq[9];
-#pragma unroll
-    for (int kx = 0; kx < 9; kx++) Hq[kx] = make_double2(0.0, 0.0);
-    for (int ky = 0; ky < 16; ky++) {
-      const int yy = ky < 8 ? ky : ky - 16;
-      const double2 w = t512[(yy * q) & 511];
-#pragma unroll
-      for (int kx = 0; kx < 9; kx++) {
+  for (int q0 = 0; q0 < 512; q0 += QC) {
+    __syncthreads();  // the previous chunk's rows are read
+    for (int i = tid; i < QC * 9; i += NT) {
+      const int q = q0 + i / 9, kx = i % 9;
+      double2 h = make_double2(0.0, 0.0);
+      for (int ky = 0; ky < 16; ky++) {
+        const int yy = ky < 8 ? ky : ky - 16;
+        const double2 w = t512[(yy * q) & 511];
         const double2 g = G[ky][kx];
-        Hq[kx].x += g.x * w.x - g.y * w.y;
-        Hq[kx].y += g.x * w.y + g.y * w.x;
+        h.x += g.x * w.x - g.y * w.y;
+        h.y += g.x * w.y + g.y * w.x;
       }
+      Hs[i / 9][kx] = h;
     }
-    const bool rowex = abs(q - oy) <= wns;
-    for (int pp = 0; pp < 512; pp++) {
-      double u = Hq[0].x;
+    __syncthreads();
+    for (int qi = par; qi < QC; qi += NC) {
+      const int q = q0 + qi;
+      const bool rowex = abs(q - oy) <= wns;
+      double2 hq[9];
 #pragma unroll
-      for (int kx = 1; kx < 9; kx++) {
-        const double2 w = t512[(kx * pp) & 511];
-        u += Hq[kx].x * w.x - Hq[kx].y * w.y;
+      for (int kx = 0; kx < 9; kx++) hq[kx] = Hs[qi][kx];
+#pragma unroll
+      for (int j = 0; j < NC; j++) {
+        const int pp = c0 + (512 / NC) * j;
+        double u = hq[0].x;
+#pragma unroll
+        for (int kx = 1; kx < 9; kx++) u += hq[kx].x * wc[j][kx - 1].x - hq[kx].y * wc[j][kx - 1].y;
+        if (q == 0 && pp == 0) u0 = u;
+        if (rowex && abs(pp - ox) <= wns) continue;
+        if (u > bb.v) bb = ValIdx{u, q * 512 + pp};
       }
-      if (q == 0 && pp == 0) u0 = u;
-      if (rowex && abs(pp - ox) <= wns) continue;
-      if (u > bb.v) bb = ValIdx{u, q * 512 + pp};
     }
   }
   __syncthreads();
