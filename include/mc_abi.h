@@ -365,7 +365,10 @@ int mc_threshold(const int *pairs, int npairs, int nframe, const double *ki, dou
                  int *good);
 
 /* Kernel timing over a region (HIP events on the context's stream around
- * every launch of the named kernel group). */
+ * every launch of the named kernel group).  mc_kernel_timing times every
+ * group (enable != 0) or none; mc_kernel_timing_groups times only the groups
+ * whose bit (1u << group) is set in mask (0: none), so that a timed region
+ * carries only the events it needs.  Either call clears earlier records. */
 #define MC_KT_SPECTRA 0
 #define MC_KT_PAIR_COLS 1
 #define MC_KT_PAIR_ROWS 2
@@ -375,6 +378,7 @@ int mc_threshold(const int *pairs, int npairs, int nframe, const double *ki, dou
 #define MC_KT_ROUNDS 6 /* graph replays of whole align-to-sum rounds */
 #define MC_KT_COUNT 7
 int mc_kernel_timing(mc_context ctx, int enable);
+int mc_kernel_timing_groups(mc_context ctx, unsigned mask);
 int mc_kernel_timing_get(mc_context ctx, int group, int *count, double *total_ms);
 
 #ifdef __cplusplus

@@ -129,7 +129,8 @@ ABI_SYMBOLS = [
     "mc_fft_log_modulus", "mc_make_pos_list", "mc_shift_cc", "mc_shift", "mc_find_shift",
     "mc_context_create", "mc_context_destroy", "mc_align_movie", "mc_load_frames",
     "mc_set_spectra", "mc_get_spectra", "mc_pair_shifts", "mc_find_peaks", "mc_sum_image",
-    "mc_pair_list", "mc_lsq", "mc_threshold", "mc_kernel_timing", "mc_kernel_timing_get",
+    "mc_pair_list", "mc_lsq", "mc_threshold", "mc_kernel_timing", "mc_kernel_timing_groups",
+    "mc_kernel_timing_get",
     "mc_sharpness", "mc_align_batch", "mc_autotune", "mc_tune_load", "mc_fsc_bins",
     "mc_fsc_table", "mc_disp_short",
 ]
@@ -181,6 +182,7 @@ def lib():
         L.mc_lsq.argtypes = [vp, i, i, vp, vp]
         L.mc_threshold.argtypes = [vp, i, i, vp, d, vp]
         L.mc_kernel_timing.argtypes = [vp, i]
+        L.mc_kernel_timing_groups.argtypes = [vp, ctypes.c_uint]
         L.mc_kernel_timing_get.argtypes = [vp, i, _P(i), _P(d)]
         L.mc_sharpness.argtypes = [vp, vp, f, _P(d), vp]
         L.mc_align_batch.argtypes = [vp, _P(BatchItem), i, _P(BatchReport)]
@@ -472,8 +474,14 @@ class Context:
         check(self.L.mc_disp_short(self.h, dev_buf.ptr, size, out.ctypes.data))
         return out
 
-    def kernel_timing(self, enable):
-        check(self.L.mc_kernel_timing(self.h, 1 if enable else 0))
+    def kernel_timing(self, enable, groups=None):
+        """Time every kernel group (enable) or none; with `groups` (names of
+        KT), only those groups."""
+        if groups is None:
+            check(self.L.mc_kernel_timing(self.h, 1 if enable else 0))
+        else:
+            mask = sum(1 << KT[g] for g in groups) if enable else 0
+            check(self.L.mc_kernel_timing_groups(self.h, mask))
 
     def kernel_time(self, group):
         c = ctypes.c_int()

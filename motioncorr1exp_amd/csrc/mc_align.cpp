@@ -112,7 +112,7 @@ struct mc_context_s {
   size_t spec_stride = 0;
   int padP_nu = -1, padP_w = -1, padP_nkx = -1;
   // kernel timing
-  bool kt_on = false;
+  unsigned kt_mask = 0;  // bit g: kernel group g is timed
   KTimer kt[MC_KT_COUNT];
   std::vector<hipEvent_t> pool;
   size_t pool_used = 0;
@@ -133,13 +133,13 @@ struct KScope {
   int g;
   hipEvent_t a = nullptr;
   KScope(mc_context_s* c_, int g_) : c(c_), g(g_) {
-    if (c->kt_on) {
+    if ((c->kt_mask >> g) & 1u) {
       a = c->ev();
       if (a) (void)hipEventRecord(a, c->st);
     }
   }
   ~KScope() {
-    if (c->kt_on && a) {
+    if (a) {
       hipEvent_t b = c->ev();
       if (b) {
         (void)hipEventRecord(b, c->st);
@@ -183,13 +183,18 @@ extern "C" int mc_context_destroy(mc_context ctx) {
   return MC_OK;
 }
 
-extern "C" int mc_kernel_timing(mc_context ctx, int enable) {
+extern "C" int mc_kernel_timing_groups(mc_context ctx, unsigned mask) {
   if (!ctx) return set_err(MC_ERR_ARG, "ctx is NULL");
+  if (mask >> MC_KT_COUNT) return set_err(MC_ERR_ARG, "group mask 0x%x has bits past group %d", mask, MC_KT_COUNT - 1);
   HIP_TRY(hipStreamSynchronize(ctx->st));
   for (auto& k : ctx->kt) k.ev.clear();
   ctx->pool_used = 0;
-  ctx->kt_on = enable != 0;
+  ctx->kt_mask = mask;
   return MC_OK;
+}
+
+extern "C" int mc_kernel_timing(mc_context ctx, int enable) {
+  return mc_kernel_timing_groups(ctx, enable ? (1u << MC_KT_COUNT) - 1 : 0u);
 }
 
 extern "C" int mc_kernel_timing_get(mc_context ctx, int group, int* count, double* total_ms) {
@@ -1204,13 +1209,13 @@ static int round_graph(mc_context_s* c, RoundGraph& gr, const std::vector<int>& 
     (void)hipGraphExecDestroy(gr.exec);
     gr.exec = nullptr;
   }
-  const bool kt = c->kt_on;
-  c->kt_on = false;  // no event records inside the graph
+  const unsigned kt = c->kt_mask;
+  c->kt_mask = 0;  // no event records inside the graph
   HIP_TRY(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
   const int rc = round_launch(c, njobs, np, n, bfactor, box, wnoise, precision);
   hipGraph_t graph = nullptr;
   const hipError_t ec = hipStreamEndCapture(c->st, &graph);
-  c->kt_on = kt;
+  c->kt_mask = kt;
   if (rc != MC_OK) {
     if (graph) (void)hipGraphDestroy(graph);
     return rc;
