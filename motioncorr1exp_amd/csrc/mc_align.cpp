@@ -533,7 +533,7 @@ static int pair_reserve(mc_context_s* c, int np, int n, int box, int precision) 
   if (precision == MC_PREC_MFMA_HILO) HIP_TRY(c->tmax.ensure(sizeof(float) * (size_t)np * H));
   HIP_TRY(c->box.ensure(sizeof(float) * (size_t)np * box * box));
   HIP_TRY(c->boxpart.ensure(sizeof(float) * std::max<size_t>(1, mc::pair_rows_part_elems(np, n, box))));
-  HIP_TRY(c->peaks.ensure(sizeof(double) * 3 * (size_t)np));
+  HIP_TRY(c->peaks.ensure(sizeof(double) * mc::find_peaks_out_elems(np)));
   return MC_OK;
 }
 
@@ -791,7 +791,7 @@ extern "C" int mc_find_peaks(mc_context ctx, const float* boxmaps, int npairs, i
   if (!ctx || !boxmaps || npairs < 1 || box < 16) return set_err(MC_ERR_ARG, "bad arguments");
   HIP_TRY(hipSetDevice(ctx->device));
   HIP_TRY(ctx->box.ensure(sizeof(float) * (size_t)npairs * box * box));
-  HIP_TRY(ctx->peaks.ensure(sizeof(double) * 3 * npairs));
+  HIP_TRY(ctx->peaks.ensure(sizeof(double) * mc::find_peaks_out_elems(npairs)));
   HIP_TRY(hipMemcpyAsync(ctx->box.p, boxmaps, sizeof(float) * (size_t)npairs * box * box,
                          hipMemcpyHostToDevice, ctx->st));
   HIP_TRY(mc::launch_find_peaks(ctx->box.as<float>(), npairs, box, n, wnoise,

@@ -87,7 +87,10 @@ hipError_t launch_pair_rows(const float2* tbuf, const float* tmax, int npairs, i
                             hipStream_t st);
 
 // Peaks: integer peak + FP64 16x16 -> 512x512 sub-pixel search per box map.
-// out: 3 doubles per pair (sx, sy, cc).
+// out: 3 doubles per pair (sx, sy, cc), followed by per-slice scratch when
+// the maps of few pairs are split over several workgroups:
+// find_peaks_out_elems(npairs) doubles in all.
+size_t find_peaks_out_elems(int npairs);
 hipError_t launch_find_peaks(const float* boxmaps, int npairs, int box, int n, int wnoise,
                              double* out, hipStream_t st);
 

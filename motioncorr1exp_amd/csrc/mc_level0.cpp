@@ -266,7 +266,7 @@ extern "C" int mc_find_shift(const float* dsrc, int n, float* hboxmap, int box, 
   float* dbox = nullptr;
   double* dres = nullptr;
   L0_TRY(hipMalloc(&dbox, sizeof(float) * (size_t)box * box));
-  hipError_t e = hipMalloc(&dres, sizeof(double) * 3);
+  hipError_t e = hipMalloc(&dres, sizeof(double) * mc::find_peaks_out_elems(1));
   if (e == hipSuccess)
     e = hipMemcpy2D(dbox, sizeof(float) * box, dsrc + (size_t)ori * (n + 2) + ori,
                     sizeof(float) * (n + 2), sizeof(float) * box, box, hipMemcpyDeviceToDevice);

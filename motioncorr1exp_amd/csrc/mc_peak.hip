@@ -33,10 +33,30 @@ MC_DEV ValIdx better(ValIdx a, ValIdx b) {
   return a;
 }
 
-// NT threads (256 or 512): each thread evaluates 512 / NT rows of the map.
-template <int NT>
+// Result of pair p from the first strict maximum (v, i) over the map points
+// other than U[0]: the sequential search starts from U[0] with a strict '>'
+// (cufunc.cu:509-525); shift = o + p/32 - box/2, cc = U/(n^2 256).
+MC_DEV void peak_result(int p, double v, int i, double u0, int offx, int offy, int box, int n,
+                        double* __restrict__ out) {
+  double bv = u0;
+  int sxp = 0, syp = 0;
+  if (i != 0x7fffffff && v > u0) {
+    bv = v;
+    sxp = i % 512;
+    syp = i / 512;
+  }
+  out[3 * p + 0] = offx + sxp / 32.0 - box / 2;
+  out[3 * p + 1] = offy + syp / 32.0 - box / 2;
+  out[3 * p + 2] = bv / n / n / 16 / 16;
+}
+
+// One workgroup per (pair, slice of 512/NS map rows), 512 threads.  NS = 1
+// writes the result; NS > 1 writes the slice's (max value, index, U[0],
+// window origin) to part[(p NS + slice) 5 ...] for k_find_peaks_final.
+template <int NT, int NS>
 __global__ __launch_bounds__(NT) void k_find_peaks(const float* __restrict__ boxmaps, int box,
-                                                   int n, int wnoise, double* __restrict__ out) {
+                                                   int n, int wnoise, double* __restrict__ out,
+                                                   double* __restrict__ part) {
   static_assert(NT == 256 || NT == 512, "threads");
   __shared__ double sv[NT];
   __shared__ int si[NT];
@@ -132,7 +152,9 @@ __global__ __launch_bounds__(NT) void k_find_peaks(const float* __restrict__ box
     for (int kx = 1; kx < 9; kx++) wc[j][kx - 1] = t512[(kx * (c0 + (512 / NC) * j)) & 511];
   ValIdx bb = {-1e300, 0x7fffffff};
   double u0 = 0.0;
-  for (int q0 = 0; q0 < 512; q0 += QC) {
+  static_assert((512 / NS) % QC == 0, "whole row chunks per slice");
+  const int qb = blockIdx.y * (512 / NS);
+  for (int q0 = qb; q0 < qb + 512 / NS; q0 += QC) {
     __syncthreads();  // the previous chunk's rows are read
     for (int i = tid; i < QC * 9; i += NT) {
       const int q = q0 + i / 9, kx = i % 9;
@@ -179,26 +201,79 @@ __global__ __launch_bounds__(NT) void k_find_peaks(const float* __restrict__ box
     __syncthreads();
   }
   if (tid == 0) {
-    // sequential search starts from U[0] with strict '>' (cufunc.cu:509-525)
-    double bv = u0;
-    int sxp = 0, syp = 0;
-    if (si[0] != 0x7fffffff && sv[0] > u0) {
-      bv = sv[0];
-      sxp = si[0] % 512;
-      syp = si[0] / 512;
+    if (NS == 1) {
+      peak_result(p, sv[0], si[0], u0, offx, offy, box, n, out);
+    } else {
+      // U[0] is evaluated by thread 0 of slice 0 (row 0, column 0)
+      double* q = part + ((size_t)p * NS + blockIdx.y) * 5;
+      q[0] = sv[0];
+      q[1] = (double)si[0];
+      q[2] = u0;
+      q[3] = offx;
+      q[4] = offy;
     }
-    out[3 * p + 0] = offx + sxp / (double)scale - box / 2;
-    out[3 * p + 1] = offy + syp / (double)scale - box / 2;
-    out[3 * p + 2] = bv / n / n / 16 / 16;
   }
+}
+
+// Combines the NS slices of each pair (one thread per pair): first maximum
+// in row-major order over the slices, then the result as for one slice.
+template <int NS>
+__global__ __launch_bounds__(64) void k_find_peaks_final(const double* __restrict__ part,
+                                                         int npairs, int box, int n,
+                                                         double* __restrict__ out) {
+  const int p = blockIdx.x * 64 + threadIdx.x;
+  if (p >= npairs) return;
+  const double* q = part + (size_t)p * NS * 5;
+  ValIdx b = {q[0], (int)q[1]};
+#pragma unroll
+  for (int s = 1; s < NS; s++) b = better(b, ValIdx{q[5 * s], (int)q[5 * s + 1]});
+  peak_result(p, b.v, b.i, q[2], (int)q[3], (int)q[4], box, n, out);
+}
+
+// Row slices per pair: one workgroup per pair fills the device from half as
+// many pairs as CUs; fewer pairs split their maps over 2 or 4 workgroups.
+static int peak_slices(int npairs) {
+  static int cus = 0;
+  if (cus <= 0) {
+    int dev = 0, c = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        c <= 0)
+      c = 256;
+    cus = c;
+  }
+  return npairs >= cus / 2 ? 1 : (npairs >= cus / 4 ? 2 : 4);
+}
+
+size_t find_peaks_out_elems(int npairs) {
+  const int ns = peak_slices(npairs);
+  return (size_t)npairs * 3 + (ns > 1 ? (size_t)npairs * ns * 5 : 0);
 }
 
 hipError_t launch_find_peaks(const float* boxmaps, int npairs, int box, int n, int wnoise,
                              double* out, hipStream_t st) {
   if (npairs <= 0) return hipSuccess;
   if (box < 16) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_find_peaks<512>, dim3(npairs), dim3(512), 0, st, boxmaps, box, n, wnoise,
-                     out);
+  double* part = out + (size_t)npairs * 3;
+  const unsigned fg = (unsigned)((npairs + 63) / 64);
+  switch (peak_slices(npairs)) {
+    case 1:
+      hipLaunchKernelGGL((k_find_peaks<512, 1>), dim3(npairs), dim3(512), 0, st, boxmaps, box, n,
+                         wnoise, out, part);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_find_peaks<512, 2>), dim3(npairs, 2), dim3(512), 0, st, boxmaps, box,
+                         n, wnoise, out, part);
+      hipLaunchKernelGGL(k_find_peaks_final<2>, dim3(fg), dim3(64), 0, st, part, npairs, box, n,
+                         out);
+      break;
+    default:
+      hipLaunchKernelGGL((k_find_peaks<512, 4>), dim3(npairs, 4), dim3(512), 0, st, boxmaps, box,
+                         n, wnoise, out, part);
+      hipLaunchKernelGGL(k_find_peaks_final<4>, dim3(fg), dim3(64), 0, st, part, npairs, box, n,
+                         out);
+      break;
+  }
   return hipGetLastError();
 }
 
