@@ -17,6 +17,8 @@
 // a_0 = 1, a_kx = 2 (the C2R row weight), ky' = ky (ky < 8) or ky - 16.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "mc_device.h"
 #include "mc_kernels.h"
 
@@ -233,14 +235,16 @@ __global__ __launch_bounds__(64) void k_find_peaks_final(const double* __restric
 // Row slices per pair: one workgroup per pair fills the device from half as
 // many pairs as CUs; fewer pairs split their maps over 2 or 4 workgroups.
 static int peak_slices(int npairs) {
-  static int cus = 0;
+  static std::atomic<int> cache[64];
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (dev < 0 || dev >= 64) dev = 0;
+  int cus = cache[dev].load(std::memory_order_relaxed);
   if (cus <= 0) {
-    int dev = 0, c = 0;
-    (void)hipGetDevice(&dev);
-    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        c <= 0)
-      c = 256;
-    cus = c;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+    cache[dev].store(cus, std::memory_order_relaxed);
   }
   return npairs >= cus / 2 ? 1 : (npairs >= cus / 4 ? 2 : 4);
 }
