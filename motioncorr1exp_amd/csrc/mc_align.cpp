@@ -299,12 +299,29 @@ extern "C" int mc_lsq(const int* pairs, int np, int nframe, const double* b, dou
   const int K = nframe - 1;
   if (K <= 0 || np < K) return set_err(MC_ERR_ARG, "least squares needs >= %d rows", K);
   std::vector<double> N((size_t)K * K, 0.0), r((size_t)2 * K, 0.0);
+  // A^T A adds 1 over the block [i, j) x [i, j) of each pair: built from a
+  // (K+1)^2 difference array (four corner updates per pair, then 2-D prefix
+  // sums), O(np + K^2).  The entries are small integers, exact in any order.
+  std::vector<int> D((size_t)(K + 1) * (K + 1), 0);
   for (int q = 0; q < np; q++) {
     const int i = pairs[2 * q], j = pairs[2 * q + 1];
+    if (i < 0 || j > K) return set_err(MC_ERR_ARG, "pair %d (%d, %d) out of range", q, i, j);
+    if (i >= j) continue;  // an empty increment range adds nothing
+    D[(size_t)i * (K + 1) + i] += 1;
+    D[(size_t)i * (K + 1) + j] -= 1;
+    D[(size_t)j * (K + 1) + i] -= 1;
+    D[(size_t)j * (K + 1) + j] += 1;
     for (int k = i; k < j; k++) {
       r[2 * k] += b[2 * q];
       r[2 * k + 1] += b[2 * q + 1];
-      for (int l = i; l < j; l++) N[(size_t)k * K + l] += 1.0;
+    }
+  }
+  for (int k = 0; k < K; k++) {
+    int row = 0;
+    for (int l = 0; l < K; l++) {
+      row += D[(size_t)k * (K + 1) + l];
+      const int up = k > 0 ? (int)N[(size_t)(k - 1) * K + l] : 0;
+      N[(size_t)k * K + l] = (double)(up + row);
     }
   }
   // Cholesky N = L L^T (lower, in place)
