@@ -116,6 +116,8 @@ struct mc_context_s {
   KTimer kt[MC_KT_COUNT];
   std::vector<hipEvent_t> pool;
   size_t pool_used = 0;
+  // stage boundaries of a group alignment (created on first use, reused)
+  hipEvent_t gev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 
   hipEvent_t ev() {
     if (pool_used < pool.size()) return pool[pool_used++];
@@ -178,6 +180,8 @@ extern "C" int mc_context_destroy(mc_context ctx) {
                     &ctx->disp};
   for (DevBuf* b : bufs) b->release();
   for (hipEvent_t e : ctx->pool) (void)hipEventDestroy(e);
+  for (hipEvent_t e : ctx->gev)
+    if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->st);
   delete ctx;
   return MC_OK;
@@ -1274,14 +1278,9 @@ static int align_group(mc_context_s* c, std::vector<MovieRun*>& rs, bool use_gra
   c->n = n;
   c->spec_stride = stride;
   c->nframe = 0;  // resident single-movie state is rebuilt below
-  hipEvent_t ev[6];
-  for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
-  struct EvGuard {
-    hipEvent_t* e;
-    ~EvGuard() {
-      for (int i = 0; i < 6; i++) (void)hipEventDestroy(e[i]);
-    }
-  } guard{ev};
+  hipEvent_t* ev = c->gev;
+  for (int i = 0; i < 6; i++)
+    if (!ev[i]) HIP_TRY(hipEventCreate(&ev[i]));
   HIP_TRY(hipEventRecord(ev[0], c->st));
 
   // 1. spectra (DFAlign.cu:705-824), frame means and unaligned sums

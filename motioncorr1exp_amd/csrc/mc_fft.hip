@@ -284,8 +284,19 @@ __global__ __launch_bounds__(256) void k_fill(const float* rowsum, int nu, int h
                                               float* fill) {
   __shared__ double red[256];
   const int f = blockIdx.x;
+  const float* rs = rowsum + (size_t)f * nu;
+  // sixteen loads per thread in flight (clamped, unused past h), then added
+  // in the order y = tid, tid + 256, ...
+  constexpr int U = 16;
   double s = 0;
-  for (int y = threadIdx.x; y < h; y += 256) s += rowsum[(size_t)f * nu + y];
+  for (int y0 = threadIdx.x; y0 < h; y0 += 256 * U) {
+    float v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) v[u] = rs[min(y0 + 256 * u, h - 1)];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (y0 + 256 * u < h) s += v[u];
+  }
   red[threadIdx.x] = s;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
