@@ -137,6 +137,7 @@ __global__ __launch_bounds__(NT) void k_pair_cols(const float2* __restrict__ spe
 //    only on tid % 16 and stay in registers;
 //  combine: SP adjacent lanes share one output row and reduce with xor
 //    shuffles (no LDS round trip).
+// A launch covers the bins kx in [kxb, kxb + nkx) of every pair.
 template <int N, int SP, bool PF, int MINW>
 __global__ __launch_bounds__(256, (N <= 4096 ? MINW : 1)) void k_pair_cols_m128(const float2* __restrict__ spectra,
                                                         size_t spec_stride,
@@ -144,7 +145,8 @@ __global__ __launch_bounds__(256, (N <= 4096 ? MINW : 1)) void k_pair_cols_m128(
                                                         int npairs,
                                                         const float* __restrict__ wxy, int box,
                                                         float2* __restrict__ tbuf,
-                                                        const float2* __restrict__ tw) {
+                                                        const float2* __restrict__ tw,
+                                                        int kxb, int nkx) {
   constexpr int NT = 256;
   constexpr int M = 128;
   constexpr int A = N / M;
@@ -191,7 +193,7 @@ __global__ __launch_bounds__(256, (N <= 4096 ? MINW : 1)) void k_pair_cols_m128(
 #pragma unroll
   for (int i = 0; i < AS; i++) ctw[i] = tw[((i * SP + ch) * R) % N];
   const int xcd = blockIdx.x % 8, lid = blockIdx.x / 8, nper = gridDim.x / 8;
-  const long long items = (long long)npairs * H;
+  const long long items = (long long)npairs * nkx;
   const long long i0 = items * xcd / 8, i1 = items * (xcd + 1) / 8;
   // With PF (software pipeline) the raw spectrum values of the next item are
   // loaded into registers while the current item goes through stage 1 and
@@ -199,7 +201,7 @@ __global__ __launch_bounds__(256, (N <= 4096 ? MINW : 1)) void k_pair_cols_m128(
   // of 64 more VGPRs, i.e. one workgroup less per CU).
   float2 ra[P0][16], rb[P0][16];
   auto fetch = [&](long long it) {
-    const int kx = (int)(it / npairs), p = (int)(it % npairs);
+    const int kx = kxb + (int)(it / npairs), p = (int)(it % npairs);
     const int2 pr = pairs[p];
     const float2* sa = spectra + (size_t)pr.x * spec_stride + (size_t)kx * N;
     const float2* sb = spectra + (size_t)pr.y * spec_stride + (size_t)kx * N;
@@ -219,7 +221,7 @@ __global__ __launch_bounds__(256, (N <= 4096 ? MINW : 1)) void k_pair_cols_m128(
   };
   if (PF && i0 + lid < i1) fetch(i0 + lid);
   for (long long it = i0 + lid; it < i1; it += nper) {
-    const int kx = (int)(it / npairs), p = (int)(it % npairs);
+    const int kx = kxb + (int)(it / npairs), p = (int)(it % npairs);
     const float wx = wxy[kx];
     if (!PF) fetch(it);
     // ---- stage 0 from the prefetched registers
@@ -1041,15 +1043,27 @@ static hipError_t pair_cols_m128(const float2* spectra, size_t spec_stride, cons
   static const int g1 = persistent_grid(k_pair_cols_m128<N, SP, true, 1>, 256);
   static const int g0 = persistent_grid(k_pair_cols_m128<N, SP, false, 1>, 256);
   static const int g4 = persistent_grid(k_pair_cols_m128<N, SP, false, 4>, 256);
+  // The launch is split into S kx segments of about 128 items per workgroup.
+  // Within one launch the workgroups of an XCD drift apart as they go, and
+  // the frame columns of the kx values in flight outgrow its L2; a new
+  // launch lines them up again (3838x3710x40: 3.09 ms in one launch, 2.93 in
+  // four).
+  const int H = N / 2 + 1;
+  const int grid = mode == 1 ? g1 : (mode == 2 ? g4 : g0);
+  const long long per_wg = (long long)npairs * H / std::max(1, grid);
+  const int S = (int)std::max(1LL, std::min(8LL, (per_wg + 64) / 128));
+  for (int sgi = 0; sgi < S; sgi++) {
+    const int kxb = H * sgi / S, nkx = H * (sgi + 1) / S - kxb;
   if (mode == 1)
     hipLaunchKernelGGL((k_pair_cols_m128<N, SP, true, 1>), dim3(g1), dim3(256), 0, st, spectra,
-                       spec_stride, pairs, npairs, wxy, box, tbuf, tw);
+                       spec_stride, pairs, npairs, wxy, box, tbuf, tw, kxb, nkx);
   else if (mode == 2)
     hipLaunchKernelGGL((k_pair_cols_m128<N, SP, false, 4>), dim3(g4), dim3(256), 0, st, spectra,
-                       spec_stride, pairs, npairs, wxy, box, tbuf, tw);
+                       spec_stride, pairs, npairs, wxy, box, tbuf, tw, kxb, nkx);
   else
     hipLaunchKernelGGL((k_pair_cols_m128<N, SP, false, 1>), dim3(g0), dim3(256), 0, st, spectra,
-                       spec_stride, pairs, npairs, wxy, box, tbuf, tw);
+                       spec_stride, pairs, npairs, wxy, box, tbuf, tw, kxb, nkx);
+  }
   return hipGetLastError();
 }
 
