@@ -304,7 +304,8 @@ __global__ __launch_bounds__(256) void k_pair_cols_mfma(const float2* __restrict
                                                         const float* __restrict__ wxy, int box,
                                                         float2* __restrict__ tbuf,
                                                         float* __restrict__ tmax,
-                                                        const float2* __restrict__ tw) {
+                                                        const float2* __restrict__ tw,
+                                                        int kxb, int nkx) {
   constexpr int NT = 256;
   constexpr int M = 128;
   constexpr int A = N / M;
@@ -369,10 +370,10 @@ __global__ __launch_bounds__(256) void k_pair_cols_mfma(const float2* __restrict
 #pragma unroll
   for (int i = 0; i < AS; i++) ctw[i] = tw[((i * SP + ch) * R) % N];
   const int xcd = blockIdx.x % 8, lid = blockIdx.x / 8, nper = gridDim.x / 8;
-  const long long items = (long long)npairs * H;
+  const long long items = (long long)npairs * nkx;
   const long long i0 = items * xcd / 8, i1 = items * (xcd + 1) / 8;
   for (long long it = i0 + lid; it < i1; it += nper) {
-    const int kx = (int)(it / npairs), p = (int)(it % npairs);
+    const int kx = kxb + (int)(it / npairs), p = (int)(it % npairs);
     const int2 pr = pairs[p];
     const float2* sa = spectra + (size_t)pr.x * spec_stride + (size_t)kx * N;
     const float2* sb = spectra + (size_t)pr.y * spec_stride + (size_t)kx * N;
@@ -1034,6 +1035,16 @@ static hipError_t pair_cols_t(const float2* spectra, size_t spec_stride, const i
   return hipGetLastError();
 }
 
+// kx segments of a persistent column-pass launch: about 128 items per
+// workgroup each, at most 8.  Within one launch the workgroups of an XCD
+// drift apart as they go, and the frame columns of the kx values in flight
+// outgrow its L2; a new launch lines them up again (3838x3710x40: 3.09 ms in
+// one launch, 2.93 in four).
+static int col_segments(int npairs, int H, int grid) {
+  const long long per_wg = (long long)npairs * H / std::max(1, grid);
+  return (int)std::max(1LL, std::min(8LL, (per_wg + 64) / 128));
+}
+
 template <int N, int SP>
 static hipError_t pair_cols_m128(const float2* spectra, size_t spec_stride, const int2* pairs,
                                  int npairs, const float* wxy, int box, float2* tbuf, int mode,
@@ -1043,15 +1054,8 @@ static hipError_t pair_cols_m128(const float2* spectra, size_t spec_stride, cons
   static const int g1 = persistent_grid(k_pair_cols_m128<N, SP, true, 1>, 256);
   static const int g0 = persistent_grid(k_pair_cols_m128<N, SP, false, 1>, 256);
   static const int g4 = persistent_grid(k_pair_cols_m128<N, SP, false, 4>, 256);
-  // The launch is split into S kx segments of about 128 items per workgroup.
-  // Within one launch the workgroups of an XCD drift apart as they go, and
-  // the frame columns of the kx values in flight outgrow its L2; a new
-  // launch lines them up again (3838x3710x40: 3.09 ms in one launch, 2.93 in
-  // four).
   const int H = N / 2 + 1;
-  const int grid = mode == 1 ? g1 : (mode == 2 ? g4 : g0);
-  const long long per_wg = (long long)npairs * H / std::max(1, grid);
-  const int S = (int)std::max(1LL, std::min(8LL, (per_wg + 64) / 128));
+  const int S = col_segments(npairs, H, mode == 1 ? g1 : (mode == 2 ? g4 : g0));
   for (int sgi = 0; sgi < S; sgi++) {
     const int kxb = H * sgi / S, nkx = H * (sgi + 1) / S - kxb;
   if (mode == 1)
@@ -1074,8 +1078,13 @@ static hipError_t pair_cols_mfma(const float2* spectra, size_t spec_stride, cons
   const float2* tw = twiddles(N);
   if (!tw) return hipErrorOutOfMemory;
   static const int g = persistent_grid(k_pair_cols_mfma<N, SP>, 256);
-  hipLaunchKernelGGL((k_pair_cols_mfma<N, SP>), dim3(g), dim3(256), 0, st, spectra, spec_stride,
-                     pairs, npairs, wxy, box, tbuf, tmax, tw);
+  const int H = N / 2 + 1;
+  const int S = col_segments(npairs, H, g);
+  for (int sgi = 0; sgi < S; sgi++) {
+    const int kxb = H * sgi / S, nkx = H * (sgi + 1) / S - kxb;
+    hipLaunchKernelGGL((k_pair_cols_mfma<N, SP>), dim3(g), dim3(256), 0, st, spectra, spec_stride,
+                       pairs, npairs, wxy, box, tbuf, tmax, tw, kxb, nkx);
+  }
   return hipGetLastError();
 }
 
