@@ -136,11 +136,13 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_3838x3710x40_pad4096.
 PAD_SIZES = (1024, 1280, 1536, 1920, 2048, 2560, 3072, 3840, 4096)
 
 
-def committed_traffic(group):
+def committed_traffic(group, launches_per_movie=1.0):
     """HBM bytes per launch of a kernel group from the committed rocprofv3
     counter summary (FETCH_SIZE and WRITE_SIZE runs of this workload, scaled
     by the copy calibration): the sum over the group's kernels of their bytes
-    per dispatch (one dispatch of each per launch of the group)."""
+    per dispatch times their dispatches per movie (`per_movie`; a launch of
+    the column pass is several kx-segment dispatches), divided by the group's
+    launches per movie."""
     if not os.path.exists(TRAFFIC_FILE):
         return None
     tot, seen = 0.0, False
@@ -150,9 +152,9 @@ def committed_traffic(group):
                 continue
             kv = dict(tok.split("=", 1) for tok in line.split()[1:])
             if kv.get("group") == group:
-                tot += float(kv["bytes"])
+                tot += float(kv["bytes"]) * float(kv.get("per_movie", "1"))
                 seen = True
-    return tot if seen else None
+    return tot / launches_per_movie if seen else None
 
 
 def batch_leg(n_movies, steps, seed=1004, tune=True, tune_cache=None):
@@ -414,7 +416,7 @@ def main():
     achieved = alg / lpm / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else 0.0
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                "traffic": committed_traffic(dom),
+                "traffic": committed_traffic(dom, lpm),
                 "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
                 "algorithmic_bytes_per_launch": alg / lpm}
     # the FFT groups are bound by vector-ALU issue: their arithmetic against
