@@ -1058,15 +1058,15 @@ static hipError_t pair_cols_m128(const float2* spectra, size_t spec_stride, cons
   const int S = col_segments(npairs, H, mode == 1 ? g1 : (mode == 2 ? g4 : g0));
   for (int sgi = 0; sgi < S; sgi++) {
     const int kxb = H * sgi / S, nkx = H * (sgi + 1) / S - kxb;
-  if (mode == 1)
-    hipLaunchKernelGGL((k_pair_cols_m128<N, SP, true, 1>), dim3(g1), dim3(256), 0, st, spectra,
-                       spec_stride, pairs, npairs, wxy, box, tbuf, tw, kxb, nkx);
-  else if (mode == 2)
-    hipLaunchKernelGGL((k_pair_cols_m128<N, SP, false, 4>), dim3(g4), dim3(256), 0, st, spectra,
-                       spec_stride, pairs, npairs, wxy, box, tbuf, tw, kxb, nkx);
-  else
-    hipLaunchKernelGGL((k_pair_cols_m128<N, SP, false, 1>), dim3(g0), dim3(256), 0, st, spectra,
-                       spec_stride, pairs, npairs, wxy, box, tbuf, tw, kxb, nkx);
+    if (mode == 1)
+      hipLaunchKernelGGL((k_pair_cols_m128<N, SP, true, 1>), dim3(g1), dim3(256), 0, st, spectra,
+                         spec_stride, pairs, npairs, wxy, box, tbuf, tw, kxb, nkx);
+    else if (mode == 2)
+      hipLaunchKernelGGL((k_pair_cols_m128<N, SP, false, 4>), dim3(g4), dim3(256), 0, st,
+                         spectra, spec_stride, pairs, npairs, wxy, box, tbuf, tw, kxb, nkx);
+    else
+      hipLaunchKernelGGL((k_pair_cols_m128<N, SP, false, 1>), dim3(g0), dim3(256), 0, st,
+                         spectra, spec_stride, pairs, npairs, wxy, box, tbuf, tw, kxb, nkx);
   }
   return hipGetLastError();
 }
