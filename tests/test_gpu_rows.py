@@ -62,3 +62,25 @@ def test_split_and_unsplit_maps_agree(loaded):
     for q in (0, 150, 299):
         _, _, one = ctx.pair_shifts(pairs[q:q + 1], 150.0, 96, want_boxes=True)
         assert np.abs(one[0] - many[q]).max() <= 5e-6 * np.abs(many[q]).max()
+
+
+@pytest.mark.parametrize("npairs", [1, 40, 96, 129])
+def test_peak_search_slices_match_oracle(loaded, o, npairs):
+    """With fewer pairs than half the CUs, the peak search splits the 512 map
+    rows of each pair over 2 or 4 workgroups and combines their first maxima
+    (FindShift, cufunc.cu:445-546).  On 256 CUs, 1 and 40 pairs take 4
+    slices, 96 take 2 and 129 one.  Box maps with one Gaussian peak at a
+    random sub-pixel position over weak noise: shifts identical to the FP64
+    oracle's search of the same FP32 maps, cc within 1e-9."""
+    ctx, _ = loaded
+    rng = np.random.default_rng(900 + npairs)
+    y, x = np.mgrid[0:96, 0:96]
+    maps = np.empty((npairs, 96, 96), np.float32)
+    for q in range(npairs):
+        x0, y0 = rng.uniform(20, 76, 2)
+        maps[q] = 1e3 * np.exp(-((x - x0) ** 2 + (y - y0) ** 2) / 8.0) + rng.normal(0, 1.0, (96, 96))
+    b, cc = ctx.find_peaks(maps, N)
+    for q in range(npairs):
+        ox, oy, occ = o.find_shift_box(maps[q].astype(np.float64), N)
+        assert (b[q, 0], b[q, 1]) == (ox, oy), q
+        assert abs(cc[q] - occ) <= 1e-9 * abs(occ), q
