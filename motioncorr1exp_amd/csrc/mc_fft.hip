@@ -546,12 +546,25 @@ __global__ __launch_bounds__(NT) void k_c2r_rows(const float2* tmp, float scale,
   constexpr int TS = NH + 1;       // LDS row stride
   __shared__ float2 buf[ROWS * TS];
   const int y0 = blockIdx.x * ROWS;
-  // tmp[kx/4][y][kx%4]: 16 lanes read 4 bins x 4 rows = one 128-byte line
-  for (int i = threadIdx.x; i < ROWS * 4 * ((H + 3) / 4); i += NT) {
+  // tmp[kx/4][y][kx%4]: 16 lanes read 4 bins x 4 rows = one 128-byte line.
+  // Every load of the workgroup is issued before its first LDS write; the
+  // bins past n/2 in the last group of four exist in tmp and are not kept.
+  constexpr int TOT = ROWS * 4 * ((H + 3) / 4);
+  constexpr int NL = (TOT + NT - 1) / NT;
+  float2 vv[NL];
+#pragma unroll
+  for (int q = 0; q < NL; q++) {
+    const int i = min(threadIdx.x + q * NT, TOT - 1);
+    const int kk = i & 3, r = (i >> 2) % ROWS, kg = (i >> 2) / ROWS;
+    vv[q] = tmp[((size_t)kg * N + y0 + r) * 4 + kk];
+  }
+#pragma unroll
+  for (int q = 0; q < NL; q++) {
+    const int i = threadIdx.x + q * NT;
     const int kk = i & 3, r = (i >> 2) % ROWS, kg = (i >> 2) / ROWS;
     const int k = 4 * kg + kk;
-    if (k >= H) continue;
-    float2 v = tmp[((size_t)kg * N + y0 + r) * 4 + kk];
+    if (i >= TOT || k >= H) continue;
+    float2 v = vv[q];
     if (k == 0 || k == NH) v.y = 0.f;
     buf[r * TS + k] = v;
   }
