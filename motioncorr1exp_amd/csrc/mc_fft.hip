@@ -514,8 +514,13 @@ hipError_t launch_frames_to_spectra(const R2CGeom& g, int f0, int nframes, float
 // than written to HBM piecewise.
 template <int N, int NT>
 __global__ __launch_bounds__(NT) void k_cols_inv(const float2* spec, float2* tmp,
-                                                 const float2* __restrict__ tw) {
-  __shared__ float2 buf[N];
+                                                 const float2* __restrict__ gtw) {
+  // padded LDS indices and split twiddle tables, as in the forward column pass
+  constexpr int LB = tw_lbits(N);
+  constexpr int PB = 4;
+  __shared__ float2 buf[lds_len<PB>(N)];
+  __shared__ float2 tlo[1 << LB], thi[N >> LB];
+  tw_split_load(tlo, thi, gtw, N, LB, threadIdx.x, NT);
   constexpr int PER = (N + NT - 1) / NT;
   const int kx = xcd_remap(blockIdx.x, gridDim.x);
   const float2* src = spec + (size_t)kx * N;
@@ -528,12 +533,12 @@ __global__ __launch_bounds__(NT) void k_cols_inv(const float2* spec, float2* tmp
 #pragma unroll
   for (int k = 0; k < PER; k++) {
     const int i = threadIdx.x + k * NT;
-    if (N % NT == 0 || i < N) buf[i] = vv[k];
+    if (N % NT == 0 || i < N) buf[lix<PB>(i)] = vv[k];
   }
   __syncthreads();
-  fft_lds<N, NT, 1, 1>(buf, tw, N);
+  fft_lds_t<N, NT, 1, 1, N, TwSplit, PB>(buf, TwSplit{tlo, thi, LB, N}, N);
   float2* dst = tmp + (size_t)(kx >> 2) * N * 4 + (kx & 3);
-  for (int i = threadIdx.x; i < N; i += NT) dst[(size_t)i * 4] = buf[i];
+  for (int i = threadIdx.x; i < N; i += NT) dst[(size_t)i * 4] = buf[lix<PB>(i)];
 }
 
 // Row pass: C2R of ROWS rows of tmp with the DC/Nyquist projection
