@@ -94,3 +94,33 @@ def test_3838x3710x40_differentiation_form():
           np.abs(g - og).max() / np.abs(og).max(), flush=True)
     assert abs(J - oJ) <= 2e-6 * abs(oJ)
     assert np.abs(g - og).max() <= 1e-4 * np.abs(og).max()
+
+
+def test_8192_canvas_matches_oracle():
+    """The largest canvas: 8192x8192x5 float32, fod 1, one align-to-sum round,
+    so every n = 8192 kernel runs: stage 1, the column pass in kx segments,
+    the row pass, the peak search, the batched and final sums and the C2R of
+    the sum image.  Pair shifts (1/32-grid near-tie flips counted) and frame
+    shifts against the FP64 oracle on the same frames, the aligned sum to 1e-4
+    relative."""
+    F, n = 5, 8192
+    frames, _ = make_movie(n, n, F, dose=1.0, step=0.8, seed=2403)
+    print("movie ready", flush=True)
+    ctx = mc.Context(0)
+    p = mc.default_params(F, FrameDistOffset=1, iter=1, CCPeakSearchDim=96)
+    dev = mc.DeviceArray(frames)
+    r = ctx.align_movie(dev, frames.shape, frames.dtype, p,
+                        want=("pair_b", "frame_shift", "sum_image"))
+    dev.free()
+    ctx.close()
+    assert r["report"]["n"] == n
+    print("gpu done; oracle ...", flush=True)
+    o = Oracle("f64").align_movie(frames, dict(fod=1, iter=1, box=96, full_c2r=0, nthreads=16),
+                                  want_boxes=False)
+    db = np.abs(r["pair_b"] - o["pair_b"])
+    fs = np.abs(r["frame_shift"] - o["frame_shift"])
+    rel = float(np.linalg.norm(r["sum_image"] - o["sum_image"]) / np.linalg.norm(o["sum_image"]))
+    print("pair-shift max diff", db.max(), "frame-shift max diff", fs.max(), "sum rel", rel, flush=True)
+    assert np.all((db < 1e-9) | (np.abs(db - 1 / 32) < 1e-9)) and int((db > 1e-9).sum()) <= 1
+    assert fs.max() <= 1 / 32 + 1e-9
+    assert rel < 1e-4
