@@ -560,16 +560,31 @@ static int pair_reserve(mc_context_s* c, int np, int n, int box, int precision) 
 
 // Column pass, row pass and peak search over the np slot pairs in pairs_d;
 // results stay on the device (peaks, box maps).  Capturable.
+// groups: pair counts of consecutive movies (a batch).  The column pass then
+// runs movie by movie, so that the frame columns its items read at a time
+// belong to one movie and stay in L2; the per-pair arithmetic is the same.
 static int pair_launch(mc_context_s* c, int np, int n, float bfactor, int box, int wnoise,
-                       int precision, int cols = -1, int rows = -1) {
+                       int precision, int cols = -1, int rows = -1,
+                       const std::vector<int>* groups = nullptr) {
   const mc_context_s::Tune* tn = pair_tune(c, n, box, precision);
   if (cols < 0) cols = tn ? tn->cols : mc::PAIR_COLS_AUTO;
   if (rows < 0) rows = tn ? tn->rows : mc::PAIR_ROWS_AUTO;
   {
     KScope ks(c, MC_KT_PAIR_COLS);
-    HIP_TRY(mc::launch_pair_cols(c->spec.as<float2>(), c->spec_stride, c->pairs_d.as<int2>(), np,
-                                 n, bfactor, box, c->tbuf.as<float2>(), c->tmax.as<float>(),
-                                 precision, cols, c->st));
+    const int H = n / 2 + 1;
+    std::vector<int> one(1, np);
+    const std::vector<int>& gs = groups && groups->size() > 1 ? *groups : one;
+    int p0 = 0;
+    for (int g : gs) {
+      if (g <= 0) continue;
+      float* tmax = precision == MC_PREC_MFMA_HILO ? c->tmax.as<float>() + (size_t)p0 * H : nullptr;
+      HIP_TRY(mc::launch_pair_cols(c->spec.as<float2>(), c->spec_stride,
+                                   c->pairs_d.as<int2>() + p0, g, n, bfactor, box,
+                                   c->tbuf.as<float2>() + (size_t)p0 * H * box, tmax, precision,
+                                   cols, c->st));
+      p0 += g;
+    }
+    if (p0 != np) return set_err(MC_ERR_ARG, "pair groups cover %d of %d pairs", p0, np);
   }
   {
     KScope ks(c, MC_KT_PAIR_ROWS);
@@ -587,13 +602,14 @@ static int pair_launch(mc_context_s* c, int np, int n, float bfactor, int box, i
 
 // Pair stage on resident slots: pairs (a, b) are slot indices.
 static int pair_stage(mc_context_s* c, const std::vector<int>& slots, float bfactor, int box,
-                      int wnoise, int precision, std::vector<double>& res, float* boxmaps_host) {
+                      int wnoise, int precision, std::vector<double>& res, float* boxmaps_host,
+                      const std::vector<int>* groups = nullptr) {
   const int np = (int)slots.size() / 2;
   const int n = c->n;
   MC_TRY(pair_reserve(c, np, n, box, precision));
   HIP_TRY(hipMemcpyAsync(c->pairs_d.p, slots.data(), sizeof(int) * slots.size(),
                          hipMemcpyHostToDevice, c->st));
-  MC_TRY(pair_launch(c, np, n, bfactor, box, wnoise, precision));
+  MC_TRY(pair_launch(c, np, n, bfactor, box, wnoise, precision, -1, -1, groups));
   res.resize(3 * (size_t)np);
   HIP_TRY(hipMemcpyAsync(res.data(), c->peaks.p, sizeof(double) * 3 * np, hipMemcpyDeviceToHost,
                          c->st));
@@ -1330,7 +1346,8 @@ static int align_group(mc_context_s* c, std::vector<MovieRun*>& rs, bool use_gra
   }
   HIP_TRY(hipEventRecord(ev[1], c->st));
 
-  // 2. pairwise shifts of every movie in one launch (DFAlign.cu:875-905)
+  // 2. pairwise shifts of every movie (DFAlign.cu:875-905): the column pass
+  // movie by movie, the row pass and the peak search over all pairs at once
   std::vector<int> slots;
   for (MovieRun* r : rs)
     for (int q = 0; q < r->P; q++) {
@@ -1338,7 +1355,9 @@ static int align_group(mc_context_s* c, std::vector<MovieRun*>& rs, bool use_gra
       slots.push_back(r->base + r->off2 + r->pairs[2 * q + 1]);
     }
   std::vector<double> res;
-  MC_TRY(pair_stage(c, slots, bfactor, box, wnoise, precision, res, nullptr));
+  std::vector<int> groups;
+  for (MovieRun* r : rs) groups.push_back(r->P);
+  MC_TRY(pair_stage(c, slots, bfactor, box, wnoise, precision, res, nullptr, &groups));
   HIP_TRY(hipEventRecord(ev[2], c->st));
   {
     size_t po = 0;
