@@ -51,44 +51,44 @@ const char *mc_version(void);
  * ==================================================================== */
 typedef struct mc_plan_s *mc_plan;
 
-/* initGPU (cufunc.cu:79-114): select device; MC_ERR_ARG if out of range. */
+/* initGPU (cufunc.cu:30-65): select device; MC_ERR_ARG if out of range. */
 int mc_init_gpu(int gpu);
-/* ResetGPU (cufunc.cu:116-120). */
+/* ResetGPU (cufunc.cu:67-71). */
 int mc_reset_gpu(void);
-/* getGPUList (cufunc.cu:129-152): number of devices. */
+/* getGPUList (cufunc.cu:80-103): number of devices. */
 int mc_device_count(void);
-/* GPUMemCheck (cufunc.cu:154-158). */
+/* GPUMemCheck (cufunc.cu:105-109). */
 int mc_mem_check(size_t *free_bytes, size_t *total_bytes);
-/* GPUMemAlloc/Zero/Free/H2D/D2H/D2D (cufunc.cu:160-194); size_t counts. */
+/* GPUMemAlloc/Zero/Free/H2D/D2H/D2D (cufunc.cu:111-145); size_t counts. */
 int mc_mem_alloc(void **buf, size_t bytes);
 int mc_mem_zero(void *buf, size_t bytes);
 int mc_mem_free(void *buf);
 int mc_mem_h2d(void *dst, const void *src, size_t bytes);
 int mc_mem_d2h(void *dst, const void *src, size_t bytes);
 int mc_mem_d2d(void *dst, const void *src, size_t bytes);
-/* GPUMemBinD2D / GPUMemBinD2H (cufunc.cu:196-225): Fourier crop of a padded
+/* GPUMemBinD2D / GPUMemBinD2H (cufunc.cu:147-176): Fourier crop of a padded
  * spectrum of size src_n to dst_n (rows 0..dst_n/2-1 and the last dst_n/2
  * rows, dst_n+2 floats each). */
 int mc_mem_bin_d2d(float *dst, const float *src, int dst_n, int src_n);
 int mc_mem_bin_d2h(float *dst_host, const float *src, int dst_n, int src_n);
-/* GPUFFTPlan / GPUIFFTPlan (cufunc.cu:251-264): n x n in-place plans.
+/* GPUFFTPlan / GPUIFFTPlan (cufunc.cu:202-215): n x n in-place plans.
  * Supported n: powers of two from 64 to 8192, and 768, 1280, 1536, 1920,
  * 2560, 3072, 3840, 5120, 6144 (2^k * 3, 5 or 15). */
 int mc_fft_plan(mc_plan *plan, int n);
 int mc_ifft_plan(mc_plan *plan, int n);
-/* GPUFFT2d / GPUIFFT2d (cufunc.cu:272-282): unnormalised, in place. */
+/* GPUFFT2d / GPUIFFT2d (cufunc.cu:223-233): unnormalised, in place. */
 int mc_fft2d(float *buf, mc_plan plan);
 int mc_ifft2d(float *buf, mc_plan plan);
-/* GPUFFTDestroy (cufunc.cu:266-270). */
+/* GPUFFTDestroy (cufunc.cu:217-221). */
 int mc_fft_destroy(mc_plan plan);
-/* GPUSync (cufunc.cu:284-289). */
+/* GPUSync (cufunc.cu:235-241). */
 int mc_sync(void);
-/* GPUAdd / GPUMultiplyNum (cufunc.cu:292-312): counts in floats. */
+/* GPUAdd / GPUMultiplyNum (cufunc.cu:243-263): counts in floats. */
 int mc_add(float *dst, const float *src, size_t nfloats);
 int mc_multiply_num(float *dst, float num, size_t nfloats);
-/* GPUFFTLogModulus (cufunc.cu:227-237): dmod[i] = log(1+|F_i|*scale). */
+/* GPUFFTLogModulus (cufunc.cu:178-188): dmod[i] = log(1+|F_i|*scale). */
 int mc_fft_log_modulus(float *dmod, const float *dfft, int n, float scale);
-/* MASK of cufunc.h:9-14; MkPosList(MASK*, n, bfactor) (cufunc.cu:358-390)
+/* MASK of cufunc.h:9-14; MkPosList(MASK*, n, bfactor) (cufunc.cu:309-341)
  * fills a host table of n*(n/2+1) entries. */
 typedef struct {
   int x;
@@ -96,12 +96,12 @@ typedef struct {
   float z;
 } mc_mask;
 int mc_make_pos_list(mc_mask *list, int n, float bfactor);
-/* GPUShiftCC (cufunc.cu:392-419): dfft <- dsum*conj(dfft*e^{i th})*z/sizec. */
+/* GPUShiftCC (cufunc.cu:343-370): dfft <- dsum*conj(dfft*e^{i th})*z/sizec. */
 int mc_shift_cc(float *dfft, const float *dsum, const mc_mask *dposlist, float sx, float sy,
                 int n);
-/* GPUShift (cufunc.cu:421-443): dfft <- dfft * e^{i th}, th = (2 pi/n)(sx x + sy y'). */
+/* GPUShift (cufunc.cu:372-394): dfft <- dfft * e^{i th}, th = (2 pi/n)(sx x + sy y'). */
 int mc_shift(float *dfft, const mc_mask *dposlist, float sx, float sy, int n);
-/* FindShift (cufunc.cu:445-546): box read-out of a C2R output (pitch n+2),
+/* FindShift (cufunc.cu:396-497): box read-out of a C2R output (pitch n+2),
  * integer and 1/32-pixel peak (FP64).  hboxmap (host, box*box floats) gets
  * the box; returns 0 and writes cc. */
 int mc_find_shift(const float *dsrc, int n, float *hboxmap, int box, float *sx, float *sy,
@@ -358,7 +358,7 @@ int mc_sum_image(mc_context ctx, const double *frame_shift, float *image);
 int mc_sharpness(mc_context ctx, const double *frame_shift, float bfactor, double *J,
                  double *grad);
 
-/* Host FP64 drift model (mfunc.h:202-298, DFAlign.cu:914-948). */
+/* Host FP64 drift model (mfunc.h:50-146, DFAlign.cu:914-948). */
 int mc_pair_list(int nframe, int fod, int *pairs, int cap);
 int mc_lsq(const int *pairs, int npairs, int nframe, const double *b, double *s);
 int mc_threshold(const int *pairs, int npairs, int nframe, const double *ki, double kithresh,

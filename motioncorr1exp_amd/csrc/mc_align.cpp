@@ -280,7 +280,7 @@ static int movie_geometry(const mc_movie_desc* m, const mc_params* p, Geom* g) {
 }
 
 // ========================================================= drift model
-// mfunc.h:202-236
+// mfunc.h:50-84
 extern "C" int mc_pair_list(int nframe, int fod, int* pairs, int cap) {
   int dist = nframe / 2 - fod;
   if (dist <= 0) dist = 1;
@@ -298,7 +298,7 @@ extern "C" int mc_pair_list(int nframe, int fod, int* pairs, int cap) {
 }
 
 // Least squares of the pair design matrix (A[p][k] = 1 for i <= k < j,
-// mfunc.h:228-233) by the FP64 normal equations A^T A s = A^T b, Cholesky.
+// mfunc.h:76-81) by the FP64 normal equations A^T A s = A^T b, Cholesky.
 extern "C" int mc_lsq(const int* pairs, int np, int nframe, const double* b, double* s) {
   const int K = nframe - 1;
   if (K <= 0 || np < K) return set_err(MC_ERR_ARG, "least squares needs >= %d rows", K);
@@ -387,7 +387,7 @@ static bool connected(const int* pairs, const int* sel, int nsel, int nframe) {
   return comps == 1;
 }
 
-// OD_Threshold (mfunc.h:253-298)
+// OD_Threshold (mfunc.h:101-146)
 extern "C" int mc_threshold(const int* pairs, int np, int nframe, const double* ki,
                             double kithresh, int* good) {
   const int K = nframe - 1;

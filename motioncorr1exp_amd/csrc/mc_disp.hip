@@ -6,8 +6,8 @@
 // (cufunc.cu:147-161 row pattern), an inverse C2R and an 8-bit mapping; the
 // log-modulus images (DFAlign.cu:842-848, 908-911, 1270-1278) take
 // log(1 + |X| scale) (cufunc.cu:178-188), lay it out as the full centred
-// n x n image (func.cpp:413-447), and bin that image by a forward R2C, the
-// same crop, a C2R and / 400^2 (func.cpp:487-523).  The 8-bit mapping clips
+// n x n image (func.cpp:459-489), and bin that image by a forward R2C, the
+// same crop, a C2R and / 400^2 (func.cpp:524-555).  The 8-bit mapping clips
 // the histogram's 0.5% tails (func.cpp:372-443).
 //
 // Here the n x n log-modulus image is written by a tiled transpose from the

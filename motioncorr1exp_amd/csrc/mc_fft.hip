@@ -3,7 +3,7 @@
 // Spectrum layout in HBM ("column-major half spectrum"): S[kx][ky], kx in
 // [0, n/2], ky in [0, n), complex float, ky contiguous.  It holds the same
 // values as the cuFFT in-place R2C result the reference keeps on the host
-// (reference cufunc.cu:272-276 called at DFAlign.cu:772,797), transposed so
+// (reference cufunc.cu:223-227 called at DFAlign.cu:772,797), transposed so
 // that every consumer of the alignment path (pair correlation, phase shift
 // and sum) streams contiguous ky vectors.
 //
@@ -17,7 +17,7 @@
 //  k_fill     : per-frame mean of the data region (pad geometry only).
 //  k_c2c_cols : one column kx per workgroup: adds fill * (spectrum of the
 //               fill mask), transforms along y in LDS, keeps the rows of
-//               the Fourier crop (reference cufunc.cu:211-225) and writes
+//               the Fourier crop (reference cufunc.cu:162-176) and writes
 //               S[kx][ky] contiguously.
 #include <hip/hip_runtime.h>
 

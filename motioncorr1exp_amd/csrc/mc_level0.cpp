@@ -1,5 +1,5 @@
 // mc_level0.cpp -- entry points of the cufunc.h seam (reference
-// src-1125/cufunc.h:16-49, cufunc.cu:79-546) on the padded in-place
+// src-1125/cufunc.h:16-49, cufunc.cu:30-497) on the padded in-place
 // spectrum layout.  All device work runs on the null stream of the current
 // device and is synchronous with respect to the host where the reference
 // calls were (cudaMemcpy), asynchronous where they were kernel launches.
@@ -108,7 +108,7 @@ extern "C" int mc_mem_d2d(void* dst, const void* src, size_t bytes) {
   return MC_OK;
 }
 
-// cufunc.cu:196-225: rows [0, dst_n/2) and the last dst_n/2 rows.
+// cufunc.cu:147-176: rows [0, dst_n/2) and the last dst_n/2 rows.
 static int bin_copy(float* dst, const float* src, int dst_n, int src_n, hipMemcpyKind kind) {
   if (dst_n < 2 || src_n < dst_n) {
     return mc_internal_set_error(MC_ERR_ARG, "bin sizes");
@@ -216,7 +216,7 @@ extern "C" int mc_fft_log_modulus(float* dmod, const float* dfft, int n, float s
   return MC_OK;
 }
 
-// MkPosList(MASK*, nsam, bfactor) (cufunc.cu:358-390), host table.
+// MkPosList(MASK*, nsam, bfactor) (cufunc.cu:309-341), host table.
 extern "C" int mc_make_pos_list(mc_mask* list, int n, float bfactor) {
   if (!list || n < 2) return mc_internal_set_error(MC_ERR_ARG, "bad pos list arguments");
   const int hb = n / 2 + 1, h = n / 2;
@@ -237,7 +237,7 @@ extern "C" int mc_make_pos_list(mc_mask* list, int n, float bfactor) {
 
 static_assert(sizeof(mc_mask) == sizeof(mc::MaskElem), "MASK layout");
 
-// GPUShiftCC / GPUShift (cufunc.cu:411-443): shx = sx*2.0*PI/nsam in float
+// GPUShiftCC / GPUShift (cufunc.cu:362-394): shx = sx*2.0*PI/nsam in float
 extern "C" int mc_shift_cc(float* dfft, const float* dsum, const mc_mask* dposlist, float sx,
                            float sy, int n) {
   const float shx = (float)(sx * 2.0 * 3.141592653589793 / n);
@@ -257,7 +257,7 @@ extern "C" int mc_shift(float* dfft, const mc_mask* dposlist, float sx, float sy
   return MC_OK;
 }
 
-// FindShift (cufunc.cu:445-546): box read-out to the host buffer plus the
+// FindShift (cufunc.cu:396-497): box read-out to the host buffer plus the
 // FP64 peak search on the device.
 extern "C" int mc_find_shift(const float* dsrc, int n, float* hboxmap, int box, float* sx,
                              float* sy, float* cc, int wnoise) {

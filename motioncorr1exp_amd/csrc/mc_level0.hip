@@ -37,7 +37,7 @@ hipError_t launch_padded_to_spec(const float2* padded, int n, float2* spec, hipS
   return transpose(padded, n, n / 2 + 1, spec, st);
 }
 
-// cuShiftCC (cufunc.cu:392-410): FP32 phase, exactly as the reference.
+// cuShiftCC (cufunc.cu:343-361): FP32 phase, exactly as the reference.
 __global__ void k_mask_shift_cc(float2* dfft, const float2* dsum, const MaskElem* mask,
                                 float shx, float shy, int size) {
   const int id = blockIdx.x * 256 + threadIdx.x;
@@ -53,7 +53,7 @@ __global__ void k_mask_shift_cc(float2* dfft, const float2* dsum, const MaskElem
   dfft[id] = v;
 }
 
-// cuShift (cufunc.cu:421-434)
+// cuShift (cufunc.cu:372-385)
 __global__ void k_mask_shift(float2* dfft, const MaskElem* mask, float shx, float shy,
                              int size) {
   const int id = blockIdx.x * 256 + threadIdx.x;
@@ -63,7 +63,7 @@ __global__ void k_mask_shift(float2* dfft, const MaskElem* mask, float shx, floa
   dfft[id] = cmul(dfft[id], make_float2(cosf(sh), sinf(sh)));
 }
 
-// cuFFTLogModulus (cufunc.cu:227-232)
+// cuFFTLogModulus (cufunc.cu:178-183)
 __global__ void k_log_modulus(float* dmod, const float2* dfft, float scale, int size) {
   const int id = blockIdx.x * 256 + threadIdx.x;
   if (id >= size) return;
@@ -106,3 +106,4 @@ hipError_t launch_dc_gather(const float2* spectra, size_t stride, int nframes, d
 }
 
 }  // namespace mc
+

@@ -1,7 +1,7 @@
 // mc_peak.hip -- integer peak and 1/32-pixel Fourier-interpolated peak of
 // each box map, in FP64.
 //
-// Reference FindShift (cufunc.cu:445-546):
+// Reference FindShift (cufunc.cu:396-497):
 //  1. first maximum of the box in row-major order, excluding
 //     |c - box/2| <= w && |r - box/2| <= w (w = -nps - 1 ... default -1);
 //  2. 16 x 16 window at o = clamp(peak - 8, 0, box - 16), real FFT
@@ -37,7 +37,7 @@ MC_DEV ValIdx better(ValIdx a, ValIdx b) {
 
 // Result of pair p from the first strict maximum (v, i) over the map points
 // other than U[0]: the sequential search starts from U[0] with a strict '>'
-// (cufunc.cu:509-525); shift = o + p/32 - box/2, cc = U/(n^2 256).
+// (cufunc.cu:460-476); shift = o + p/32 - box/2, cc = U/(n^2 256).
 MC_DEV void peak_result(int p, double v, int i, double u0, int offx, int offy, int box, int n,
                         double* __restrict__ out) {
   double bv = u0;
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(NT) void k_find_peaks(const float* __restrict__ box
     sincospi((double)i / 256.0, &s, &c);
     t512[i] = make_double2(c, s);
   }
-  // 1. integer peak (cufunc.cu:458-473)
+  // 1. integer peak (cufunc.cu:409-424)
   ValIdx best = {-1e9, 0x7fffffff};
   for (int i = tid; i < box * box; i += NT) {
     const int r = i / box, c = i % box;
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(NT) void k_find_peaks(const float* __restrict__ box
       ix = si[0] % box;
       iy = si[0] / box;
     }
-    int offx = ix - 8, offy = iy - 8;  // cufunc.cu:479-484
+    int offx = ix - 8, offy = iy - 8;  // cufunc.cu:430-435
     if (offx + 16 > box) offx = box - 16;
     else if (offx < 0) offx = 0;
     if (offy + 16 > box) offy = box - 16;

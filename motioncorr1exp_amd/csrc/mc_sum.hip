@@ -1,7 +1,7 @@
 // mc_sum.hip -- Fourier phase-shift-and-sum of a stack of spectra.
 //
-// Reference: cuShift (cufunc.cu:421-443) multiplies a frame spectrum by
-// e^{i (shx x + shy y')}, shx = sx 2 pi/n, and cuAdd (cufunc.cu:292-301)
+// Reference: cuShift (cufunc.cu:372-394) multiplies a frame spectrum by
+// e^{i (shx x + shy y')}, shx = sx 2 pi/n, and cuAdd (cufunc.cu:243-252)
 // accumulates it into the sum (iteration sums DFAlign.cu:974-989, final sum
 // DFAlign.cu:1143-1267).  One pass here reads every frame once and writes
 // the sum once: the F shifted spectra are never materialised.  The phase
@@ -414,7 +414,7 @@ hipError_t launch_sharpness(const float2* spectra, size_t spec_stride, int nfram
   return hipGetLastError();
 }
 
-// ------------------------------------------- elementwise (cufunc.cu:292-312)
+// ------------------------------------------- elementwise (cufunc.cu:243-263)
 __global__ void k_add(float* __restrict__ dst, const float* __restrict__ src, size_t n) {
   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (i < n) dst[i] += src[i];

@@ -159,7 +159,7 @@ int or_fft_c2c(R *data, int n, int sign) {
   return 0;
 }
 
-/* cufunc.cu:272-276 (cufftExecR2C in place, padded pitch n+2) */
+/* cufunc.cu:223-227 (cufftExecR2C in place, padded pitch n+2) */
 int or_r2c2d(R *buf, int n) {
   if (n < 2 || (n & 1)) return -1;
   const int h = n / 2 + 1;
@@ -193,7 +193,7 @@ int or_r2c2d(R *buf, int n) {
   return 0;
 }
 
-/* cufunc.cu:278-282 (cufftExecC2R in place); DC/Nyquist imaginary parts of
+/* cufunc.cu:229-233 (cufftExecC2R in place); DC/Nyquist imaginary parts of
  * each row are ignored after the column pass (see header). */
 int or_c2r2d(R *buf, int n) {
   if (n < 2 || (n & 1)) return -1;
@@ -294,7 +294,7 @@ int or_pad2fft(const float *in, int nx, int ny, R *out, int offsetx,
   return npad;
 }
 
-/* cufunc.cu:211-225: rows 0..n/2-1 and the last n/2 rows, n+2 floats each */
+/* cufunc.cu:162-176: rows 0..n/2-1 and the last n/2 rows, n+2 floats each */
 int or_bin_crop(R *dst, const R *src, int dst_n, int src_n) {
   const size_t rowb = sizeof(R) * (size_t)(dst_n + 2);
   for (int i = 0; i < dst_n / 2; i++) {
@@ -308,7 +308,7 @@ int or_bin_crop(R *dst, const R *src, int dst_n, int src_n) {
 /* ======================================================================
  * Spectral kernels
  * ==================================================================== */
-/* cufunc.cu:358-390.  Element (kx, ky) of the n x (n/2+1) spectrum with
+/* cufunc.cu:309-341.  Element (kx, ky) of the n x (n/2+1) spectrum with
  * y' = ky (ky < n/2) or ky - n; z = (-1)^{x+y'} exp(-B (x^2+y'^2)/(2 n^2)).
  * The FP32 build evaluates exactly as the reference does (float m, expf). */
 R or_weight(int n, float bfactor, int kx, int ky) {
@@ -323,7 +323,7 @@ R or_weight(int n, float bfactor, int kx, int ky) {
   return ((kx + yy) % 2 == 0) ? (R)z : -(R)z;
 }
 
-/* cufunc.cu:392-419, sx = sy = 0 (DFAlign.cu:892-894,1004-1006) */
+/* cufunc.cu:343-370, sx = sy = 0 (DFAlign.cu:892-894,1004-1006) */
 int or_cross_power(R *dfft, const R *dsum, int n, float bfactor) {
   const int h = n / 2 + 1;
 #if !defined(MC_ORACLE_F32)
@@ -352,7 +352,7 @@ int or_cross_power(R *dfft, const R *dsum, int n, float bfactor) {
   return 0;
 }
 
-/* cufunc.cu:421-443 */
+/* cufunc.cu:372-394 */
 int or_phase_shift(R *dfft, int n, double sx, double sy) {
   const int h = n / 2 + 1;
 #if defined(MC_ORACLE_F32)
@@ -410,12 +410,12 @@ static void cosmask(cpx *pf, int ns) {
     }
 }
 
-/* cufunc.cu:458-545 applied to an extracted box map */
+/* cufunc.cu:409-496 applied to an extracted box map */
 double or_find_shift_box(const R *boxmap, int n, int box, double *psx,
                          double *psy, int wnoise) {
   double best = -1e9;
   int sx = 0, sy = 0;
-  /* cufunc.cu:460-473: first maximum in row-major order, noise excluded */
+  /* cufunc.cu:411-424: first maximum in row-major order, noise excluded */
   for (int j = 0; j < box; j++)
     for (int i = 0; i < box; i++) {
       if (abs(i - box / 2) <= wnoise && abs(j - box / 2) <= wnoise) continue;
@@ -425,7 +425,7 @@ double or_find_shift_box(const R *boxmap, int n, int box, double *psx,
         sy = j;
       }
     }
-  /* cufunc.cu:476-486 */
+  /* cufunc.cu:427-437 */
   const int sub = 16;
   int offx = sx - sub / 2, offy = sy - sub / 2;
   if (offx + sub > box) offx = box - sub;
@@ -440,7 +440,7 @@ double or_find_shift_box(const R *boxmap, int n, int box, double *psx,
   }
   or_r2c2d(subm, sub);
   cosmask((cpx *)subm, sub);
-  /* cufunc.cu:490-504: zero-pad to 512 x 512 and inverse */
+  /* cufunc.cu:441-455: zero-pad to 512 x 512 and inverse */
   const int scale = 32, pad = sub * scale;
   const int wns = scale * wnoise;
   R *pm = (R *)calloc((size_t)(pad + 2) * pad, sizeof(R));
@@ -450,7 +450,7 @@ double or_find_shift_box(const R *boxmap, int n, int box, double *psx,
            sizeof(R) * 18);
   }
   or_c2r2d(pm, pad);
-  /* cufunc.cu:505-525 */
+  /* cufunc.cu:456-476 */
   const int ox = (box / 2 - offx) * scale, oy = (box / 2 - offy) * scale;
   int sxp = 0, syp = 0;
   R bestp = pm[0];
@@ -466,13 +466,13 @@ double or_find_shift_box(const R *boxmap, int n, int box, double *psx,
     }
   }
   free(pm);
-  /* cufunc.cu:527-545 */
+  /* cufunc.cu:478-496 */
   *psx = offx + sxp / (double)scale - box / 2;
   *psy = offy + syp / (double)scale - box / 2;
   return (double)bestp / n / n / sub / sub;
 }
 
-/* cufunc.cu:445-546 */
+/* cufunc.cu:396-497 */
 double or_find_shift(const R *cmap, int n, R *boxmap, int box, double *psx,
                      double *psy, int wnoise) {
   const int ori = (n - box) / 2;
@@ -525,7 +525,7 @@ int or_pruned_box(const R *X, int n, int box, R *boxmap) {
 /* ======================================================================
  * Drift model (mfunc.h)
  * ==================================================================== */
-/* mfunc.h:202-236 */
+/* mfunc.h:50-84 */
 int or_pair_list(int nframe, int fod, int *pairs, int cap) {
   int dist = nframe / 2 - fod;
   if (dist <= 0) dist = 1;
@@ -543,7 +543,7 @@ int or_pair_list(int nframe, int fod, int *pairs, int cap) {
 }
 
 /* Least squares for the pair design matrix A[p][k] = 1 for i <= k < j
- * (mfunc.h:228-233), solved by Householder QR in FP64 (replaces SP++
+ * (mfunc.h:76-81), solved by Householder QR in FP64 (replaces SP++
  * lsSolver at DFAlign.cu:915,936). */
 int or_lsq(const int *pairs, int np, int nframe, const double *b, double *s) {
   const int K = nframe - 1;
@@ -621,7 +621,7 @@ static int uf_find(int *par, int a) {
 
 /* Rank of the selected design rows == nframe-1 iff the pair graph on the
  * frames is connected (row (i,j) = e_j - e_i in position coordinates).
- * Replaces CSVD::rank (mfunc.h:291-292). */
+ * Replaces CSVD::rank (mfunc.h:139-140). */
 static int rows_full_rank(const int *pairs, const int *sel, int nsel,
                           int nframe) {
   int *par = (int *)malloc(sizeof(int) * (size_t)nframe);
@@ -643,14 +643,14 @@ static int cmp_int(const void *a, const void *b) {
   return (x > y) - (x < y);
 }
 
-/* mfunc.h:238-298 */
+/* mfunc.h:86-146 */
 int or_threshold(const int *pairs, int np, int nframe, const double *ki,
                  double kithresh, int *good) {
   const int K = nframe - 1;
   int *order = (int *)malloc(sizeof(int) * (size_t)np);
   double *tki = (double *)malloc(sizeof(double) * (size_t)np);
   memcpy(tki, ki, sizeof(double) * (size_t)np);
-  /* mfunc.h:263-268: repeated first-minimum selection */
+  /* mfunc.h:111-116: repeated first-minimum selection */
   for (int i = 0; i < np; i++) {
     int pos = 0;
     double m = tki[0];
@@ -662,7 +662,7 @@ int or_threshold(const int *pairs, int np, int nframe, const double *ki,
     order[i] = pos;
     tki[pos] = 1e20;
   }
-  /* mfunc.h:271-275 */
+  /* mfunc.h:119-123 */
   int i;
   for (i = np - 1; i >= K; i--)
     if (ki[order[i]] < kithresh) break;
@@ -1056,7 +1056,7 @@ int or_fsc(const R *a, const R *b, int n, double dx, double dy, double *out) {
   return 0;
 }
 
-/* cufunc.cu:178-188 and func.cpp:413-447 */
+/* cufunc.cu:178-188 and func.cpp:459-489 */
 int or_logmod_image(const R *spec, int n, float scale, R *img) {
   if (n < 4 || (n & 1)) return -1;
   const int hn = n / 2;
@@ -1079,7 +1079,7 @@ int or_logmod_image(const R *spec, int n, float scale, R *img) {
   return 0;
 }
 
-/* func.cpp:487-523 up to buf2DispShort */
+/* func.cpp:524-555 up to buf2DispShort */
 int or_fft_display(const R *spec, int n, float scale, int d, R *out) {
   if (n < d || d < 2 || (d & 1)) return -1;
   R *buf = (R *)calloc((size_t)n * (n + 2), sizeof(R));

@@ -14,11 +14,11 @@
  *
  * Reference files followed (paths relative to the reference tree):
  *   src-1125/DFAlign.cu:456-1355   control flow, clamps, quirks
- *   src-1125/cufunc.cu:211-225     Fourier-crop binning
- *   src-1125/cufunc.cu:358-443     weight table, cross-power, phase shift
- *   src-1125/cufunc.cu:445-546     box read-out, integer peak, sub-pixel peak
+ *   src-1125/cufunc.cu:162-176     Fourier-crop binning
+ *   src-1125/cufunc.cu:309-394     weight table, cross-power, phase shift
+ *   src-1125/cufunc.cu:396-497     box read-out, integer peak, sub-pixel peak
  *   src-1125/func.cpp:80-103,157-173,193-201,348-370
- *   src-1125/mfunc.h:202-298       pair list, design matrix, outlier rejection
+ *   src-1125/mfunc.h:50-146       pair list, design matrix, outlier rejection
  */
 #ifndef MC_ORACLE_H
 #define MC_ORACLE_H
@@ -38,13 +38,13 @@ typedef MC_REAL or_real;
 int or_fft_c2c(or_real *data, int n, int sign);
 /* 2-D real-to-complex transform of an n x n image held in the in-place padded
  * layout (n rows of n+2 reals).  Result: n rows x (n/2+1) complex.
- * Follows cufftExecR2C in place (cufunc.cu:272-276). */
+ * Follows cufftExecR2C in place (cufunc.cu:223-227). */
 int or_r2c2d(or_real *buf, int n);
 /* 2-D complex-to-real inverse of the same layout, unnormalised.  Column
  * transforms are complex; the row transform takes the real part of
  * sum_{kx=0}^{n/2} a_kx Y[kx] e^{+2 pi i kx c/n} with a_0 = a_{n/2} = 1 and
  * a_kx = 2 otherwise (imaginary parts of the DC and Nyquist bins of each row
- * are ignored).  Follows cufftExecC2R in place (cufunc.cu:278-282). */
+ * are ignored).  Follows cufftExecC2R in place (cufunc.cu:229-233). */
 int or_c2r2d(or_real *buf, int n);
 
 /* ---- geometry ---------------------------------------------------------- */
@@ -61,19 +61,19 @@ int or_crop2fft(const float *in, int nx, int ny, or_real *out, int offsetx,
  * rest of the canvas holds the mean of the region.  Pitch npad+2. */
 int or_pad2fft(const float *in, int nx, int ny, or_real *out, int offsetx,
                int offsety, int npad);
-/* cufunc.cu:211-225: Fourier-crop a spectrum of size src_n to dst_n. */
+/* cufunc.cu:162-176: Fourier-crop a spectrum of size src_n to dst_n. */
 int or_bin_crop(or_real *dst, const or_real *src, int dst_n, int src_n);
 
 /* ---- spectral kernels -------------------------------------------------- */
-/* cufunc.cu:358-390: weight z(x,y') of the MASK table for element id of the
+/* cufunc.cu:309-341: weight z(x,y') of the MASK table for element id of the
  * n x (n/2+1) spectrum (row-major, row pitch n/2+1 complex). */
 or_real or_weight(int n, float bfactor, int kx, int ky);
-/* cufunc.cu:392-419 with sx = sy = 0: dfft <- dsum * conj(dfft) * z / sizec */
+/* cufunc.cu:343-370 with sx = sy = 0: dfft <- dsum * conj(dfft) * z / sizec */
 int or_cross_power(or_real *dfft, const or_real *dsum, int n, float bfactor);
-/* cufunc.cu:421-443: F <- F * exp(i (2 pi/n)(sx*x + sy*y')) */
+/* cufunc.cu:372-394: F <- F * exp(i (2 pi/n)(sx*x + sy*y')) */
 int or_phase_shift(or_real *dfft, int n, double sx, double sy);
 
-/* cufunc.cu:445-546: box read-out of the C2R output (pitch n+2), integer
+/* cufunc.cu:396-497: box read-out of the C2R output (pitch n+2), integer
  * peak, 16x16 -> 512x512 Fourier interpolation, 1/32-pixel peak.
  * boxmap (box*box) receives the box.  Returns cc. */
 double or_find_shift(const or_real *cmap, int n, or_real *boxmap, int box,
@@ -85,11 +85,11 @@ double or_find_shift_box(const or_real *boxmap, int n, int box, double *sx,
 /* Pruned inverse: box x box window of C2R(X) around the map centre, where X
  * is the n x (n/2+1) cross-power spectrum; element [r][c] equals
  * C2R(X)[(n-box)/2 + r][(n-box)/2 + c].  Exact-arithmetic equal to the full
- * inverse followed by the read-out of cufunc.cu:453-456. */
+ * inverse followed by the read-out of cufunc.cu:404-407. */
 int or_pruned_box(const or_real *X, int n, int box, or_real *boxmap);
 
 /* ---- drift model (mfunc.h) --------------------------------------------- */
-/* mfunc.h:202-236.  Returns the pair count; fills pairs (2*count ints). */
+/* mfunc.h:50-84.  Returns the pair count; fills pairs (2*count ints). */
 int or_pair_list(int nframe, int fod, int *pairs, int cap);
 /* Least-squares A s = b for the 0/1 design matrix of the pair list (FP64,
  * Householder QR).  b, s: interleaved (x,y).  Returns 0 on success. */
@@ -98,7 +98,7 @@ int or_lsq(const int *pairs, int npairs, int nframe, const double *b,
 /* ki_p = |A s - b|_p  (DFAlign.cu:916) */
 void or_residual(const int *pairs, int npairs, const double *s,
                  const double *b, double *ki);
-/* mfunc.h:253-298: outlier rejection; returns the number of kept rows and
+/* mfunc.h:101-146: outlier rejection; returns the number of kept rows and
  * writes their indices (ascending) to good. */
 int or_threshold(const int *pairs, int npairs, int nframe, const double *ki,
                  double kithresh, int *good);
@@ -181,9 +181,9 @@ int or_fsc(const or_real *a, const or_real *b, int n, double dx, double dy,
 #define OR_DISPDIM 400 /* DFAlign.h:13 */
 /* Log-modulus log(1 + |X| scale) of the half spectrum spec (padded layout;
  * cufunc.cu:178-188) laid out as the full n x n image (pitch n) with DC at
- * row n/2, column n/2 (func.cpp:413-447 FFTModulusToDispBuf). */
+ * row n/2, column n/2 (func.cpp:459-489 FFTModulusToDispBuf). */
 int or_logmod_image(const or_real *spec, int n, float scale, or_real *img);
-/* func.cpp:487-523 (BinFFTDispBufToChar up to the display mapping): R2C of
+/* func.cpp:524-555 (BinFFTDispBufToChar up to the display mapping): R2C of
  * the log-modulus image, Fourier crop to d x d (the row pattern of
  * cufunc.cu:147-161), C2R, / d^2.  out: d x d.  n >= d. */
 int or_fft_display(const or_real *spec, int n, float scale, int d, or_real *out);
@@ -191,7 +191,7 @@ int or_fft_display(const or_real *spec, int n, float scale, int d, or_real *out)
  * crop of the sum spectrum to d x d, C2R, no scaling.  n >= d. */
 int or_sum_display(const or_real *spec, int n, int d, or_real *out);
 /* func.cpp:372-411 histMinMax (400 bins, threshold 0.005) and
- * func.cpp:429-443 buf2DispShort, in the reference's float arithmetic.
+ * func.cpp:441-455 buf2DispShort, in the reference's float arithmetic.
  * Returns 1 (out untouched) when the clipped range is empty, else 0. */
 int or_disp_short(const float *buf, int size, short *out);
 

@@ -1,7 +1,7 @@
 """Writes tests/golden/fft_golden.npz: FP64 numpy (pocketfft) 2-D real FFT
 vectors used to pin the oracle's transforms.  Inputs are seeded random
 images of sizes 16, 26, 32 and 64 plus the 26 x 26 ramp h[y][x] = x + 1 of
-the reference's own FFT self-check (src-1125/cufunc.cu:572-585).
+the reference's own FFT self-check (src-1125/cufunc.cu:523-582).
 
     python tests/golden/make_golden.py
 """

@@ -225,7 +225,7 @@ def test_find_shift_on_grid_and_subpixel(o):
 
 def test_find_shift_noise_exclusion(o):
     """wNoise excludes the central (2w+1)^2 window: a stronger centre peak is
-    skipped and the off-centre one found (cufunc.cu:465,515)."""
+    skipped and the off-centre one found (cufunc.cu:416,466)."""
     box, n = 96, 512
     y, x = np.mgrid[0:box, 0:box]
     m = 2.0 * np.exp(-((x - 48) ** 2 + (y - 48) ** 2) / 4.0) + np.exp(-((x - 60) ** 2 + (y - 41) ** 2) / 4.0)
@@ -233,6 +233,186 @@ def test_find_shift_noise_exclusion(o):
     assert (sx, sy) == (0, 0)
     sx, sy, _ = o.find_shift_box(m, n, wnoise=2)
     assert abs(sx - 12) < 0.1 and abs(sy + 7) < 0.1
+
+
+def _np_find_shift(m, n, wnoise=-1):
+    """FindShift (cufunc.cu:396-497) restated in numpy, FP64, independently of
+    the oracle's FFT: the 512 x 512 map is evaluated as the matrix product
+    Re(Ey . (mask * W * rfft2(window)) . Ex).  Returns (sx, sy, cc, U, (ox, oy))."""
+    box = m.shape[0]
+    # integer peak: first maximum in row-major order, noise window skipped (:409-424)
+    jj, ii = np.mgrid[0:box, 0:box]
+    skip = (np.abs(ii - box // 2) <= wnoise) & (np.abs(jj - box // 2) <= wnoise)
+    v = np.where(skip, -np.inf, m).ravel()
+    sx = sy = 0
+    if v.max() > -1e9:
+        k = int(np.argmax(v))  # first occurrence of the maximum
+        sy, sx = divmod(k, box)
+    # 16 x 16 window (:428-437); clamps applied as the reference orders them
+    offx, offy = sx - 8, sy - 8
+    offx = box - 16 if offx + 16 > box else max(offx, 0)
+    offy = box - 16 if offy + 16 > box else max(offy, 0)
+    G = np.fft.rfft2(m[offy:offy + 16, offx:offx + 16])  # (16, 9)
+    # cosmask2d (func.cpp:348-370): r = int(sqrt(kx^2 + y'^2)), y' = ky or ky - 16
+    ky = np.arange(16)[:, None]
+    yp = np.where(ky < 8, ky, ky - 16)
+    kx = np.arange(9)[None, :]
+    r = np.floor(np.sqrt(kx * kx + yp * yp))
+    G = G * np.where(r < 8, np.cos(r * np.pi / 8) / 2 + 0.5, 0.0)
+    # zero-pad to 512 (:446-453): rows 0..7 -> 0..7, rows 8..15 -> 504..511,
+    # then the unnormalised C2R; column weights 1 (DC) and 2 (kx = 1..8)
+    q = np.arange(512)
+    Ey = np.exp(2j * np.pi * np.outer(q, yp[:, 0]) / 512)  # (512, 16)
+    Ex = np.exp(2j * np.pi * np.outer(np.arange(9), q) / 512)  # (9, 512)
+    a = np.where(np.arange(9) == 0, 1.0, 2.0)
+    U = (Ey @ (G * a[None, :]) @ Ex).real
+    # first strict maximum from U[0], scaled noise window skipped (:457-476)
+    ox, oy = (box // 2 - offx) * 32, (box // 2 - offy) * 32
+    qq, pp = np.mgrid[0:512, 0:512]
+    skip = (np.abs(pp - ox) <= 32 * wnoise) & (np.abs(qq - oy) <= 32 * wnoise)
+    u = np.where(skip, -np.inf, U).ravel()
+    best, sxp, syp = U[0, 0], 0, 0
+    k = int(np.argmax(u))
+    if u[k] > best:
+        best = u[k]
+        syp, sxp = divmod(k, 512)
+    return (offx + sxp / 32 - box // 2, offy + syp / 32 - box // 2,
+            best / n / n / 16 / 16, U, (sxp, syp))
+
+
+def _top2_margin(U, pick):
+    """Relative gap between the picked maximum and the best other point."""
+    u = U.copy()
+    u[pick[1], pick[0]] = -np.inf
+    return (U[pick[1], pick[0]] - u.max()) / abs(U[pick[1], pick[0]])
+
+
+def _check_same(o, m, n, wnoise=-1):
+    sx, sy, cc, U, pick = _np_find_shift(m, n, wnoise)
+    osx, osy, occ = o.find_shift_box(m, n, wnoise)
+    assert (osx, osy) == (sx, sy), ((osx, osy), (sx, sy))
+    assert abs(occ - cc) <= 1e-12 * abs(cc) + 1e-300, (occ, cc)
+    return U, pick
+
+
+def test_find_shift_matches_numpy_restatement_random_maps(o):
+    """Random box maps (noise only, and Gaussian peaks anywhere, including the
+    border clamps of the 16 x 16 window) at boxes 32-128: identical shifts
+    and cc within 1e-12."""
+    rng = np.random.default_rng(2024)
+    worst = 1.0
+    for t in range(60):
+        box = int(rng.choice([32, 48, 64, 96, 128]))
+        n = int(rng.choice([256, 512, 4096]))
+        y, x = np.mgrid[0:box, 0:box]
+        m = rng.normal(0, 1.0, (box, box))
+        if t % 3:
+            x0, y0 = rng.uniform(-2, box + 1, 2)  # peaks past the edges clamp the window
+            m += rng.uniform(3, 50) * np.exp(-((x - x0) ** 2 + (y - y0) ** 2) / rng.uniform(1, 12))
+        U, pick = _check_same(o, m, n)
+        worst = min(worst, _top2_margin(U, pick))
+    print("smallest top-2 margin of the random maps: %.3g" % worst)
+
+
+@pytest.mark.parametrize("wnoise", [0, 1, 3])
+def test_find_shift_matches_numpy_restatement_noise_window(o, wnoise):
+    """-nps > 0: the integer and the 1/32 searches both skip the (scaled)
+    central window, also when the window origin is clamped."""
+    rng = np.random.default_rng(77 + wnoise)
+    for t in range(12):
+        box = int(rng.choice([48, 96]))
+        y, x = np.mgrid[0:box, 0:box]
+        c = box // 2
+        m = 4.0 * np.exp(-((x - c) ** 2 + (y - c) ** 2) / 3.0) + rng.normal(0, 0.05, (box, box))
+        x0, y0 = c + rng.uniform(-6, 6), c + rng.uniform(-6, 6)
+        m += np.exp(-((x - x0) ** 2 + (y - y0) ** 2) / 3.0)
+        _check_same(o, m, 512, wnoise)
+
+
+def test_find_shift_matches_numpy_restatement_near_ties(o):
+    """Maps built so that two neighbouring 1/32-grid points nearly tie: a
+    Gaussian's sub-pixel position is bisected to where the pick switches
+    between them, down to a relative top-2 gap of about 1e-9.  Oracle and
+    restatement must still pick the same point."""
+    rng = np.random.default_rng(31337)
+    box, n = 64, 1024
+    y, x = np.mgrid[0:box, 0:box]
+    done = 0
+    for t in range(8):
+        noise = rng.normal(0, 0.02, (box, box))
+        w = rng.uniform(1.5, 6)
+        y0 = 32 + rng.uniform(-3, 3)
+        g = 30 + rng.integers(-3, 3) + rng.integers(0, 32) / 32
+
+        def pick_at(x0):
+            m = np.exp(-((x - x0) ** 2 + (y - y0) ** 2) / w) + noise
+            return m, _np_find_shift(m, n)
+
+        lo, hi = g, g + 1 / 32
+        plo, phi = pick_at(lo)[1][4], pick_at(hi)[1][4]
+        if plo == phi:
+            continue
+        for _ in range(40):
+            mid = 0.5 * (lo + hi)
+            if pick_at(mid)[1][4] == plo:
+                lo = mid
+            else:
+                hi = mid
+            m, r = pick_at(mid)
+            gap = _top2_margin(r[3], r[4])
+            if gap < 1e-9:
+                break
+        if gap < 1e-12:  # below FP64 evaluation noise: either pick is right
+            continue
+        _check_same(o, m, n)
+        done += 1
+        print("near tie %d: relative top-2 gap %.2g" % (t, gap))
+    assert done >= 4
+
+
+def test_find_shift_comparison_catches_planted_mistakes(o):
+    """The comparison has power: a copy of the restatement with one planted
+    mistake (no cosine mask, window origin off by one, C2R column weight
+    dropped, inverse with the wrong sign, cc without the 16^2 factor)
+    disagrees with the oracle on at least one of these maps."""
+    rng = np.random.default_rng(5)
+    box, n = 96, 512
+    y, x = np.mgrid[0:box, 0:box]
+    maps = [8 * np.exp(-((x - x0) ** 2 + (y - y0) ** 2) / 5.0) + rng.normal(0, 0.3, (box, box))
+            for x0, y0 in rng.uniform(20, 76, (6, 2))]
+    ref = [o.find_shift_box(m, n) for m in maps]
+
+    def variant(m, mistake):
+        jj = np.argmax(m)
+        sy, sx = divmod(int(jj), box)
+        offx = min(max(sx - (7 if mistake == "origin" else 8), 0), box - 16)
+        offy = min(max(sy - 8, 0), box - 16)
+        G = np.fft.rfft2(m[offy:offy + 16, offx:offx + 16])
+        ky = np.arange(16)[:, None]
+        yp = np.where(ky < 8, ky, ky - 16)
+        r = np.floor(np.sqrt(np.arange(9)[None, :] ** 2 + yp * yp))
+        if mistake != "nomask":
+            G = G * np.where(r < 8, np.cos(r * np.pi / 8) / 2 + 0.5, 0.0)
+        sg = -1 if mistake == "sign" else 1
+        q = np.arange(512)
+        Ey = np.exp(sg * 2j * np.pi * np.outer(q, yp[:, 0]) / 512)
+        Ex = np.exp(sg * 2j * np.pi * np.outer(np.arange(9), q) / 512)
+        a = np.ones(9) if mistake == "weight" else np.where(np.arange(9) == 0, 1.0, 2.0)
+        U = (Ey @ (G * a[None, :]) @ Ex).real
+        k = int(np.argmax(U))
+        syp, sxp = divmod(k, 512)
+        cc = U.ravel()[k] / n / n / (1 if mistake == "ccscale" else 256)
+        return offx + sxp / 32 - box // 2, offy + syp / 32 - box // 2, cc
+
+    for m, (rx, ry, rc) in zip(maps, ref):  # the faithful copy agrees
+        vx, vy, vc = variant(m, None)
+        assert (vx, vy) == (rx, ry) and abs(vc - rc) <= 1e-12 * abs(rc)
+    for mistake in ("nomask", "origin", "weight", "sign", "ccscale"):
+        caught = False
+        for m, (rx, ry, rc) in zip(maps, ref):
+            vx, vy, vc = variant(m, mistake)
+            caught |= (vx, vy) != (rx, ry) or abs(vc - rc) > 1e-9 * abs(rc)
+        assert caught, mistake
 
 
 # ---------------------------------------------------------- drift model
@@ -497,7 +677,7 @@ def test_fsc_table_halves_of_a_movie(o):
 
 # ------------------------------------------------------- quick-look images
 def test_logmod_image_positions(o):
-    """func.cpp:413-447: the right half holds kx = i - n/2 of row ky = (j + n/2)
+    """func.cpp:459-489: the right half holds kx = i - n/2 of row ky = (j + n/2)
     mod n, the left half its point mirror, and row 0 copies row 1; each
     pixel is log(1 + |X| scale) (cufunc.cu:178-188)."""
     n, hn, sc = 64, 32, 1.0 / 64  # scale exact in float
@@ -531,7 +711,7 @@ def test_sum_display_is_cosine_on_the_display_grid(o):
 
 def test_fft_display_matches_numpy(o):
     """R2C of the log-modulus image, Fourier crop, C2R and / d^2 equal numpy's
-    normalised irfft2 of the cropped rfft2 (func.cpp:487-523)."""
+    normalised irfft2 of the cropped rfft2 (func.cpp:524-555)."""
     n, d = 512, 400
     rng = np.random.default_rng(50)
     spec = np.fft.rfft2(rng.poisson(3.0, (n, n)).astype(float))
