@@ -68,10 +68,10 @@ hipError_t launch_spectrum_to_image(const float2* spec, int n, float scale, floa
 //   M128_PF = the same with next-item register prefetch (fewer resident
 //   workgroups), GENERIC = the M-generic persistent kernel, M128_W4 = M = 128
 //   compiled for 4 waves per SIMD (128 VGPRs); row pass: AUTO = FP32 MFMA
-//   streaming each wave's kx range from memory (box <= 96; when pairs are
-//   few the kx range is also split over workgroups whose partial maps go to
-//   `part`, pair_rows_part_elems(npairs, n, box) floats, and are added in
-//   order), VALU = register-tiled vector ALU kernel, MFMA8 / MFMA4 = FP32
+//   streaming each wave's kx range from memory (box <= 96; from n = 2048
+//   the kx range is also split over workgroups, a count set by n alone, whose
+//   partial maps go to `part`, pair_rows_part_elems(npairs, n, box) floats,
+//   and are added in order), VALU = register-tiled vector ALU kernel, MFMA8 / MFMA4 = FP32
 //   MFMA staged through LDS with 8 / 4 waves per (pair, 32-row block).
 enum { PAIR_COLS_AUTO = 0, PAIR_COLS_M128_PF = 1, PAIR_COLS_GENERIC = 2, PAIR_COLS_M128_W4 = 3,
        PAIR_COLS_VARIANTS = 4 };
@@ -151,7 +151,7 @@ hipError_t launch_spec_to_padded(const float2* spec, int n, float2* padded, hipS
 hipError_t launch_padded_to_spec(const float2* padded, int n, float2* spec, hipStream_t st);
 
 // cufunc.h-compatible kernels on the padded layout with a MASK table
-// (x, y, z) per element (reference cufunc.cu:392-443, 227-237).
+// (x, y, z) per element (reference cufunc.cu:343-394, 178-188).
 struct MaskElem {
   int x, y;
   float z;

@@ -1,11 +1,11 @@
 // mc_pair.hip -- B-factor-weighted cross-power of frame pairs and the pruned
 // inverse that produces only the box x box window of each correlation map.
 //
-// Reference: cuShiftCC (cufunc.cu:392-410, called with sx = sy = 0 at
+// Reference: cuShiftCC (cufunc.cu:343-361, called with sx = sy = 0 at
 // DFAlign.cu:894,1006) forms dfft <- dsum * conj(dfft) * z / sizec with
-// z = (-1)^{x+y'} exp(-B (x^2+y'^2)/(2 n^2)) (MkPosList, cufunc.cu:358-390);
+// z = (-1)^{x+y'} exp(-B (x^2+y'^2)/(2 n^2)) (MkPosList, cufunc.cu:309-341);
 // a full n x n C2R follows (DFAlign.cu:897,1008) and only the central box is
-// read back (FindShift, cufunc.cu:447-456).  Here the product, the weight
+// read back (FindShift, cufunc.cu:398-407).  Here the product, the weight
 // and the inverse restricted to the box are fused:
 //
 //  k_pair_cols : one (pair, kx) per workgroup.  The product X[ky] is formed
@@ -875,7 +875,7 @@ __global__ __launch_bounds__(256) void k_pair_rows_mfma16(const float2* __restri
 
 // Weight table: wxy[kx] = (-1)^kx exp(m kx^2), kx in [0, n/2]; wxy[n/2+1+ky]
 // = (-1)^{y'} exp(m y'^2) / sizec, so z/sizec = wxy[kx] * wxy[n/2+1+ky] with
-// m = -0.5 B/n^2 as in MkPosList (cufunc.cu:358-390).
+// m = -0.5 B/n^2 as in MkPosList (cufunc.cu:309-341).
 static const float* weights(int n, float bfactor) {
   static std::mutex mu;
   static std::map<std::tuple<int, int, float>, float*> cache;
@@ -967,46 +967,21 @@ static const float* row_twiddles_stream(int n, int box) {
   return d;
 }
 
-// Compute units of the current device (queried once per device).
-static int device_cus() {
-  static std::atomic<int> cache[64];
-  int dev = 0;
-  (void)hipGetDevice(&dev);
-  if (dev < 0 || dev >= 64) dev = 0;
-  int cus = cache[dev].load(std::memory_order_relaxed);
-  if (cus <= 0) {
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        cus <= 0)
-      cus = 256;
-    cache[dev].store(cus, std::memory_order_relaxed);
-  }
-  return cus;
-}
-
 // Waves per workgroup of the streaming row pass; two workgroups share a CU.
 constexpr int ROWS_NW = 4;
 
-// K splits of the streaming row pass: the count ks in [1, 8] that minimises
-// (rounds of workgroups over 2 per CU) x (bins per wave + a fixed cost of
-// 16 bins for the operand prologue and the wave reduction).
-static int pair_rows_splits(int npairs, int n) {
-  const long long slots = 2LL * device_cus();
-  const double H = n / 2 + 1;
-  int best = 1;
-  double bc = 1e300;
-  for (int ks = 1; ks <= 8; ks++) {
-    const long long rounds = ((long long)npairs * ks + slots - 1) / slots;
-    const double cost = (double)rounds * (H / (ks * ROWS_NW) + 16.0);
-    if (cost < bc) {
-      bc = cost;
-      best = ks;
-    }
-  }
-  return best;
+// K splits of the streaming row pass: ks = clamp((n/2+1)/512, 1, 8), so that
+// a wave walks about 128 bins.  The split fixes the order in which the FP32
+// partial products of a box map are added, so it depends on the transform
+// size alone: a pair gives the same map whatever the number of pairs in the
+// launch (one movie or a batch, a full pair set or an align-to-sum round)
+// and whatever the device's CU count.
+static int pair_rows_splits(int n) {
+  return std::max(1, std::min(8, (n / 2 + 1) / 512));
 }
 
 size_t pair_rows_part_elems(int npairs, int n, int box) {
-  const int ks = pair_rows_splits(std::max(1, npairs), n);
+  const int ks = pair_rows_splits(n);
   return ks > 1 ? (size_t)npairs * ks * box * box : 0;
 }
 
@@ -1190,7 +1165,7 @@ hipError_t launch_pair_rows(const float2* tbuf, const float* tmax, int npairs, i
   if (precision == 0 && variant == PAIR_ROWS_AUTO && box <= 96) {
     const float* ef = row_twiddles_stream(n, box);
     if (!ef) return hipErrorOutOfMemory;
-    const int ks = pair_rows_splits(npairs, n);
+    const int ks = pair_rows_splits(n);
     if (ks > 1 && !part) return hipErrorInvalidValue;
     float* dst = ks > 1 ? part : boxmaps;
     const float* tf = reinterpret_cast<const float*>(tbuf);

@@ -4,7 +4,8 @@ gives it (same kernels, same per-pair arithmetic), with the align-to-sum
 rounds replayed from a captured HIP graph or launched eagerly
 (MC_NO_GRAPH=1), in one sub-batch or split by a small memory budget.  One
 movie of the batch is also checked against the FP64 oracle (shifts within
-0.01 px except 1/32-grid flips, sum within 1e-4 relative L2)."""
+0.01 px, sum within 1e-4 relative L2).  tests/test_gpu_batch_cfg4.py covers
+movies of the batch workload's sizes."""
 import os
 
 import numpy as np
@@ -60,15 +61,13 @@ def _batch(ctx, movies):
 
 
 def _same(a, b):
+    """Batch and single runs use the same kernels with the same per-pair
+    arithmetic and summation order, so every output is bit-identical."""
     for k in WANT:
-        if k == "sum_image":
-            rel = np.linalg.norm(a[k] - b[k]) / np.linalg.norm(b[k])
-            assert rel < 1e-6, (k, rel)
-        else:
-            assert np.array_equal(a[k], b[k]) or np.allclose(a[k], b[k], rtol=1e-6, atol=1e-9), k
+        assert np.array_equal(a[k], b[k]), k
     for k in ("niter", "npairs", "ngood", "n", "nframe"):
         assert a["report"][k] == b["report"][k], k
-    assert a["report"]["rmsd"] == pytest.approx(b["report"]["rmsd"], rel=1e-9, abs=1e-12)
+    assert a["report"]["rmsd"] == b["report"]["rmsd"]
 
 
 def test_batch_matches_single_movies(ctx, movies, singles):
@@ -112,8 +111,9 @@ def test_batch_movie_matches_oracle(ctx, movies):
                                                     iter=prm["iter"], full_c2r=1, nthreads=8),
                                   want_boxes=False)
     d = np.abs(r["frame_shift"] - o["frame_shift"])
-    ok = (d <= 0.01) | (np.abs(d - 1 / 32) < 1e-9) | (np.abs(d - 1 / 16) < 1e-9)
-    assert ok.all() and int((d > 0.01).sum()) <= 2
+    flips = int((d > 0.01).sum())
+    print("frame shift max error %.3g px, frames > 0.01 px: %d" % (d.max(), flips), flush=True)
+    assert flips == 0
     rel = np.linalg.norm(r["sum_image"] - o["sum_image"]) / np.linalg.norm(o["sum_image"])
     assert rel < 1e-4, rel
 

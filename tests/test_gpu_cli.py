@@ -66,11 +66,14 @@ def test_cli_mrc_float(tmp_path, movie):
     shifts = _final_shifts(log)
     ref = np.vstack([[0, 0], np.cumsum(o["final_shift"], axis=0)])
     assert shifts.shape == ref.shape
-    assert np.abs(shifts - ref).max() <= 1 / 32 + 1e-4  # printed with %8.4f; flips counted as 1/32
+    # the log prints shifts with %8.4f: 0.01 px plus half a unit of the last digit
+    err = np.abs(shifts - ref)
+    print("CLI frame shifts vs oracle: max %.3g px, > 0.01 px: %d" % (err.max(), int((err > 0.01).sum())))
+    assert err.max() <= 0.01 + 5e-5
     s = _read_mrc(tmp_path / "stack_SumCorr.mrc")[0]
     assert s.shape == (512, 512)
     rel = np.linalg.norm(s - o["sum_image"]) / np.linalg.norm(o["sum_image"])
-    assert rel < 5e-4, rel
+    assert rel < 1e-4, rel
     raw = _read_mrc(tmp_path / "stack_SumRaw.mrc")[0]
     rel = np.linalg.norm(raw - o["rawsum_image"]) / np.linalg.norm(o["rawsum_image"])
     assert rel < 1e-5, rel
@@ -91,7 +94,7 @@ def test_cli_mode0_bytes_and_gzip(tmp_path, movie):
     assert np.array_equal(a, b)
     o = Oracle("f64").align_movie(q.astype(np.float32), dict(fod=4, atm=0, nthreads=8), want_boxes=False)
     rel = np.linalg.norm(a[0] - o["sum_image"]) / np.linalg.norm(o["sum_image"])
-    assert rel < 5e-4, rel
+    assert rel < 1e-4, rel
 
 
 def test_cli_tiff_bin2_and_outputs(tmp_path, movie):
@@ -120,7 +123,7 @@ def test_cli_tiff_bin2_and_outputs(tmp_path, movie):
                                                nthreads=8), want_boxes=True)
     assert cc.shape[0] == len(o["pairs"])
     rel = np.linalg.norm(s[0] - o["sum_image"]) / np.linalg.norm(o["sum_image"])
-    assert rel < 5e-4, rel
+    assert rel < 1e-4, rel
 
 
 def test_cli_errors(tmp_path):

@@ -1,11 +1,12 @@
 """GPU tests of the pair stage's box maps over the launch shapes of the row
 pass (the C2R row transform restricted to the box, reference DFAlign.cu:897
-with FindShift's read-back, cufunc.cu:447-456): pair counts from 1 to 300
-(with fewer pairs than CUs the kx range of a pair is split over up to 8
-workgroups whose partial maps are added in order) and boxes of one, two and
-three 32-row blocks, whole and partial.  Spectra of random real images are
-set directly, so the only difference from the FP64 oracle's cross power
-(cufunc.cu:392-410) and pruned box inverse is the FP32 arithmetic."""
+with FindShift's read-back, cufunc.cu:396-424): pair counts from 1 to 300
+(at n = 2048 the kx range of every pair is split over 2 workgroups whose
+partial maps are added in order, whatever the pair count) and boxes of one,
+two and three 32-row blocks, whole and partial.  Spectra of random real
+images are set directly, so the only difference from the FP64 oracle's
+cross power (cufunc.cu:343-370) and pruned box inverse is the FP32
+arithmetic."""
 import numpy as np
 import pytest
 
@@ -53,22 +54,29 @@ def test_box_maps_match_oracle(loaded, o, npairs, box):
     assert worst < 1e-5
 
 
-def test_split_and_unsplit_maps_agree(loaded):
-    """The same pair alone (kx range split over 8 workgroups) and among 300
-    (one workgroup per pair) gives the same map up to FP32 summation order."""
+@pytest.mark.parametrize("box", [64, 96])
+def test_map_independent_of_launch_pair_count(loaded, box):
+    """The same pair alone, among 7, 40 and 300 gives a bit-identical map and
+    shift: the row pass splits a pair's kx range by n alone, so the FP32
+    summation order does not change with the launch (a single movie, a batch
+    or an align-to-sum round)."""
     ctx, _ = loaded
     pairs = _pairs(300)
-    _, _, many = ctx.pair_shifts(pairs, 150.0, 96, want_boxes=True)
-    for q in (0, 150, 299):
-        _, _, one = ctx.pair_shifts(pairs[q:q + 1], 150.0, 96, want_boxes=True)
-        assert np.abs(one[0] - many[q]).max() <= 5e-6 * np.abs(many[q]).max()
+    b_all, cc_all, many = ctx.pair_shifts(pairs, 150.0, box, want_boxes=True)
+    for cnt in (1, 7, 40):
+        for q in (0, 150, 299):
+            sub = pairs[q:q + cnt] if q + cnt <= 300 else pairs[q - cnt + 1:q + 1]
+            at = 0 if q + cnt <= 300 else cnt - 1
+            b, cc, one = ctx.pair_shifts(sub, 150.0, box, want_boxes=True)
+            assert np.array_equal(one[at], many[q]), (cnt, q)
+            assert np.array_equal(b[at], b_all[q]) and cc[at] == cc_all[q], (cnt, q)
 
 
 @pytest.mark.parametrize("npairs", [1, 40, 96, 129])
 def test_peak_search_slices_match_oracle(loaded, o, npairs):
     """With fewer pairs than half the CUs, the peak search splits the 512 map
     rows of each pair over 2 or 4 workgroups and combines their first maxima
-    (FindShift, cufunc.cu:445-546).  On 256 CUs, 1 and 40 pairs take 4
+    (FindShift, cufunc.cu:396-497).  On 256 CUs, 1 and 40 pairs take 4
     slices, 96 take 2 and 129 one.  Box maps with one Gaussian peak at a
     random sub-pixel position over weak noise: shifts identical to the FP64
     oracle's search of the same FP32 maps, cc within 1e-9."""
