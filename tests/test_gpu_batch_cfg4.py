@@ -7,8 +7,8 @@ mc_align_batch call.  Every movie must get bit for bit what mc_align_movie
 gives it alone (pair shifts, box maps, round measurements, final shifts,
 sum), and every movie is checked against the FP64 oracle: per-frame shifts
 within 0.01 px and aligned sum within 1e-4 relative L2.  The number of
-frames off by more than 0.01 px (1/32-grid flips of the peak pick, see
-below) is printed per movie and must be 0."""
+frames off by more than 0.01 px (a 1/32-grid flip of the peak pick on a
+near tie would show here) is printed per movie and must be 0."""
 import numpy as np
 import pytest
 

@@ -104,7 +104,8 @@ def test_mfma_form_box_maps_match_fp64_oracle(ctx):
     scale = np.abs(obm).reshape(len(obm), -1).max(1)[:, None, None]
     assert (np.abs(bm - obm) / scale).max() < 1e-5
     db = np.abs(r["pair_b"] - o["pair_b"])
-    assert np.all((db < 1e-9) | (np.abs(db - 1 / 32) < 1e-9))
+    print("hi/lo form vs oracle: pair components beyond 0.01 px: %d" % int((db > 0.01).sum()))
+    assert int((db > 0.01).sum()) == 0
 
 
 @pytest.mark.parametrize("n,F,box", [(1024, 8, 96), (4096, 10, 96)])
@@ -120,7 +121,9 @@ def test_mfma_form_matches_fp32_form(ctx, n, F, box):
     d = r1["frame_shift"] - r0["frame_shift"]
     assert np.sqrt(np.mean(d ** 2)) <= 0.01
     db = np.abs(r1["pair_b"] - r0["pair_b"])
-    assert np.all((db < 1e-9) | (np.abs(db - 1 / 32) < 1e-9))
+    print("n %d: hi/lo vs FP32 frame shift RMS %.3g px, pair components beyond 0.01 px: %d"
+          % (n, np.sqrt(np.mean(d ** 2)), int((db > 0.01).sum())))
+    assert int((db > 0.01).sum()) == 0
     rel = np.linalg.norm(r1["sum_image"] - r0["sum_image"]) / np.linalg.norm(r0["sum_image"])
     assert rel < 1e-4, rel
     dev.free()
@@ -137,7 +140,9 @@ def test_mfma_form_rejects_unsupported_geometry(ctx):
 # ---------------------------------------------------------------- autotuner
 def test_autotune_picks_valid_shapes_and_keeps_results(ctx, tmp_path):
     """mc_autotune times every candidate, keeps the fastest, and the tuned
-    path gives the same shifts as the default shapes (1/32-grid flips aside);
+    path gives the same shifts as the default shapes within 0.01 px (it may
+    add the box maps in another order; frames beyond 0.01 px are counted and
+    must be 0);
     the cache file round-trips through mc_tune_load on a fresh context."""
     F, n = 8, 1024
     frames, drift = make_movie(n, n, F, dose=2.0, step=0.7, seed=123)
@@ -155,7 +160,8 @@ def test_autotune_picks_valid_shapes_and_keeps_results(ctx, tmp_path):
     assert rep["ms_stage1"][cols_p][shape][[1, 2, 4, 8].index(rep["chunk_frames"])] == best
     tuned = ctx.align_movie(d, frames.shape, frames.dtype, prm, want=("frame_shift", "sum_image"))
     dd = np.abs(tuned["frame_shift"] - ref["frame_shift"])
-    assert np.all((dd < 1e-6) | (np.abs(dd - 1 / 32) < 1e-6) | (np.abs(dd - 1 / 16) < 1e-6))
+    print("tuned vs default shapes: frame shift max difference %.3g px" % dd.max())
+    assert int((dd > 0.01).sum()) == 0
     c1 = mc.Context(0)
     assert c1.tune_load(cache) == 1
     again = c1.align_movie(d, frames.shape, frames.dtype, prm, want=("frame_shift",))

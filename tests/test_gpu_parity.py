@@ -3,8 +3,9 @@ oracle on the same seeded inputs.
 
 Tolerances: FFT outputs within 2e-6 of the spectrum maximum (FP32 Stockham
 vs FP64); box maps within 1e-5 of their maximum; per-pair and per-frame
-shifts within 0.01 px except 1/32-grid flips (counted, at most a few per
-movie and each exactly 1/32 px); aligned sum within 1e-4 relative L2.
+shifts within 0.01 px, with the count of those beyond (1/32-grid flips of
+the peak pick on a near tie) printed and required to be 0; aligned sum
+within 1e-4 relative L2.
 """
 import numpy as np
 import pytest
@@ -28,9 +29,13 @@ def o64():
     return Oracle("f64")
 
 
-def _grid_flips(a, b):
+def _off(a, b, what):
+    """Count of components more than 0.01 px from the oracle (printed)."""
     d = np.abs(np.asarray(a) - np.asarray(b))
-    return int(np.sum((d > 0.01) & (np.abs(d - 1 / 32) < 1e-6)))
+    d = d[np.isfinite(d)]
+    k = int(np.sum(d > 0.01))
+    print("%s: max difference %.3g px, beyond 0.01 px: %d" % (what, d.max(), k), flush=True)
+    return k
 
 
 # ----------------------------------------------------------------- level 0
@@ -132,13 +137,10 @@ def test_boxmaps_and_pairs_match_oracle(ctx, cfg1):
     bm, obm = res["boxmaps"], o["boxmaps"]
     scale = np.abs(obm).reshape(len(obm), -1).max(1)[:, None, None]
     assert (np.abs(bm - obm) / scale).max() < 1e-5
-    db = np.abs(res["pair_b"] - o["pair_b"])
-    assert np.all((db < 1e-9) | (np.abs(db - 1 / 32) < 1e-9))
-    assert _grid_flips(res["pair_b"], o["pair_b"]) <= 2
+    assert _off(res["pair_b"], o["pair_b"], "pair shifts") == 0
     assert np.allclose(res["pair_cc"], o["pair_cc"], rtol=1e-4)
-    dfs = np.abs(res["final_shift"] - o["final_shift"])
-    assert np.all((dfs <= 0.0101) | (np.abs(dfs - 1 / 32) < 1e-9) | (np.abs(dfs - 1 / 16) < 1e-9))
-    assert np.sum(dfs > 0.0101) <= 2
+    assert _off(res["final_shift"], o["final_shift"], "final increments") == 0
+    assert _off(res["frame_shift"], o["frame_shift"], "frame shifts") == 0
     rel = np.linalg.norm(res["sum_image"] - o["sum_image"]) / np.linalg.norm(o["sum_image"])
     assert rel < 1e-4, rel
 
@@ -215,7 +217,7 @@ def test_three_times_power_of_two_sizes_match_oracle(ctx, o64, geom):
     """Transform sizes 2^k * 3, 5, 15 (radix-3 / radix-5 stages): whole-movie
     parity with the FP64 oracle, as for the power-of-two sizes: spectra
     within 5e-7 of the maximum, box maps within 1e-5, shifts within 0.01 px
-    (1/32 flips counted), sum within 1e-4."""
+    (beyond: counted, must be 0), sum within 1e-4."""
     if geom == "crop1280":
         frames, drift = make_movie(1300, 1290, 6, dose=2.0, step=0.8, seed=305)
         op = dict(bin=1.0, crop_nsam=1280, fod=1, bfactor=150.0, box=96, full_c2r=1, iter=1,
@@ -253,10 +255,7 @@ def test_three_times_power_of_two_sizes_match_oracle(ctx, o64, geom):
     bm, obm = res["boxmaps"], o["boxmaps"]
     scale = np.abs(obm).reshape(len(obm), -1).max(1)[:, None, None]
     assert (np.abs(bm - obm) / scale).max() < 1e-5
-    db = np.abs(res["pair_b"] - o["pair_b"])
-    assert np.all((db < 1e-9) | (np.abs(db - 1 / 32) < 1e-9))
-    dfs = np.abs(res["frame_shift"] - o["frame_shift"])
-    dfs = dfs[np.isfinite(dfs)]
-    assert np.all((dfs <= 0.0101) | (np.abs(dfs - 1 / 32) < 1e-9) | (np.abs(dfs - 1 / 16) < 1e-9))
+    assert _off(res["pair_b"], o["pair_b"], "pair shifts") == 0
+    assert _off(res["frame_shift"], o["frame_shift"], "frame shifts") == 0
     rel = np.linalg.norm(res["sum_image"] - o["sum_image"]) / np.linalg.norm(o["sum_image"])
     assert rel < 1e-4, rel
