@@ -353,8 +353,10 @@ int mc_sum_image(mc_context ctx, const double *frame_shift, float *image);
  * adjoint dJ/ds_f (grad: 2F doubles) on the resident spectra.  Frames with a
  * NaN x shift are left out of the sum and get a zero gradient.  With
  * grad_steps > 0, mc_align_movie maximises J over the summed frames' shifts
- * before the final sum (backtracking gradient ascent, first step
- * grad_step_px, default 0.05 px); the peak-search shifts are the start. */
+ * before the final sum (trust-region gradient ascent: first step
+ * grad_step_px, default 0.05 px, doubled after an accepted step up to 4x the
+ * first, halved after a rejected one); the peak-search shifts are the
+ * start. */
 int mc_sharpness(mc_context ctx, const double *frame_shift, float bfactor, double *J,
                  double *grad);
 

@@ -680,8 +680,9 @@ static int sharpness_eval(mc_context_s* c, int base, int F, int sum, const std::
   return MC_OK;
 }
 
-// Gradient ascent of J over the summed frames' shifts with a backtracking
-// trust region: s += step * g / max|g|; a step that lowers J is rejected and
+// Gradient ascent of J over the summed frames' shifts with a trust region:
+// s += step * g / max|g|.  A step that raises J is accepted and the next one
+// doubled (up to 4x the first step); a step that does not is rejected and
 // the step halved.
 static int refine_shifts(mc_context_s* c, int base, int F, int sum, std::vector<double>& fs,
                          float bfactor, int steps, float step_px, double* j0, double* j1,
@@ -691,7 +692,8 @@ static int refine_shifts(mc_context_s* c, int base, int F, int sum, std::vector<
   std::vector<double> g;
   MC_TRY(sharpness_eval(c, base, F, sum, fs, &J, &g));
   *j0 = J;
-  double step = step_px > 0 ? step_px : 0.05;
+  const double step0 = step_px > 0 ? step_px : 0.05;
+  double step = step0;
   int acc = 0;
   for (int t = 0; t < steps; t++) {
     double gmax = 0;
@@ -713,6 +715,7 @@ static int refine_shifts(mc_context_s* c, int base, int F, int sum, std::vector<
       g.swap(gt);
       J = Jt;
       acc++;
+      step = std::min(2.0 * step, 4.0 * step0);
     } else {
       step *= 0.5;
     }
