@@ -106,6 +106,17 @@ int mc_shift(float *dfft, const mc_mask *dposlist, float sx, float sy, int n);
  * the box; returns 0 and writes cc. */
 int mc_find_shift(const float *dsrc, int n, float *hboxmap, int box, float *sx, float *sy,
                   float *cc, int wnoise);
+/* No counterpart in the reference.  mc_find_shift for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): same arguments and results, but the box
+ * copy and the peak record live in a fixed device array of the library
+ * (MC_NOALLOC_MAX_BOX^2 floats + 256 bytes, one per GPU, held until the
+ * process ends) and the call makes no allocation.  box > MC_NOALLOC_MAX_BOX
+ * returns MC_ERR_ARG.  All calls on one GPU share the array: calls that
+ * overlap on one GPU (several host threads) race. */
+#define MC_NOALLOC_MAX_BOX 256
+int mc_find_shift_noalloc(const float *dsrc, int n, float *hboxmap, int box, float *sx,
+                          float *sy, float *cc, int wnoise);
 
 /* ======================================================================
  * Level 1: resident movie pipeline

@@ -127,6 +127,7 @@ ABI_SYMBOLS = [
     "mc_mem_d2d", "mc_mem_bin_d2d", "mc_mem_bin_d2h", "mc_fft_plan", "mc_ifft_plan",
     "mc_fft2d", "mc_ifft2d", "mc_fft_destroy", "mc_sync", "mc_add", "mc_multiply_num",
     "mc_fft_log_modulus", "mc_make_pos_list", "mc_shift_cc", "mc_shift", "mc_find_shift",
+    "mc_find_shift_noalloc",
     "mc_context_create", "mc_context_destroy", "mc_align_movie", "mc_load_frames",
     "mc_set_spectra", "mc_get_spectra", "mc_pair_shifts", "mc_find_peaks", "mc_sum_image",
     "mc_pair_list", "mc_lsq", "mc_threshold", "mc_kernel_timing", "mc_kernel_timing_groups",
@@ -169,6 +170,7 @@ def lib():
         L.mc_shift_cc.argtypes = [vp, vp, vp, f, f, i]
         L.mc_shift.argtypes = [vp, vp, f, f, i]
         L.mc_find_shift.argtypes = [vp, i, vp, i, _P(f), _P(f), _P(f), i]
+        L.mc_find_shift_noalloc.argtypes = [vp, i, vp, i, _P(f), _P(f), _P(f), i]
         L.mc_context_create.argtypes = [i, _P(vp)]
         L.mc_context_destroy.argtypes = [vp]
         L.mc_align_movie.argtypes = [vp, _P(MovieDesc), _P(Params), _P(Outputs), _P(Report)]

@@ -1,9 +1,17 @@
 // mc_level0.hip -- kernels behind the cufunc.h-compatible entry points
-// (padded row-major spectrum layout) and small helpers.
+// (padded row-major spectrum layout), small helpers, and the FindShift entry
+// point that allocates nothing (mc_find_shift_noalloc).
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
+#include <mutex>
+
+#include "../../include/mc_abi.h"
 #include "mc_device.h"
 #include "mc_kernels.h"
+
+// Records the text returned by mc_last_error() (defined in mc_align.cpp).
+int mc_internal_set_error(int code, const char* msg);
 
 namespace mc {
 
@@ -107,3 +115,82 @@ hipError_t launch_dc_gather(const float2* spectra, size_t stride, int nframes, d
 
 }  // namespace mc
 
+// ------------------------------------------------ FindShift without allocation
+// Device memory of mc_find_shift_noalloc: the box copied out of the caller's
+// map (MC_NOALLOC_MAX_BOX^2 floats), then the peak record of the search (the
+// result triple and the per-slice maxima, find_peaks_out_elems(1) doubles).
+// One copy per GPU, held until the process ends.  Every element a call reads
+// is written earlier in the same call, so nothing needs clearing between
+// calls; calls that overlap on one GPU race on it.
+namespace {
+constexpr size_t kNoallocBoxBytes = sizeof(float) * MC_NOALLOC_MAX_BOX * MC_NOALLOC_MAX_BOX;
+constexpr size_t kNoallocPeakDoubles = 32;
+constexpr size_t kNoallocBytes = kNoallocBoxBytes + sizeof(double) * kNoallocPeakDoubles;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current (the host-side name is not a device
+// address).
+static int noalloc_scratch(unsigned char** out) {
+  static std::mutex mu;
+  static unsigned char* ptrs[64] = {};
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return mc_internal_set_error(MC_ERR_DEVICE, hipGetErrorString(e));
+  if (dev < 0 || dev >= 64) return mc_internal_set_error(MC_ERR_DEVICE, "device index out of range");
+  std::lock_guard<std::mutex> lk(mu);
+  if (!ptrs[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      char buf[256];
+      snprintf(buf, sizeof buf, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+      return mc_internal_set_error(MC_ERR_DEVICE, buf);
+    }
+    ptrs[dev] = static_cast<unsigned char*>(ptr);
+  }
+  *out = ptrs[dev];
+  return MC_OK;
+}
+
+// mc_find_shift (FindShift, cufunc.cu:396-497) with its box copy and peak
+// record in `scratch`: the same copy, kernels and read-backs, so the same
+// results, and no hipMalloc or hipFree.
+extern "C" int mc_find_shift_noalloc(const float* dsrc, int n, float* hboxmap, int box,
+                                     float* sx, float* sy, float* cc, int wnoise) {
+  if (!dsrc || box < 16 || box > n)
+    return mc_internal_set_error(MC_ERR_ARG, "bad box");
+  if (box > MC_NOALLOC_MAX_BOX) {
+    char buf[160];
+    snprintf(buf, sizeof buf, "box %d exceeds MC_NOALLOC_MAX_BOX (%d) of mc_find_shift_noalloc",
+             box, MC_NOALLOC_MAX_BOX);
+    return mc_internal_set_error(MC_ERR_ARG, buf);
+  }
+  if (mc::find_peaks_out_elems(1) > kNoallocPeakDoubles)
+    return mc_internal_set_error(MC_ERR_ARG, "peak record exceeds the fixed scratch");
+  unsigned char* base = nullptr;
+  const int rc = noalloc_scratch(&base);
+  if (rc != MC_OK) return rc;
+  float* dbox = reinterpret_cast<float*>(base);
+  double* dres = reinterpret_cast<double*>(base + kNoallocBoxBytes);
+  const int ori = (n - box) / 2;
+  hipError_t e = hipMemcpy2D(dbox, sizeof(float) * box, dsrc + (size_t)ori * (n + 2) + ori,
+                             sizeof(float) * (n + 2), sizeof(float) * box, box,
+                             hipMemcpyDeviceToDevice);
+  if (e == hipSuccess) e = mc::launch_find_peaks(dbox, 1, box, n, wnoise, dres, nullptr);
+  double r[3] = {0, 0, 0};
+  if (e == hipSuccess) e = hipMemcpy(r, dres, sizeof r, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && hboxmap)
+    e = hipMemcpy(hboxmap, dbox, sizeof(float) * (size_t)box * box, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    char buf[256];
+    snprintf(buf, sizeof buf, "FindShift (no allocation): %s", hipGetErrorString(e));
+    return mc_internal_set_error(MC_ERR_DEVICE, buf);
+  }
+  if (sx) *sx = (float)r[0];
+  if (sy) *sy = (float)r[1];
+  if (cc) *cc = (float)r[2];
+  return MC_OK;
+}
