@@ -72,8 +72,8 @@ int mc_mem_d2d(void *dst, const void *src, size_t bytes);
 int mc_mem_bin_d2d(float *dst, const float *src, int dst_n, int src_n);
 int mc_mem_bin_d2h(float *dst_host, const float *src, int dst_n, int src_n);
 /* GPUFFTPlan / GPUIFFTPlan (cufunc.cu:202-215): n x n in-place plans.
- * Supported n: powers of two from 64 to 8192, and 768, 1280, 1536, 1920,
- * 2560, 3072, 3840, 5120, 6144 (2^k * 3, 5 or 15). */
+ * Supported n: any even size from 64 to 8192 (Stockham plans for powers of
+ * two and 2^k * 3, 5 or 15; chirp-z transforms for the others). */
 int mc_fft_plan(mc_plan *plan, int n);
 int mc_ifft_plan(mc_plan *plan, int n);
 /* GPUFFT2d / GPUIFFT2d (cufunc.cu:223-233): unnormalised, in place. */

@@ -58,17 +58,10 @@ int mc_internal_set_error(int code, const char* msg) {
 extern "C" const char* mc_last_error(void) { return g_err.c_str(); }
 extern "C" const char* mc_version(void) { return "motioncorr1exp_amd 0.1 (gfx950)"; }
 
-static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
-// Transform sizes: powers of two in [64, 8192] and 2^k * {3, 5, 15} from 768.
-static bool fft_size_ok(int n) {
-  switch (n) {
-    case 768: case 1280: case 1536: case 1920: case 2560: case 3072: case 3840: case 5120:
-    case 6144:
-      return true;
-    default:
-      return is_pow2(n) && n >= 64 && n <= 8192;
-  }
-}
+// Transform sizes: any even size in [64, 8192], as cuFFT plans any size
+// (cufunc.cu:202-215).  Powers of two and 2^k * {3, 5, 15} from 768 have
+// Stockham plans; the others run as chirp-z transforms.
+static bool fft_size_ok(int n) { return n >= 64 && n <= 8192 && n % 2 == 0; }
 
 // ------------------------------------------------------- device buffers
 struct DevBuf {
@@ -908,7 +901,8 @@ static int prepare_run(MovieRun& r) {
     return set_err(MC_ERR_ARG, "unknown precision form %d", p->precision);
   if (!mc::pair_form_supported(p->precision, r.g.n, r.box))
     return set_err(MC_ERR_ARG,
-                   "precision form %d needs n in [512, 8192] and peak box <= 96 (n %d, box %d)",
+                   "precision form %d needs n a multiple of 512 in [512, 8192] and peak box "
+                   "<= 96 (n %d, box %d)",
                    p->precision, r.g.n, r.box);
   r.K = r.F - 1;
   r.P = mc_pair_list(r.F, p->FrameDistOffset, nullptr, 0);
@@ -916,7 +910,7 @@ static int prepare_run(MovieRun& r) {
   mc_pair_list(r.F, p->FrameDistOffset, r.pairs.data(), r.P);
   MC_TRY(check_corrections(m));
   if (r.out && (r.out->disp_raw_fft || r.out->disp_corr_sum || r.out->disp_corr_fft) &&
-      (r.g.n < MC_DISPDIM || r.g.n % 128))
+      r.g.n < MC_DISPDIM)
     return set_err(MC_ERR_ARG, "quick-look images need a transform size >= %d (n %d)",
                    MC_DISPDIM, r.g.n);
   r.nslot = r.F * ((m->frames2 || (m->corr && m->corr->second)) ? 2 : 1) + 1;
@@ -1000,8 +994,8 @@ static void iter_update(MovieRun& r, const double* res) {
   r.it++;
 }
 
-// Quick-look images of one sum spectrum ([kx][ky], size n >= 400, n % 128
-// == 0) into host buffers of MC_DISPDIM^2 shorts (DFAlign.cu:73-179,
+// Quick-look images of one sum spectrum ([kx][ky], even size n >= 400)
+// into host buffers of MC_DISPDIM^2 shorts (DFAlign.cu:73-179,
 // 842-848, 908-911, 1270-1285; func.cpp:372-555).  sum_img: crop to 400,
 // C2R, 8-bit mapping.  fft_img: centred log-modulus image, its R2C binned
 // to 400 by the stage-1 transform, C2R, / 400^2, 8-bit mapping.

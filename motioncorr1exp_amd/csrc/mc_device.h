@@ -163,6 +163,7 @@ __host__ __device__ constexpr int pow2_stage_count(int N) {
 __host__ __device__ constexpr int pow2_stage_radix(int N, int s) {
   // 16:[16] 32:[8,4] 64:[8,8] 128:[16,8] 256:[16,16] 512:[8,8,8]
   // 1024:[16,8,8] 2048:[16,16,8] 4096:[16,16,16] 8192:[16,16,8,4]
+  // 16384:[16,16,16,4]
   return N <= 16 ? N
        : N == 32 ? (s == 0 ? 8 : 4)
        : N == 64 ? 8
@@ -172,7 +173,8 @@ __host__ __device__ constexpr int pow2_stage_radix(int N, int s) {
        : N == 1024 ? (s == 0 ? 16 : 8)
        : N == 2048 ? (s < 2 ? 16 : 8)
        : N == 4096 ? 16
-       : (s < 2 ? 16 : (s == 2 ? 8 : 4));
+       : N == 8192 ? (s < 2 ? 16 : (s == 2 ? 8 : 4))
+       : (s < 3 ? 16 : 4);
 }
 __host__ __device__ constexpr int odd_part(int N) { return N % 2 ? N : odd_part(N / 2); }
 __host__ __device__ constexpr int stage_count(int N) {

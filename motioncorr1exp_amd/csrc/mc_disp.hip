@@ -52,10 +52,34 @@ __global__ __launch_bounds__(256) void k_logmod_image(const float2* __restrict__
   }
 }
 
+// The same centred image for any even n (tiles would straddle column n/2):
+// one thread per output pixel, the spectrum element read directly.
+__global__ __launch_bounds__(256) void k_logmod_image_any(const float2* __restrict__ S, int n,
+                                                          float scale, float* __restrict__ img) {
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long long)n * n) return;
+  const int hn = n / 2;
+  const int j = (int)(t / n), i = (int)(t % n);
+  int kx, ky;
+  if (i >= hn) {
+    kx = i - hn;
+    ky = (j + hn) % n;
+  } else {
+    kx = hn - i;
+    ky = j == 0 ? hn - 1 : ((hn - j) % n + n) % n;
+  }
+  const float2 v = S[(size_t)kx * n + ky];
+  img[t] = logf(1.f + sqrtf(v.x * v.x + v.y * v.y) * scale);
+}
+
 hipError_t launch_logmod_image(const float2* spec, int n, float scale, float* img,
                                hipStream_t st) {
-  if (n < 128 || n % 128) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_logmod_image, dim3(n / 64, n / 64), dim3(256), 0, st, spec, n, scale, img);
+  if (n < 2 || n % 2) return hipErrorInvalidValue;
+  if (n % 128 == 0)
+    hipLaunchKernelGGL(k_logmod_image, dim3(n / 64, n / 64), dim3(256), 0, st, spec, n, scale, img);
+  else
+    hipLaunchKernelGGL(k_logmod_image_any, dim3((unsigned)(((long long)n * n + 255) / 256)),
+                       dim3(256), 0, st, spec, n, scale, img);
   return hipGetLastError();
 }
 

@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "mc_device.h"
+#include "mc_frames.h"
 #include "mc_kernels.h"
 
 namespace mc {
@@ -55,36 +56,6 @@ const float2* twiddles(int n) {
   }
   cache[key] = d;
   return d;
-}
-
-// ----------------------------------------------------------------- loads
-template <int DT> struct Elem;
-template <> struct Elem<DT_INT8> { typedef int8_t T; };
-template <> struct Elem<DT_UINT8> { typedef uint8_t T; };
-template <> struct Elem<DT_INT16> { typedef int16_t T; };
-template <> struct Elem<DT_UINT16> { typedef uint16_t T; };
-template <> struct Elem<DT_FLOAT32> { typedef float T; };
-
-// two-level twiddle split for a table of size ntw
-__host__ __device__ constexpr int tw_lbits(int ntw) { return ilog2c(ntw) / 2; }
-
-// Corrected value of frame pixel (X, Y): dark/gain at the source pixel of
-// the nearest-neighbour stretch (DFAlign.cu:718-766).  The float operations
-// are the reference's: (raw - dark) * gain, raw * gain or raw - dark; the
-// source index is int(X * xs + 0.5) with the product in float and the
-// rounding addition in double.
-MC_DEV int2 stretch_src(const R2CGeom& g, int X, int Y) {
-  if ((g.xs < 1.f || g.ys < 1.f) && X > 0 && Y > 0) {
-    X = (int)((double)((float)X * g.xs) + 0.5);
-    Y = (int)((double)((float)Y * g.ys) + 0.5);
-  }
-  return make_int2(X, Y);
-}
-MC_DEV float correct(const R2CGeom& g, float v, float d, float gn) {
-  if (g.dark && g.gain) return (v - d) * gn;
-  if (g.gain) return v * gn;
-  if (g.dark) return v - d;
-  return v;
 }
 
 // ------------------------------------------------------------- row pass
@@ -306,6 +277,12 @@ __global__ __launch_bounds__(256) void k_fill(const float* rowsum, int nu, int h
   if (threadIdx.x == 0) fill[f] = (float)(red[0] / ((double)w * (double)h));
 }
 
+hipError_t launch_fill(const float* rowsum, int nu, int h, int w, float* fill, int nframes,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_fill, dim3(nframes), dim3(256), 0, st, rowsum, nu, h, w, fill);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------- column pass
 // Loads column kx of the row-pass scratch (rows [0, h_data); the canvas rows
 // beyond are the fill) into LDS, adding fill x the spectrum P of the fill
@@ -502,7 +479,9 @@ hipError_t launch_frames_to_spectra(const R2CGeom& g, int f0, int nframes, float
     MC_F2S(768) MC_F2S(1536) MC_F2S(3072) MC_F2S(6144)
     MC_F2S(1280) MC_F2S(2560) MC_F2S(5120) MC_F2S(1920) MC_F2S(3840)
 #undef MC_F2S
-    default: return hipErrorInvalidValue;
+    default:  // any other even size: chirp-z transforms (mc_czt.hip)
+      return launch_frames_to_spectra_czt(g, f0, nframes, spectra, spec_stride, rt, rowsum, fill,
+                                          padP, st);
   }
 }
 
@@ -643,7 +622,8 @@ hipError_t launch_spectrum_to_image(const float2* spec, int n, float scale, floa
     case 5120: return spectrum_to_image_t<5120>(spec, scale, img, img_pitch, tmp, st);
     case 1920: return spectrum_to_image_t<1920>(spec, scale, img, img_pitch, tmp, st);
     case 3840: return spectrum_to_image_t<3840>(spec, scale, img, img_pitch, tmp, st);
-    default: return hipErrorInvalidValue;
+    default:  // any other even size: chirp-z transforms (mc_czt.hip)
+      return launch_spectrum_to_image_czt(spec, n, scale, img, img_pitch, tmp, st);
   }
 }
 

@@ -17,7 +17,7 @@ struct R2CGeom {
   size_t frame_pitch;   // bytes
   int offx, offy;       // region origin in the frame
   int w, h;             // data region (<= nu); the rest of the canvas is fill
-  int nu;               // canvas (unbinned) size, power of two
+  int nu;               // canvas (unbinned) size, even
   int n;                // binned size (<= nu), even
   int pad_mode;         // 1: canvas outside the region holds the frame mean
   int vec = 0;          // set by the launcher: element pairs are 2-element aligned
@@ -49,6 +49,24 @@ enum { R2C_AUTO = 0, R2C_R4_T256 = 1, R2C_R2_T256 = 2, R2C_R4_T512 = 3, R2C_VARI
 hipError_t launch_frames_to_spectra(const R2CGeom& g, int f0, int nframes, float2* spectra,
                                     size_t spec_stride, float2* rt, float* rowsum,
                                     float* fill, const float2* padP, int variant, hipStream_t st);
+
+// Per-frame mean of the data region from the row sums (pad geometry).
+hipError_t launch_fill(const float* rowsum, int nu, int h, int w, float* fill, int nframes,
+                       hipStream_t st);
+
+// Transforms of sizes without a Stockham plan (any even size in [64, 8192]):
+// Bluestein / chirp-z on power-of-two transforms (mc_czt.hip).  The stage-1
+// and sum-image entry points above dispatch to them by size; the pair
+// column pass computes only the box rows of each column inverse.
+int czt_len(int need);
+hipError_t launch_frames_to_spectra_czt(const R2CGeom& g, int f0, int nframes, float2* spectra,
+                                        size_t spec_stride, float2* rt, float* rowsum,
+                                        float* fill, const float2* padP, hipStream_t st);
+hipError_t launch_spectrum_to_image_czt(const float2* spec, int n, float scale, float* img,
+                                        size_t img_pitch, float2* tmp, hipStream_t st);
+hipError_t launch_pair_cols_czt(const float2* spectra, size_t spec_stride, const int2* pairs,
+                                int npairs, const float* wxy, int n, int box, float2* tbuf,
+                                hipStream_t st);
 
 // Inverse 2-D transform of one spectrum ([kx][ky] layout, size n) to an
 // n x n real image scaled by `scale` (C2R with the DC/Nyquist projection).

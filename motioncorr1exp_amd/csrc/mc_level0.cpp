@@ -31,15 +31,9 @@ static int l0_err(int code, const char* what, hipError_t e) {
       return l0_err(e_ == hipErrorOutOfMemory ? MC_ERR_NOMEM : MC_ERR_DEVICE, #expr, e_); \
   } while (0)
 
-static bool size_ok(int n) {
-  switch (n) {
-    case 768: case 1280: case 1536: case 1920: case 2560: case 3072: case 3840: case 5120:
-    case 6144:
-      return true;
-    default:
-      return n >= 64 && n <= 8192 && (n & (n - 1)) == 0;
-  }
-}
+// Any even size in [64, 8192]: Stockham plans for powers of two and
+// 2^k * {3, 5, 15}, chirp-z transforms otherwise.
+static bool size_ok(int n) { return n >= 64 && n <= 8192 && n % 2 == 0; }
 
 struct mc_plan_s {
   int n;
@@ -130,7 +124,7 @@ extern "C" int mc_mem_bin_d2h(float* dst, const float* src, int dst_n, int src_n
 
 static int make_plan(mc_plan* plan, int n, int inverse) {
   if (!plan) return mc_internal_set_error(MC_ERR_ARG, "plan is NULL");
-  if (!size_ok(n)) return mc_internal_set_error(MC_ERR_ARG, "FFT size must be a power of two in [64, 8192] or 768, 1280, 1536, 1920, 2560, 3072, 3840, 5120, 6144");
+  if (!size_ok(n)) return mc_internal_set_error(MC_ERR_ARG, "FFT size must be even, in [64, 8192]");
   mc_plan_s* p = new mc_plan_s();
   p->n = n;
   p->inverse = inverse;

@@ -1106,8 +1106,21 @@ static hipError_t pair_cols_n(const float2* spectra, size_t spec_stride, const i
   return hipErrorInvalidValue;
 }
 
+// Sizes of the decimated column pass (n = 128 A, and 64); other even sizes
+// take the chirp-z column pass (mc_czt.hip).
+static bool pair_cols_decimated(int n) {
+  switch (n) {
+    case 64: case 128: case 256: case 512: case 1024: case 2048: case 4096: case 8192:
+    case 768: case 1536: case 3072: case 6144: case 1280: case 2560: case 5120: case 1920:
+    case 3840:
+      return true;
+    default:
+      return false;
+  }
+}
+
 bool pair_form_supported(int precision, int n, int box) {
-  if (precision == 0) return box >= 1 && box <= n && box <= 256 && (n % 128 == 0 || n == 64);
+  if (precision == 0) return box >= 1 && box <= n && box <= 256 && n >= 64 && n <= 8192 && n % 2 == 0;
   // the hi/lo column kernel needs 32-column blocks of 8 n/128 columns
   if (precision == 1) return n >= 512 && n <= 8192 && n % 512 == 0 && box >= 1 && box <= 96;
   return false;
@@ -1133,6 +1146,8 @@ hipError_t launch_pair_cols(const float2* spectra, size_t spec_stride, const int
       default: return hipErrorInvalidValue;
     }
   }
+  if (!pair_cols_decimated(n))
+    return launch_pair_cols_czt(spectra, spec_stride, pairs, npairs, wxy, n, box, tbuf, st);
   switch (n) {
     case 64: return pair_cols_n<64>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
     case 128: return pair_cols_n<128>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
