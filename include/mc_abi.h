@@ -253,13 +253,17 @@ int mc_align_movie(mc_context ctx, const mc_movie_desc *movie, const mc_params *
  * size, peak box, B-factor, precision form and noise window form a group; a group is split
  * into sub-batches whose resident spectra fit in device memory
  * (MC_BATCH_MEM_MB, default 60% of the free memory).  In a sub-batch, stage 1
- * runs per movie, the pair stage of all movies is one persistent launch, the
- * drift model runs per movie on the host, the align-to-sum rounds of the
- * movies still iterating are one batched sum launch plus one pair-stage
- * launch, captured once as a HIP graph and replayed while that set of movies
- * is unchanged, and the final sums run per movie.  Every movie gets the same
- * results as mc_align_movie would give it; the ms_* fields of its report are
- * those of its sub-batch.  The movies' frame buffers must all be resident. */
+ * runs per movie; the pair column pass runs movie by movie (the frame
+ * columns in flight then belong to one movie and stay in L2), and one row
+ * pass and one peak search cover every pair of every movie; the drift model
+ * runs per movie on the host; the align-to-sum rounds of the movies still
+ * iterating are one batched sum launch plus one pair-stage sequence,
+ * captured once as a HIP graph and replayed while that set of movies is
+ * unchanged; the final sums run per movie.  Every movie gets bit for bit the
+ * results mc_align_movie would give it (the per-pair arithmetic and its
+ * summation order do not depend on the other movies); the ms_* fields of
+ * its report are those of its sub-batch.  The movies' frame buffers must all
+ * be resident. */
 typedef struct {
   const mc_movie_desc *movie;
   const mc_params *params;
