@@ -203,23 +203,12 @@ def committed_traffic(group, launches_per_movie=1.0):
     return tot / launches_per_movie if seen else None
 
 
-class _TorchDev:
-    """A torch device tensor seen as a device buffer of the C ABI (.ptr)."""
-
-    def __init__(self, t):
-        self.t = t
-        self.ptr = t.data_ptr()
-
-    def free(self):
-        self.t = None
-
-
 def batch_leg(n_movies, steps, seed=1004, tune=True, tune_cache=None, oracle=True):
     """Batching form: n_movies movies of seeded uneven size (square sides
     uniform over the even values 1024..4096, padded with the region mean to
     the next supported transform size), 8..50 frames, dose uniform in [0.5, 1.5],
     random-walk drift N(0, 0.7^2), uint8 Poisson counts of the random-field
-    phantom generated on the device (synth.make_movie_torch); bin 1, fod 2,
+    phantom (synth.make_movie); bin 1, fod 2,
     bft 150, pbx 96, 3 align-to-sum rounds.  Times mc_align_batch (one call =
     one step over all movies) and the same movies through mc_align_movie one
     by one; reports batch-vs-single shift differences (max and RMS) and, for
@@ -228,22 +217,30 @@ def batch_leg(n_movies, steps, seed=1004, tune=True, tune_cache=None, oracle=Tru
     import numpy as np
 
     import motioncorr1exp_amd as mc
-    from motioncorr1exp_amd.synth import make_movie_torch
+    from motioncorr1exp_amd.synth import make_movie
 
     rng = np.random.default_rng(seed)
     t0 = time.time()
-    movies = []
+    specs = []
     for i in range(n_movies):
         side = int(rng.integers(512, 2049)) * 2
         F = int(rng.integers(8, 51))
         dose = float(rng.uniform(0.5, 1.5))
-        pad = min(z for z in PAD_SIZES if z >= side)
-        t, drift = make_movie_torch(side, side, F, dose=dose, step=0.7, seed=seed * 100 + i)
+        specs.append((side, F, dose, min(z for z in PAD_SIZES if z >= side)))
+    pick = sorted(range(n_movies), key=lambda i: (specs[i][3], specs[i][1]))
+    pick = (pick[0], pick[-1])
+    movies, host = [], {}
+    for i, (side, F, dose, pad) in enumerate(specs):
+        # generated on the host one movie at a time, uploaded, host copy dropped
+        frames, drift = make_movie(side, side, F, dose=dose, step=0.7, seed=seed * 100 + i,
+                                   dtype=np.uint8)
+        if i in pick:
+            host[i] = frames
         prm = mc.default_params(F, pad=pad, bin=1.0, FrameDistOffset=2, bfactor=150.0,
                                 CCPeakSearchDim=96, iter=3)
-        movies.append(dict(dev=_TorchDev(t), shape=(F, side, side), dtype=np.uint8, params=prm,
-                           want=("frame_shift",), drift=drift))
-    log("batch: %d random-field movies generated on the device in %.1fs" % (n_movies, time.time() - t0))
+        movies.append(dict(dev=mc.DeviceArray(frames), shape=(F, side, side), dtype=np.uint8,
+                           params=prm, want=("frame_shift",), drift=drift))
+    log("batch: %d random-field movies generated in %.1fs" % (n_movies, time.time() - t0))
     ctx = mc.Context(0)
     if tune:
         # shipped table first; geometries it lacks are tuned on their first movie
@@ -281,8 +278,6 @@ def batch_leg(n_movies, steps, seed=1004, tune=True, tune_cache=None, oracle=Tru
     for m in movies:
         n = m["params"]["pad"]
         sizes[n] = sizes.get(n, 0) + 1
-    pick = sorted(range(n_movies), key=lambda i: (movies[i]["params"]["pad"], movies[i]["shape"][0]))
-    pick = (pick[0], pick[-1])
     if oracle:  # one more (untimed) batch call that also returns these movies' sums
         res2, _ = ctx.align_batch([dict(m, want=("frame_shift", "sum_image")) if i in pick else m
                                    for i, m in enumerate(movies)])
@@ -308,7 +303,7 @@ def batch_leg(n_movies, steps, seed=1004, tune=True, tune_cache=None, oracle=Tru
         for i in pick:
             m, r = movies[i], res2[i]
             assert np.array_equal(r["frame_shift"], res[i]["frame_shift"])
-            frames = m["dev"].t.cpu().numpy().astype(np.float32)
+            frames = host[i].astype(np.float32)
             o = Oracle("f64").align_movie(frames, dict(pad=m["params"]["pad"], fod=2, box=96,
                                                        iter=3, full_c2r=0, nthreads=16),
                                           want_boxes=False)
@@ -385,6 +380,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-itr3", action="store_true")
     ap.add_argument("--no-precision", action="store_true", help="skip the MFMA hi/lo form")
+    ap.add_argument("--no-refgeom", action="store_true",
+                    help="skip the reference's default geometry (n = 3710, no padding)")
     ap.add_argument("--frames", type=int, default=40)
     ap.add_argument("--no-batch", action="store_true", help="skip the batching form")
     ap.add_argument("--no-grad", action="store_true", help="skip the differentiation form")
@@ -564,6 +561,29 @@ def main():
         out["itr3"] = {"movies_per_s": k3 / dt3, "ms_per_movie": dt3 * 1e3 / k3,
                        "niter": r3["report"]["niter"]}
         log("itr3: %.3f ms/movie" % (dt3 * 1e3 / k3))
+
+    if not args.no_refgeom:
+        # the reference's default geometry: square crop min(nx, ny) = 3710 =
+        # 2*5*7*53, no padding (chirp-z transforms)
+        pr = dict(params, pad=0)
+        step(pr)
+        mc.check(L.mc_sync())
+        ctx.kernel_timing(True)
+        t0 = time.perf_counter()
+        kr = max(2, args.steps // 2)
+        for _ in range(kr):
+            rr = step(pr)
+        mc.check(L.mc_sync())
+        dtr = time.perf_counter() - t0
+        kernr = {g: ctx.kernel_time(g)[1] / kr for g in mc.KT}
+        ctx.kernel_timing(False)
+        out["reference_geometry"] = {
+            "workload": "the same movie at the reference's default square crop, n = %d "
+                        "(no -pad; chirp-z transforms), itr 0" % rr["report"]["n"],
+            "n": rr["report"]["n"], "movies_per_s": kr / dtr, "ms_per_movie": dtr * 1e3 / kr,
+            "kernels_ms_per_movie": kernr}
+        log("reference geometry n=%d: %.3f ms/movie %s" % (rr["report"]["n"], dtr * 1e3 / kr,
+                                                         {k: round(v, 3) for k, v in kernr.items()}))
 
     if not args.no_precision:
         # precision form: pair-stage DFTs as FP16 hi/lo MFMA products

@@ -177,41 +177,3 @@ def make_movie_gpu(nx, ny, nframe, dose=1.0, step=0.7, seed=1004, dtype=np.uint8
         raise RuntimeError("mcs_gpu_movie failed (HIP error %d)" % rc)
     return d, drift
 
-
-def make_movie_torch(nx, ny, nframe, dose=1.0, step=0.7, seed=1001, quad_sigma=0.0,
-                     dtype="uint8", drift=None, device="cuda:0"):
-    """make_movie's random-field movie generated on the GPU with torch (data
-    generation for benchmarks; the alignment path never calls it): the same
-    phantom construction, exact Fourier shift on the same 1.25x periodic
-    canvas, crop and Poisson counts, drawn from torch's seeded generator, so
-    the movie is a different draw of the same distribution.  Returns
-    (torch tensor [nframe, ny, nx] on `device`, drift t[nframe, 2])."""
-    import torch
-
-    m = _canvas_size(nx, ny)
-    g = torch.Generator(device=device)
-    g.manual_seed(int(seed))
-    white = torch.randn((m, m), generator=g, device=device, dtype=torch.float32)
-    W = torch.fft.rfft2(white)
-    ky = torch.fft.fftfreq(m, device=device)[:, None]
-    kx = torch.fft.rfftfreq(m, device=device)[None, :]
-    W *= torch.sqrt(1.0 / (1.0 + (kx * kx + ky * ky) / 0.05 ** 2))
-    W[0, 0] = 0
-    gf = torch.fft.irfft2(W, s=(m, m))
-    gf = (gf - gf.mean()) / gf.std()
-    P = torch.fft.rfft2(torch.exp(0.5 * gf - 0.125) * dose)
-    if drift is None:
-        drift = make_drift(nframe, step, seed, quad_sigma)
-    tdt = {"uint8": torch.uint8, "float32": torch.float32}[str(dtype)]
-    out = torch.empty((nframe, ny, nx), dtype=tdt, device=device)
-    for f in range(nframe):
-        tx, ty = (float(v) for v in drift[f])
-        # separable phase factors, as make_movie forms them
-        py = torch.exp(-2j * np.pi * ky.double() * ty).to(torch.complex64)
-        px = torch.exp(-2j * np.pi * kx.double() * tx).to(torch.complex64)
-        img = torch.fft.irfft2(P * py * px, s=(m, m))[:ny, :nx].clamp_(min=0)
-        out[f] = torch.poisson(img, generator=g).to(tdt)
-    if out.is_cuda:
-        torch.cuda.synchronize(out.device)
-    return out, drift
-
