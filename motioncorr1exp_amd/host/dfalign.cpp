@@ -72,6 +72,20 @@ int CDFAlign::getNFrame(const char* filename) {
   return n;
 }
 
+// DFAlign.cu:395-421: the header of whichever format opens first.
+MRCHeader CDFAlign::getMrcHeader(const char* filename) {
+  MRCHeader h;
+  memset(&h, 0, sizeof h);
+  MRC* s = nullptr;
+  if (isMRCZ(filename)) s = new MRCZ();
+  else if (isMRC(filename)) s = new MRC();
+  else if (isTIFF(filename)) s = new MRCT();
+  if (!s) return h;
+  if (s->open(filename, "rb") > 0) s->getHeader(&h);
+  delete s;
+  return h;
+}
+
 // DFAlign.cu:423-453: the worker runs on its own thread; RunAlign blocks.
 void CDFAlign::RunAlign() {
   std::thread t(&CDFAlign::AlignWorker, this);
@@ -297,6 +311,9 @@ void CDFAlign::AlignWorker() {
   mp.iterthres = m_iterthres;
   mp.pad = para.pad;
   mp.fftscale = para.fftscale;
+  mp.precision = para.precision;
+  mp.grad_steps = para.grad_steps;
+  mp.grad_step_px = para.grad_step_px;
 
   const int box = para.CCPeakSearchDim;
   const int P = mc_pair_list(nframe, para.FrameDistOffset, nullptr, 0);

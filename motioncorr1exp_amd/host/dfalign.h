@@ -9,6 +9,8 @@
 #include <string>
 #include <vector>
 
+#include "mrcio.h"
+
 #define DISPDIM 400
 
 // APARA (DFAlign.h:16-63), plus the geometry option of this build.
@@ -42,6 +44,11 @@ struct APARA {
   float fscMax;
   // new in this build: 0 = reference square crop; N = pad to N x N
   int pad;
+  // new in this build: pair-stage precision form (MC_PREC_*), adjoint
+  // sharpness refinement steps (0 = off) and its first step in pixels
+  int precision;
+  int grad_steps;
+  float grad_step_px;
 };
 
 class CDFAlign {
@@ -83,6 +90,9 @@ class CDFAlign {
   void RunAlign();
   int getNFrame();
   int getNFrame(const char* filename);
+  // header of an MRC, gzip-MRC or TIFF stack (zeros if none opens),
+  // DFAlign.h:150
+  MRCHeader getMrcHeader(const char* filename);
 
  private:
   void AlignWorker();

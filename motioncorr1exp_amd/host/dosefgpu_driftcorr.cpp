@@ -1,7 +1,8 @@
 // dosefgpu_driftcorr -- command-line drift correction of a dose-fractionated
 // stack on one MI355X.  Options, defaults, clamps, default output names and
 // the parameter block of the log follow the reference CLI
-// (src-1125/dosefgpu_driftcorr.cpp:8-650); -pad is new.
+// (src-1125/dosefgpu_driftcorr.cpp:8-650); -pad, -prc, -grd and -grs are
+// new.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -29,6 +30,9 @@ static APARA default_para() {
   p.bDispSumCorr = p.bDispFFTRaw = p.bDispFFTCorr = p.bDispCCMap = true;
   p.fscMax = 0.25f;
   p.pad = 0;
+  p.precision = 0;
+  p.grad_steps = 0;
+  p.grad_step_px = 0.05f;
   return p;
 }
 
@@ -58,6 +62,14 @@ static bool verify_para(CDFAlign& a) {
   }
   if (p.CCPeakSearchDim < 16) {
     printf("Peak search box (-pbx) must be at least 16. Abort\n");
+    return false;
+  }
+  if (p.precision != 0 && p.precision != 1) {
+    printf("Precision form (-prc) must be 0 or 1. Abort\n");
+    return false;
+  }
+  if (p.grad_steps < 0 || p.grad_step_px < 0) {
+    printf("Refinement steps (-grd) and step (-grs) must not be negative. Abort\n");
     return false;
   }
   return true;
@@ -138,6 +150,9 @@ static bool get_para(int narg, char* argv[], CDFAlign& a) {
     else if (opt_is(o, "-itr")) a.m_iter = atoi(v);
     else if (opt_is(o, "-ith")) a.m_iterthres = (float)atof(v);
     else if (opt_is(o, "-pad")) p.pad = atoi(v);
+    else if (opt_is(o, "-prc")) p.precision = atoi(v);
+    else if (opt_is(o, "-grd")) p.grad_steps = atoi(v);
+    else if (opt_is(o, "-grs")) p.grad_step_px = (float)atof(v);
     else {
       printf("Undefined option: %s .Abort.\n", o);
       return false;
@@ -217,6 +232,8 @@ static void show_para(CDFAlign& a) {
   fprintf(fp, "  -itr  %d\n", a.m_iter);
   fprintf(fp, "  -ith  %f\n", a.m_iterthres);
   if (p.pad) fprintf(fp, "  -pad  %d\n", p.pad);
+  if (p.precision) fprintf(fp, "  -prc  %d\n", p.precision);
+  if (p.grad_steps) fprintf(fp, "  -grd  %d\n  -grs  %f\n", p.grad_steps, p.grad_step_px);
   fprintf(fp, "******Parameter End******\n\n");
   fclose(fp);
 }
@@ -258,7 +275,10 @@ static void usage() {
       {"-fnm", "FileName.mrc", "Gain reference"},
       {"-itr", "0", "Number of iteration for iterative alignment"},
       {"-ith", "0.0", "Convergence threshold for iterative alignment (RMSD shift)"},
-      {"-pad", "0", "Pad the frame into an N x N canvas (2^k or 2^k*{3,5,15}) instead of square crop."},
+      {"-pad", "0", "Pad the frame into an N x N canvas (N even) instead of square crop."},
+      {"-prc", "0", "Pair-stage precision: 0 FP32, 1 FP16 hi/lo MFMA (n a multiple of 512)."},
+      {"-grd", "0", "Sharpness-gradient refinement steps of the final shifts (0: off)."},
+      {"-grs", "0.05", "First refinement step in pixels."},
   };
   printf("Dose Fractionation Tool:\nDrift correction (MI355X)\n\n");
   printf("    Input: InputStack.mrc [OPTION VALUE] ...\n");

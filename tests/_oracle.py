@@ -9,7 +9,9 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_DIR = os.path.join(ROOT, "oracle")
+# MC_SANITIZED=1: the AddressSanitizer/UBSan build (oracle/_san, `make -C
+# oracle sanitize`; tests/test_sanitize.py sets it)
+ORACLE_DIR = os.path.join(ROOT, "oracle", "_san" if os.environ.get("MC_SANITIZED") else "")
 
 
 class OrParams(ctypes.Structure):

@@ -141,3 +141,34 @@ def test_cli_errors(tmp_path):
     assert "Frame distance offset is too big" in r.stdout
     r = subprocess.run([CLI, "s.mrc", "-scx", "0.5"], cwd=tmp_path, capture_output=True, text=True)
     assert "Stretching factor along X-axis needs to be >=1.0." in r.stdout
+
+
+def test_cli_new_form_flags(tmp_path):
+    """-pad, -prc 1 (FP16 hi/lo pair stage) and -grd/-grs (sharpness
+    refinement) reach the library: the log's parameter block lists them and
+    the final shifts equal those of mc_align_movie with the same parameters
+    (to the printed precision)."""
+    if not os.path.exists(CLI):
+        pytest.skip("CLI not built")
+    import motioncorr1exp_amd as mc
+    frames, _ = make_movie(1000, 990, 8, dose=2.0, step=0.7, seed=4040)
+    open(tmp_path / "f.mrc", "wb").write(_mrc_bytes(frames, 2))
+    _run(tmp_path, "f.mrc", ["-pad", "1024", "-prc", "1", "-grd", "5", "-grs", "0.04",
+                             "-dsp", "0", "-fod", "1"])
+    log = open(tmp_path / "f_Log.txt").read()
+    assert "  -pad  1024\n" in log and "  -prc  1\n" in log
+    assert "  -grd  5\n  -grs  0.040000\n" in log
+    shifts = _final_shifts(log)
+    ctx = mc.Context(0)
+    d = mc.DeviceArray(frames)
+    p = mc.default_params(8, pad=1024, precision=1, grad_steps=5, grad_step_px=0.04,
+                          FrameDistOffset=1)
+    r = ctx.align_movie(d, frames.shape, frames.dtype, p, want=("final_shift",))
+    ctx.close()
+    ref = np.vstack([[0, 0], np.cumsum(r["final_shift"], axis=0)])
+    err = np.abs(shifts - ref).max()
+    print("CLI -prc 1 -grd 5 vs library: max difference %.3g px" % err)
+    assert err <= 5e-5 + 1e-9
+    r = subprocess.run([CLI, "f.mrc", "-prc", "3"], cwd=tmp_path, capture_output=True, text=True)
+    assert "Precision form (-prc) must be 0 or 1." in r.stdout
+

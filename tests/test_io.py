@@ -11,7 +11,9 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HOST = os.path.join(ROOT, "motioncorr1exp_amd", "libmchost.so")
+# MC_SANITIZED=1: the AddressSanitizer/UBSan build (tests/test_sanitize.py)
+HOST = os.path.join(ROOT, "motioncorr1exp_amd", "_san" if os.environ.get("MC_SANITIZED") else "",
+                    "libmchost.so")
 REF = os.path.join(ROOT, "oracle", "_ref", "libref_io.so")
 
 
