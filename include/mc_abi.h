@@ -287,7 +287,10 @@ int mc_align_batch(mc_context ctx, const mc_batch_item *items, int count,
  *   stage 1: row-pass shape (rows per workgroup x threads: 4x256, 2x256,
  *            4x512 for canvases >= 4096) x frames per launch (1, 2, 4, 8)
  *            x column pass (one workgroup per column, or persistent with
- *            register twiddles);
+ *            register twiddles), then, for the per-column pass on canvases
+ *            1024 to 8192, the radix plan of its FFT (balanced, reversed
+ *            order, radix-8 chain) x its LDS index padding (one float2 per
+ *            16, or none);
  *   pair stage (FP32 form): column kernel (M=128, M=128 with next-item
  *            prefetch, M-generic, M=128 at 4 waves per SIMD) and row kernel
  *            (FP32 MFMA with 4 or 8 waves per pair, vector ALU).
@@ -313,6 +316,11 @@ typedef struct {
                                [row shape][chunk 1, 2, 4, 8]; 0 = not run  */
   float ms_pair_cols[4];  /* 0 = not run                                    */
   float ms_pair_rows[4];
+  int cols_plan;          /* per-column pass of stage 1: radix plan cols_plan/2
+                             (0 balanced, 1 reversed order, 2 radix-8 chain)
+                             and LDS padding cols_plan%2 (0 one float2 per 16,
+                             1 none); r2c_variant holds it in bits 4-6      */
+  float ms_cols_plan[6];  /* 0 = not run (canvases 1024 to 8192 only)      */
 } mc_tune_report;
 
 int mc_autotune(mc_context ctx, const mc_movie_desc *movie, const mc_params *p,

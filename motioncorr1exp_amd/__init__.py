@@ -113,7 +113,8 @@ class TuneReport(ctypes.Structure):
     _fields_ = [("r2c_variant", ctypes.c_int), ("chunk_frames", ctypes.c_int),
                 ("pair_cols_variant", ctypes.c_int), ("pair_rows_variant", ctypes.c_int),
                 ("ms_stage1", ((ctypes.c_float * 4) * 4) * 2), ("ms_pair_cols", ctypes.c_float * 4),
-                ("ms_pair_rows", ctypes.c_float * 4)]
+                ("ms_pair_rows", ctypes.c_float * 4), ("cols_plan", ctypes.c_int),
+                ("ms_cols_plan", ctypes.c_float * 6)]
 
 
 class MaskElem(ctypes.Structure):
@@ -403,7 +404,8 @@ class Context:
                 "pair_cols_variant": r.pair_cols_variant,
                 "pair_rows_variant": r.pair_rows_variant,
                 "ms_stage1": [[list(row) for row in blk] for blk in r.ms_stage1],
-                "ms_pair_cols": list(r.ms_pair_cols), "ms_pair_rows": list(r.ms_pair_rows)}
+                "ms_pair_cols": list(r.ms_pair_cols), "ms_pair_rows": list(r.ms_pair_rows),
+                "cols_plan": r.cols_plan, "ms_cols_plan": list(r.ms_cols_plan)}
 
     def tune_load(self, cache_path):
         n = ctypes.c_int()

@@ -1702,6 +1702,25 @@ extern "C" int mc_autotune(mc_context ctx, const mc_movie_desc* m, const mc_para
         }
       }
     }
+  // ---- radix plan and LDS padding of the per-column pass (canvases with
+  // plan variants), with the chosen row shape and chunk
+  tr.cols_plan = 0;
+  if (!(bv & mc::R2C_COLS_P) && (r.g.nsu == 1024 || r.g.nsu == 2048 || r.g.nsu == 4096 ||
+                                 r.g.nsu == 8192)) {
+    for (int cpl = 0; cpl < mc::R2C_CPLANS; cpl++) {
+      const int vv = (bv & (mc::R2C_COLS_P - 1)) | (cpl << mc::R2C_CPLAN_SHIFT);
+      float ms = 0;
+      MC_TRY(time_min(c, reps, &ms, [&] {
+        return run_stage1(c, m->frames, m, p, r.g, first, F, 0, vv, bc);
+      }));
+      tr.ms_cols_plan[cpl] = ms;
+      if (cpl == 0 || ms < best) {
+        best = ms;
+        bv = vv;
+        tr.cols_plan = cpl;
+      }
+    }
+  }
   // ---- pair stage (spectra of the last stage-1 run are resident)
   std::vector<int> slots(r.pairs);
   MC_TRY(pair_reserve(c, r.P, n, box, p->precision));
@@ -1792,7 +1811,8 @@ extern "C" int mc_tune_load(mc_context ctx, const char* cache_path, int* loaded)
         json_int(line, "precision", &t.precision) && json_int(line, "r2c", &t.r2c) &&
         json_int(line, "chunk", &t.chunk) && json_int(line, "cols", &t.cols) &&
         json_int(line, "rows", &t.rows) && t.r2c >= 0 &&
-        (t.r2c & ~mc::R2C_COLS_P) < mc::R2C_VARIANTS &&
+        (t.r2c & (mc::R2C_COLS_P - 1)) < mc::R2C_VARIANTS &&
+        (t.r2c >> mc::R2C_CPLAN_SHIFT) < mc::R2C_CPLANS &&
         t.cols >= 0 && t.cols < mc::PAIR_COLS_VARIANTS && t.rows >= 0 &&
         t.rows < mc::PAIR_ROWS_VARIANTS && t.chunk >= 0) {
       set_tune(ctx, t);  // later lines win

@@ -176,20 +176,45 @@ __host__ __device__ constexpr int pow2_stage_radix(int N, int s) {
        : N == 8192 ? (s < 2 ? 16 : (s == 2 ? 8 : 4))
        : (s < 3 ? 16 : 4);
 }
+// Radix plans of the power-of-two part (the autotuner chooses among them):
+//   plan 0: the balanced plans above;
+//   plan 1: the same radices in the reverse order (e.g. 2048: [8,16,16]);
+//   plan 2: a radix-8 chain, the remainder as a last radix-16 or radix-4
+//           stage (4096: [8,8,8,8], 2048: [8,8,8,4], 1024: [8,8,16]); up to
+//           8192.
+__host__ __device__ constexpr int r8_stage_count(int N) {
+  return N <= 16 ? 1 : (ilog2c(N) % 3 == 2 ? ilog2c(N) / 3 + 1 : ilog2c(N) / 3);
+}
+__host__ __device__ constexpr int r8_stage_radix(int N, int s) {
+  return N <= 16 ? N
+       : (ilog2c(N) % 3 == 1 && s == ilog2c(N) / 3 - 1) ? 16
+       : (ilog2c(N) % 3 == 2 && s == ilog2c(N) / 3) ? 4
+       : 8;
+}
+__host__ __device__ constexpr int plan_p2_count(int N, int P) {
+  return P == 2 ? r8_stage_count(N) : pow2_stage_count(N);
+}
+__host__ __device__ constexpr int plan_p2_radix(int N, int s, int P) {
+  return P == 2 ? r8_stage_radix(N, s)
+       : P == 1 ? pow2_stage_radix(N, pow2_stage_count(N) - 1 - s)
+       : pow2_stage_radix(N, s);
+}
 __host__ __device__ constexpr int odd_part(int N) { return N % 2 ? N : odd_part(N / 2); }
-__host__ __device__ constexpr int stage_count(int N) {
-  return pow2_stage_count(N / odd_part(N)) + (odd_part(N) % 3 == 0 ? 1 : 0) +
+// Stages of a length N = 2^k * {1, 3, 5, 15} under plan P: the power-of-two
+// stages, then a radix-3 and/or a radix-5 stage.
+__host__ __device__ constexpr int stage_count(int N, int P = 0) {
+  return plan_p2_count(N / odd_part(N), P) + (odd_part(N) % 3 == 0 ? 1 : 0) +
          (odd_part(N) % 5 == 0 ? 1 : 0);
 }
-__host__ __device__ constexpr int stage_radix(int N, int s) {
-  return s < pow2_stage_count(N / odd_part(N))
-             ? pow2_stage_radix(N / odd_part(N), s)
-             : ((s == pow2_stage_count(N / odd_part(N)) && odd_part(N) % 3 == 0) ? 3 : 5);
+__host__ __device__ constexpr int stage_radix(int N, int s, int P = 0) {
+  return s < plan_p2_count(N / odd_part(N), P)
+             ? plan_p2_radix(N / odd_part(N), s, P)
+             : ((s == plan_p2_count(N / odd_part(N), P) && odd_part(N) % 3 == 0) ? 3 : 5);
 }
 // Table index modulo the table size (any size; a power of two masks).
 MC_DEV int tw_mod(int e, int ntw) { return (ntw & (ntw - 1)) == 0 ? (e & (ntw - 1)) : (e % ntw); }
-__host__ __device__ constexpr int stage_ns(int N, int s) {
-  return s == 0 ? 1 : stage_ns(N, s - 1) * stage_radix(N, s - 1);
+__host__ __device__ constexpr int stage_ns(int N, int s, int P = 0) {
+  return s == 0 ? 1 : stage_ns(N, s - 1, P) * stage_radix(N, s - 1, P);
 }
 
 // Twiddle sources: e^{+2 pi i e/ntw} for e in [0, ntw).
@@ -262,10 +287,10 @@ template <int PB, int STRIDE, int R = 0> MC_DEV int lix_at(int li0, int i0, int 
 
 // Full in-LDS FFT of NB transforms of length N (buf[b*TS + lix(i)]), NT
 // threads.  Data and result in natural order.  Caller synchronises before.
-template <int N, int NT, int S, int NB, int STAGE, int TS, class TW, int PB = 0>
+template <int N, int NT, int S, int NB, int STAGE, int TS, class TW, int PB = 0, int PLAN = 0>
 MC_DEV void fft_lds_stage(float2* buf, const TW& tw, int ntw) {
-  constexpr int R = stage_radix(N, STAGE);
-  constexpr int Ns = stage_ns(N, STAGE);
+  constexpr int R = stage_radix(N, STAGE, PLAN);
+  constexpr int Ns = stage_ns(N, STAGE, PLAN);
   constexpr int NBF = NB * (N / R);
   constexpr int PER = (NBF + NT - 1) / NT;
   float2 v[PER][R];
@@ -298,12 +323,13 @@ MC_DEV void fft_lds_stage(float2* buf, const TW& tw, int ntw) {
   __syncthreads();
 }
 
-template <int N, int NT, int S, int NB, int TS, class TW, int PB = 0>
+template <int N, int NT, int S, int NB, int TS, class TW, int PB = 0, int PLAN = 0>
 MC_DEV void fft_lds_t(float2* buf, const TW& tw, int ntw) {
-  fft_lds_stage<N, NT, S, NB, 0, TS, TW, PB>(buf, tw, ntw);
-  if constexpr (stage_count(N) > 1) fft_lds_stage<N, NT, S, NB, 1, TS, TW, PB>(buf, tw, ntw);
-  if constexpr (stage_count(N) > 2) fft_lds_stage<N, NT, S, NB, 2, TS, TW, PB>(buf, tw, ntw);
-  if constexpr (stage_count(N) > 3) fft_lds_stage<N, NT, S, NB, 3, TS, TW, PB>(buf, tw, ntw);
+  static_assert(stage_count(N, PLAN) <= 4, "at most four stages");
+  fft_lds_stage<N, NT, S, NB, 0, TS, TW, PB, PLAN>(buf, tw, ntw);
+  if constexpr (stage_count(N, PLAN) > 1) fft_lds_stage<N, NT, S, NB, 1, TS, TW, PB, PLAN>(buf, tw, ntw);
+  if constexpr (stage_count(N, PLAN) > 2) fft_lds_stage<N, NT, S, NB, 2, TS, TW, PB, PLAN>(buf, tw, ntw);
+  if constexpr (stage_count(N, PLAN) > 3) fft_lds_stage<N, NT, S, NB, 3, TS, TW, PB, PLAN>(buf, tw, ntw);
 }
 template <int N, int NT, int S, int NB, int TS = N>
 MC_DEV void fft_lds(float2* buf, const float2* tw, int ntw) {

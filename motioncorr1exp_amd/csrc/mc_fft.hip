@@ -311,14 +311,15 @@ MC_DEV void col_load(const float2* src, int h_data, bool fill, float fv, float2 
   }
 }
 
-template <int NU, int NT>
+// PLAN selects the radix plan of the column FFT and PB the LDS index padding
+// (one float2 after every 2^PB; 0: none); the autotuner chooses both.
+template <int NU, int NT, int PLAN = 0, int PB = 4>
 __global__ __launch_bounds__(NT) void k_c2c_cols(const float2* rt, size_t rt_fstride, int nkx,
                                                  int h_data, const float* fill,
                                                  const float2* __restrict__ padP, int n,
                                                  float2* spec, size_t spec_stride,
                                                  const float2* __restrict__ gtw) {
   constexpr int LB = tw_lbits(NU);
-  constexpr int PB = 4;
   __shared__ float2 buf[lds_len<PB>(NU)];
   __shared__ float2 tlo[1 << LB], thi[NU >> LB];
   tw_split_load(tlo, thi, gtw, NU, LB, threadIdx.x, NT);
@@ -333,12 +334,39 @@ __global__ __launch_bounds__(NT) void k_c2c_cols(const float2* rt, size_t rt_fst
   }
   col_load<NU, NT, PB>(src, h_data, fill != nullptr, fv, P, kx, buf);
   __syncthreads();
-  fft_lds_t<NU, NT, -1, 1, NU, TwSplit, PB>(buf, TwSplit{tlo, thi, LB, NU}, NU);
+  fft_lds_t<NU, NT, -1, 1, NU, TwSplit, PB, PLAN>(buf, TwSplit{tlo, thi, LB, NU}, NU);
   float2* dst = spec + (size_t)f * spec_stride + (size_t)kx * n;
   for (int i = threadIdx.x; i < n; i += NT) {
     const int si = i < n / 2 ? i : NU - n + i;
     dst[i] = buf[lix<PB>(si)];
   }
+}
+
+// Column-pass variant v of the autotuner: radix plan v / 2, LDS padding
+// (v % 2 ? none : one float2 per 16); plans 1 and 2 are compiled for the
+// canvases 1024 to 8192.
+template <int NU> constexpr bool cols_plans_compiled() {
+  return NU == 1024 || NU == 2048 || NU == 4096 || NU == 8192;
+}
+template <int NU, int NTC>
+static hipError_t launch_cols_plan(int cplan, dim3 grid, hipStream_t st, const float2* rt,
+                                   size_t rt_fstride, int nkx, int h, const float* fill,
+                                   const float2* padP, int n, float2* spectra,
+                                   size_t spec_stride, const float2* tw) {
+#define MC_COLS(PL, PBV) hipLaunchKernelGGL((k_c2c_cols<NU, NTC, PL, PBV>), grid, dim3(NTC), 0, st, rt, rt_fstride, nkx, h, fill, padP, n, spectra, spec_stride, tw)
+  if constexpr (cols_plans_compiled<NU>()) {
+    switch (cplan) {
+      case 1: MC_COLS(0, 0); return hipGetLastError();
+      case 2: MC_COLS(1, 4); return hipGetLastError();
+      case 3: MC_COLS(1, 0); return hipGetLastError();
+      case 4: MC_COLS(2, 4); return hipGetLastError();
+      case 5: MC_COLS(2, 0); return hipGetLastError();
+      default: break;
+    }
+  }
+  MC_COLS(0, 4);
+#undef MC_COLS
+  return hipGetLastError();
 }
 
 // Persistent form of the column pass: each workgroup walks a contiguous
@@ -432,9 +460,11 @@ static hipError_t frames_to_spectra_t(const R2CGeom& g, int f0, int nframes, flo
     gv.vec = ((uintptr_t)g.src % a == 0 && g.row_pitch % a == 0 && g.frame_pitch % a == 0 &&
               (g.offx & 1) == 0) ? 1 : 0;
   }
-  // bit R2C_COLS_P selects the persistent column pass
+  // bit R2C_COLS_P selects the persistent column pass, bits from
+  // R2C_CPLAN_SHIFT the column plan and padding of the per-column pass
   const bool cols_persistent = (variant & R2C_COLS_P) != 0;
-  variant &= ~R2C_COLS_P;
+  const int cplan = (variant >> R2C_CPLAN_SHIFT) & 7;
+  variant &= R2C_COLS_P - 1;
   // row-pass shape: 4 rows x 512 threads on canvases >= 4096 by default
   if (variant == R2C_AUTO) variant = (4 * N / 16 >= 512) ? R2C_R4_T512 : R2C_R4_T256;
   if (variant == R2C_R4_T512 && 4 * N / 16 < 512) variant = R2C_R4_T256;
@@ -463,9 +493,9 @@ static hipError_t frames_to_spectra_t(const R2CGeom& g, int f0, int nframes, flo
                        nitems, g.h, g.pad_mode ? fill : nullptr, padP, g.n, spectra, spec_stride,
                        tw);
   } else {
-    hipLaunchKernelGGL((k_c2c_cols<NU, NTC>), dim3(nkx * nframes), dim3(NTC), 0, st, rt,
-                       rt_fstride, nkx, g.h, g.pad_mode ? fill : nullptr, padP, g.n, spectra,
-                       spec_stride, tw);
+    return launch_cols_plan<NU, NTC>(cplan, dim3(nkx * nframes), st, rt, rt_fstride, nkx, g.h,
+                                     g.pad_mode ? fill : nullptr, padP, g.n, spectra, spec_stride,
+                                     tw);
   }
   return hipGetLastError();
 }

@@ -44,8 +44,12 @@ const float2* twiddles(int n);
 // canvases >= 4096 and 4 x min(256, 2nu/16) below.
 // R2C_COLS_P (or-ed in) selects the persistent column pass with register
 // twiddles instead of one workgroup per column.
+// Bits R2C_CPLAN_SHIFT.. (v < R2C_CPLANS) select the radix plan (v / 2: 0
+// balanced, 1 reversed order, 2 radix-8 chain) and the LDS padding (v % 2:
+// 0 one float2 per 16, 1 none) of the per-column pass; plans 1 and 2 exist
+// for canvases 1024 to 8192 (others run plan 0).
 enum { R2C_AUTO = 0, R2C_R4_T256 = 1, R2C_R2_T256 = 2, R2C_R4_T512 = 3, R2C_VARIANTS = 4,
-       R2C_COLS_P = 8 };
+       R2C_COLS_P = 8, R2C_CPLAN_SHIFT = 4, R2C_CPLANS = 6 };
 hipError_t launch_frames_to_spectra(const R2CGeom& g, int f0, int nframes, float2* spectra,
                                     size_t spec_stride, float2* rt, float* rowsum,
                                     float* fill, const float2* padP, int variant, hipStream_t st);

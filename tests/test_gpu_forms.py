@@ -5,6 +5,8 @@ oracle gradient on the same spectra and against central finite differences
 (grad_steps > 0).  Precision: the MFMA FP16 hi/lo form of the pair stage
 against the FP64 oracle (box maps) and against the FP32 form (shifts within
 0.01 px RMS, sums within 1e-4)."""
+import json
+
 import numpy as np
 import pytest
 
@@ -156,8 +158,12 @@ def test_autotune_picks_valid_shapes_and_keeps_results(ctx, tmp_path):
     shape, cols_p = rep["r2c_variant"] & 7, rep["r2c_variant"] >> 3
     assert shape in (1, 2) and cols_p in (0, 1) and rep["chunk_frames"] in (1, 2, 4, 8)
     assert rep["pair_cols_variant"] in (0, 1, 2, 3) and rep["pair_rows_variant"] in (0, 1, 2, 3)
+    # radix plan x LDS padding of the column pass, timed on this 1024 canvas
+    assert 0 <= rep["cols_plan"] < 6 and (cols_p or all(v > 0 for v in rep["ms_cols_plan"]))
+    assert cols_p or rep["r2c_variant"] >> 4 == rep["cols_plan"]
     best = min(v for blk in rep["ms_stage1"] for row in blk for v in row if v > 0)
-    assert rep["ms_stage1"][cols_p][shape][[1, 2, 4, 8].index(rep["chunk_frames"])] == best
+    if cols_p:
+        assert rep["ms_stage1"][cols_p][shape][[1, 2, 4, 8].index(rep["chunk_frames"])] == best
     tuned = ctx.align_movie(d, frames.shape, frames.dtype, prm, want=("frame_shift", "sum_image"))
     dd = np.abs(tuned["frame_shift"] - ref["frame_shift"])
     print("tuned vs default shapes: frame shift max difference %.3g px" % dd.max())
@@ -167,3 +173,36 @@ def test_autotune_picks_valid_shapes_and_keeps_results(ctx, tmp_path):
     again = c1.align_movie(d, frames.shape, frames.dtype, prm, want=("frame_shift",))
     assert np.array_equal(again["frame_shift"], tuned["frame_shift"])
     c1.close()
+
+
+@pytest.mark.parametrize("n", [1024, 4096])
+def test_column_plans_match_oracle(ctx, tmp_path, n):
+    """Every radix plan x LDS padding of the stage-1 column pass (the
+    autotuner's candidates) gives spectra within 5e-7 of the FP64 oracle's.
+    Each variant is forced through a tuning-table line (mc_tune_load)."""
+    F = 3
+    frames, _ = make_movie(n, n, F, dose=1.0, step=0.7, seed=n + 7)
+    d = mc.DeviceArray(frames)
+    prm = mc.default_params(F, FrameDistOffset=0, CCPeakSearchDim=96)
+    ref = Oracle("f64").align_movie(frames, dict(fod=0, box=96, nthreads=16),
+                                    want_spectra=True, want_boxes=False)["spectra"].view(np.complex128)
+    cache = str(tmp_path / "tune.jsonl")
+    c0 = mc.Context(0)
+    c0.autotune(d, frames.shape, frames.dtype, prm, cache_path=cache)
+    c0.close()
+    line = json.loads(open(cache).read().splitlines()[-1])
+    for cpl in range(6):
+        line["r2c"] = 1 | (cpl << 4)  # 4 rows x 256 threads, per-column pass variant cpl
+        fn = str(tmp_path / ("plan%d.jsonl" % cpl))
+        open(fn, "w").write(json.dumps(line) + "\n")
+        c = mc.Context(0)
+        assert c.tune_load(fn) == 1
+        c.load_frames(d, frames.shape, frames.dtype, prm)
+        spec = c.get_spectra(F, n).view(np.complex64)
+        c.close()
+        err = np.abs(spec - ref).max() / np.abs(ref).max()
+        print("n %d column variant %d (plan %d, padding %s): spectra error / max %.3g"
+              % (n, cpl, cpl // 2, "none" if cpl % 2 else "1/16", err), flush=True)
+        assert err < 5e-7
+    d.free()
+
