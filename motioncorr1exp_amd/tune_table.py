@@ -46,11 +46,11 @@ def main():
         dev, _ = make_movie_gpu(nx, ny, F, dose=1.0, step=0.7, seed=500 + i, dtype=dt)
         prm = mc.default_params(F, pad=pad, bin=bin_, FrameDistOffset=fod, CCPeakSearchDim=box)
         rep = ctx.autotune(dev, (F, ny, nx), dt, prm, cache_path=args.out)
-        print("%dx%dx%d %s pad %d bin %g: %s pair cols ms %s rows ms %s" % (
+        print("%dx%dx%d %s pad %d bin %g: %s pair cols ms %s rows ms %s column plans ms %s" % (
             nx, ny, F, np.dtype(dt).name, pad, bin_,
             {k: v for k, v in rep.items() if not k.startswith("ms_")},
-            ["%.3f" % v for v in rep["ms_pair_cols"]], ["%.3f" % v for v in rep["ms_pair_rows"]]),
-            flush=True)
+            ["%.3f" % v for v in rep["ms_pair_cols"]], ["%.3f" % v for v in rep["ms_pair_rows"]],
+            ["%.3f" % v for v in rep["ms_cols_plan"]]), flush=True)
         dev.free()
     ctx.close()
 
