@@ -249,21 +249,22 @@ int mc_align_movie(mc_context ctx, const mc_movie_desc *movie, const mc_params *
                    mc_outputs *out, mc_report *rep);
 
 /* Many movies (no reference counterpart: the reference aligns one stack per
- * process run, DFAlign.cu:423-453).  Movies with the same binned transform
- * size, peak box, B-factor, precision form and noise window form a group; a group is split
- * into sub-batches whose resident spectra fit in device memory
- * (MC_BATCH_MEM_MB, default 60% of the free memory).  In a sub-batch, stage 1
- * runs per movie; the pair column pass runs movie by movie (the frame
- * columns in flight then belong to one movie and stay in L2), and one row
- * pass and one peak search cover every pair of every movie; the drift model
- * runs per movie on the host; the align-to-sum rounds of the movies still
- * iterating are one batched sum launch plus one pair-stage sequence,
- * captured once as a HIP graph and replayed while that set of movies is
- * unchanged; the final sums run per movie.  Every movie gets bit for bit the
- * results mc_align_movie would give it (the per-pair arithmetic and its
- * summation order do not depend on the other movies); the ms_* fields of
- * its report are those of its sub-batch.  The movies' frame buffers must all
- * be resident. */
+ * process run, DFAlign.cu:423-453).  Movies with the same peak box,
+ * B-factor, precision form and noise window form a group, whatever their
+ * transform sizes; a group is split into sub-batches whose resident spectra
+ * and pair-stage buffers fit in device memory (MC_BATCH_MEM_MB, default 60%
+ * of the free memory).  In a sub-batch, stage 1 runs per movie; the pair
+ * column pass of every movie on the default column kernel is one persistent
+ * launch sequence across sizes (a plan table, one record per movie), the
+ * others keep per-movie launches; the row pass and the peak search run once
+ * per transform size; the drift model runs per movie on the host; each
+ * align-to-sum round of the movies still iterating (all sizes) is one HIP
+ * graph, captured once and replayed while that set of movies is unchanged;
+ * the final sums run per movie.  Every movie gets bit for bit the results
+ * mc_align_movie would give it: alone, a movie runs the same kernels (the
+ * default column kernel with its one plan), and the row pass's summation
+ * order depends on n only.  The ms_* fields of its report are those of its
+ * sub-batch.  The movies' frame buffers must all be resident. */
 typedef struct {
   const mc_movie_desc *movie;
   const mc_params *params;

@@ -92,7 +92,7 @@ struct mc_context_s {
   int device = 0;
   hipStream_t st = nullptr;
   DevBuf spec, rt, rowsum, fill, padP, pairs_d, tbuf, box, boxpart, peaks, shifts, image, tmp, means,
-      sum2img, wk, gpart, gout, jobs, tmax, fsc, disp;
+      sum2img, wk, gpart, gout, jobs, tmax, fsc, disp, plans;
   int graph_captures = 0, graph_replays = 0;
   // autotuned launch shapes (mc_autotune / mc_tune_load)
   struct Tune {
@@ -103,6 +103,7 @@ struct mc_context_s {
   // resident spectra
   int n = 0, nframe = 0, nslots = 0, off2 = 0, sum_slot = 0;
   size_t spec_stride = 0;
+  size_t spec_off = 0;  // float2 offset of slot 0: the current size group's region in a batch
   int padP_nu = -1, padP_w = -1, padP_nkx = -1;
   // kernel timing
   unsigned kt_mask = 0;  // bit g: kernel group g is timed
@@ -441,10 +442,13 @@ static int ensure_spectra(mc_context_s* c, int n, int nframe, int two) {
   c->off2 = two ? nframe : 0;
   c->sum_slot = nslots - 1;
   c->spec_stride = stride;
+  c->spec_off = 0;
   return MC_OK;
 }
 
-static float2* slot(mc_context_s* c, int s) { return c->spec.as<float2>() + c->spec_stride * s; }
+static float2* slot(mc_context_s* c, int s) {
+  return c->spec.as<float2>() + c->spec_off + c->spec_stride * s;
+}
 
 static const mc_context_s::Tune* stage1_tune(const mc_context_s* c, int nu, int n, int dtype) {
   for (const auto& t : c->tunes)
@@ -551,33 +555,19 @@ static int pair_reserve(mc_context_s* c, int np, int n, int box, int precision) 
   return MC_OK;
 }
 
-// Column pass, row pass and peak search over the np slot pairs in pairs_d;
-// results stay on the device (peaks, box maps).  Capturable.
-// groups: pair counts of consecutive movies (a batch).  The column pass then
-// runs movie by movie, so that the frame columns its items read at a time
-// belong to one movie and stay in L2; the per-pair arithmetic is the same.
+// Column pass, row pass and peak search over the np slot pairs in pairs_d
+// (one movie's region); results stay on the device (peaks, box maps).
+// Capturable.
 static int pair_launch(mc_context_s* c, int np, int n, float bfactor, int box, int wnoise,
-                       int precision, int cols = -1, int rows = -1,
-                       const std::vector<int>* groups = nullptr) {
+                       int precision, int cols = -1, int rows = -1) {
   const mc_context_s::Tune* tn = pair_tune(c, n, box, precision);
   if (cols < 0) cols = tn ? tn->cols : mc::PAIR_COLS_AUTO;
   if (rows < 0) rows = tn ? tn->rows : mc::PAIR_ROWS_AUTO;
   {
     KScope ks(c, MC_KT_PAIR_COLS);
-    const int H = n / 2 + 1;
-    std::vector<int> one(1, np);
-    const std::vector<int>& gs = groups && groups->size() > 1 ? *groups : one;
-    int p0 = 0;
-    for (int g : gs) {
-      if (g <= 0) continue;
-      float* tmax = precision == MC_PREC_MFMA_HILO ? c->tmax.as<float>() + (size_t)p0 * H : nullptr;
-      HIP_TRY(mc::launch_pair_cols(c->spec.as<float2>(), c->spec_stride,
-                                   c->pairs_d.as<int2>() + p0, g, n, bfactor, box,
-                                   c->tbuf.as<float2>() + (size_t)p0 * H * box, tmax, precision,
-                                   cols, c->st));
-      p0 += g;
-    }
-    if (p0 != np) return set_err(MC_ERR_ARG, "pair groups cover %d of %d pairs", p0, np);
+    HIP_TRY(mc::launch_pair_cols(slot(c, 0), c->spec_stride, c->pairs_d.as<int2>(), np, n, bfactor,
+                                 box, c->tbuf.as<float2>(), c->tmax.as<float>(), precision, cols,
+                                 c->st));
   }
   {
     KScope ks(c, MC_KT_PAIR_ROWS);
@@ -595,14 +585,13 @@ static int pair_launch(mc_context_s* c, int np, int n, float bfactor, int box, i
 
 // Pair stage on resident slots: pairs (a, b) are slot indices.
 static int pair_stage(mc_context_s* c, const std::vector<int>& slots, float bfactor, int box,
-                      int wnoise, int precision, std::vector<double>& res, float* boxmaps_host,
-                      const std::vector<int>* groups = nullptr) {
+                      int wnoise, int precision, std::vector<double>& res, float* boxmaps_host) {
   const int np = (int)slots.size() / 2;
   const int n = c->n;
   MC_TRY(pair_reserve(c, np, n, box, precision));
   HIP_TRY(hipMemcpyAsync(c->pairs_d.p, slots.data(), sizeof(int) * slots.size(),
                          hipMemcpyHostToDevice, c->st));
-  MC_TRY(pair_launch(c, np, n, bfactor, box, wnoise, precision, -1, -1, groups));
+  MC_TRY(pair_launch(c, np, n, bfactor, box, wnoise, precision));
   res.resize(3 * (size_t)np);
   HIP_TRY(hipMemcpyAsync(res.data(), c->peaks.p, sizeof(double) * 3 * np, hipMemcpyDeviceToHost,
                          c->st));
@@ -1207,6 +1196,155 @@ static int final_sum(mc_context_s* c, MovieRun& r) {
   return MC_OK;
 }
 
+// Movies of one transform size within an alignment group: their slots form
+// one region of the spectrum buffer with a uniform stride.
+struct SizeGroup {
+  int n = 0;
+  size_t stride = 0, off = 0;  // float2 per slot; region start in spec
+  int nslots = 0;
+  std::vector<MovieRun*> rs;
+};
+
+static void use_group(mc_context_s* c, const SizeGroup& g) {
+  c->n = g.n;
+  c->spec_stride = g.stride;
+  c->spec_off = g.off;
+}
+
+// The pair stage of several movies (each of its own size group): span i has
+// np pairs of slot indices of its group's region, stored in pairs_d from pair
+// po, with its T columns from tbuf element to.  Built and uploaded outside
+// any graph capture; pairs_launch below only launches.
+struct PairSpan {
+  int gi = 0, np = 0;
+  size_t po = 0, to = 0;
+  int cols = mc::PAIR_COLS_AUTO, body = -1;
+};
+struct PairPlan {
+  std::vector<PairSpan> sp;
+  size_t np = 0;
+  long long items[2] = {0, 0};  // plan-kernel items of each register class
+  int nplans[2] = {0, 0};       // class-0 table, then the class-1 table
+  float bfactor = 0.f;
+  int box = 0, wnoise = 0, precision = 0;
+};
+
+// Reserves the pair buffers and uploads pairs (host slot pairs, span by span)
+// and the column-pass plan table.  With more than one span the spans that
+// the plan kernel covers run as one launch sequence (mc::PairColsPlan).
+static int pairs_prepare(mc_context_s* c, const std::vector<SizeGroup>& gs, PairPlan& L,
+                         const std::vector<int>& slots) {
+  size_t np = 0, tel = 0, part = 0, pscr = 0;
+  std::vector<size_t> gnp(gs.size(), 0);
+  for (PairSpan& q : L.sp) {
+    const int n = gs[q.gi].n;
+    q.po = np;
+    q.to = tel;
+    np += q.np;
+    tel += (size_t)q.np * (n / 2 + 1) * L.box;
+    gnp[q.gi] += q.np;
+    const mc_context_s::Tune* tn = pair_tune(c, n, L.box, L.precision);
+    q.cols = tn ? tn->cols : mc::PAIR_COLS_AUTO;
+    // several movies: the default-variant ones share one plan table (a
+    // single movie runs the same plan kernel with its plan passed by value)
+    q.body = (L.precision == MC_PREC_FP32 && L.sp.size() > 1)
+                 ? mc::pair_cols_plan_body(n, L.box, q.cols) : -1;
+  }
+  for (size_t g = 0; g < gs.size(); g++) {
+    part = std::max(part, mc::pair_rows_part_elems((int)gnp[g], gs[g].n, L.box));
+    pscr = std::max(pscr, mc::find_peaks_out_elems((int)gnp[g]) - 3 * gnp[g]);
+  }
+  if (slots.size() != 2 * np) return set_err(MC_ERR_ARG, "pair spans cover %zu of %zu pairs", np, slots.size() / 2);
+  L.np = np;
+  HIP_TRY(c->pairs_d.ensure(sizeof(int) * 2 * std::max<size_t>(1, np)));
+  HIP_TRY(c->tbuf.ensure(sizeof(float2) * std::max<size_t>(1, tel)));
+  if (L.precision == MC_PREC_MFMA_HILO) HIP_TRY(c->tmax.ensure(sizeof(float) * std::max<size_t>(1, tel / L.box)));
+  HIP_TRY(c->box.ensure(sizeof(float) * std::max<size_t>(1, np) * L.box * L.box));
+  HIP_TRY(c->boxpart.ensure(sizeof(float) * std::max<size_t>(1, part)));
+  HIP_TRY(c->peaks.ensure(sizeof(double) * (3 * np + pscr + 1)));
+  HIP_TRY(hipMemcpyAsync(c->pairs_d.p, slots.data(), sizeof(int) * slots.size(), hipMemcpyHostToDevice,
+                         c->st));
+  std::vector<mc::PairColsPlan> plans[2];
+  for (int k = 0; k < 2; k++) L.items[k] = 0;
+  for (const PairSpan& q : L.sp) {
+    if (q.body < 0 || q.np == 0) continue;
+    const SizeGroup& g = gs[q.gi];
+    const int k = mc::pair_cols_plan_class(q.body);
+    mc::PairColsPlan pl;
+    pl.spectra = c->spec.as<float2>() + g.off;
+    pl.pairs = c->pairs_d.as<int2>() + q.po;
+    pl.wxy = mc::pair_weights(g.n, L.bfactor);
+    pl.tw = mc::twiddles(g.n);
+    if (!pl.wxy || !pl.tw) return set_err(MC_ERR_DEVICE, "pair tables for n %d", g.n);
+    pl.tbuf = c->tbuf.as<float2>() + q.to;
+    pl.spec_stride = g.stride;
+    pl.item0 = L.items[k];
+    pl.npairs = q.np;
+    pl.box = L.box;
+    pl.body = q.body;
+    plans[k].push_back(pl);
+    L.items[k] += (long long)q.np * (g.n / 2 + 1);
+  }
+  for (int k = 0; k < 2; k++) L.nplans[k] = (int)plans[k].size();
+  plans[0].insert(plans[0].end(), plans[1].begin(), plans[1].end());
+  if (!plans[0].empty()) {
+    HIP_TRY(c->plans.ensure(sizeof(mc::PairColsPlan) * plans[0].size()));
+    HIP_TRY(hipMemcpyAsync(c->plans.p, plans[0].data(), sizeof(mc::PairColsPlan) * plans[0].size(),
+                           hipMemcpyHostToDevice, c->st));
+    // the host vector goes out of scope: the copy must have read it
+    HIP_TRY(hipStreamSynchronize(c->st));
+  }
+  return MC_OK;
+}
+
+// Column pass (plan kernel and per-span launches), then the row pass and the
+// peak search per size group.  Results: peaks (3 doubles per pair, span
+// order) and box maps.  Capturable.
+static int pairs_launch(mc_context_s* c, const std::vector<SizeGroup>& gs, const PairPlan& L) {
+  const int box = L.box;
+  {
+    KScope ks(c, MC_KT_PAIR_COLS);
+    for (int k = 0; k < 2; k++)
+      if (L.nplans[k] > 0)
+        HIP_TRY(mc::launch_pair_cols_plans(c->plans.as<mc::PairColsPlan>() + (k ? L.nplans[0] : 0),
+                                           L.nplans[k], L.items[k], k, c->st));
+    for (const PairSpan& q : L.sp) {
+      if (q.body >= 0 || q.np == 0) continue;
+      const SizeGroup& g = gs[q.gi];
+      float* tmax = L.precision == MC_PREC_MFMA_HILO ? c->tmax.as<float>() + q.to / box : nullptr;
+      HIP_TRY(mc::launch_pair_cols(c->spec.as<float2>() + g.off, g.stride,
+                                   c->pairs_d.as<int2>() + q.po, q.np, g.n, L.bfactor, box,
+                                   c->tbuf.as<float2>() + q.to, tmax, L.precision, q.cols, c->st));
+    }
+  }
+  // consecutive spans of one size group share a row-pass and a peak launch
+  size_t i = 0;
+  while (i < L.sp.size()) {
+    size_t j = i;
+    int np = 0;
+    while (j < L.sp.size() && L.sp[j].gi == L.sp[i].gi) np += L.sp[j++].np;
+    const PairSpan& q = L.sp[i];
+    const int n = gs[q.gi].n;
+    i = j;
+    if (np == 0) continue;
+    const mc_context_s::Tune* tn = pair_tune(c, n, box, L.precision);
+    const int rows = tn ? tn->rows : mc::PAIR_ROWS_AUTO;
+    float* maps = c->box.as<float>() + q.po * box * box;
+    {
+      KScope ks(c, MC_KT_PAIR_ROWS);
+      float* tmax = L.precision == MC_PREC_MFMA_HILO ? c->tmax.as<float>() + q.to / box : nullptr;
+      HIP_TRY(mc::launch_pair_rows(c->tbuf.as<float2>() + q.to, tmax, np, n, box, maps,
+                                   c->boxpart.as<float>(), L.precision, rows, c->st));
+    }
+    {
+      KScope ks(c, MC_KT_PEAKS);
+      HIP_TRY(mc::launch_find_peaks(maps, np, box, n, L.wnoise, c->peaks.as<double>() + 3 * q.po,
+                                    c->st, c->peaks.as<double>() + 3 * L.np));
+    }
+  }
+  return MC_OK;
+}
+
 // Graph of one align-to-sum round of a set of movies: the batched sums, the
 // pair column/row passes and the peak search, replayed while the set of
 // iterating movies is unchanged.  Only kernel launches are captured: the
@@ -1226,18 +1364,28 @@ static bool graphs_enabled() {
   return !(e && atoi(e));
 }
 
-static int round_launch(mc_context_s* c, int njobs, int np, int n, float bfactor, int box,
-                        int wnoise, int precision) {
+// One round: per size group the batched shift-and-sum of its iterating
+// movies (jobs [j0, j0 + nj) of the job table), then the pair stage.
+struct RoundJobs {
+  int gi, j0, nj;
+};
+static int round_launch(mc_context_s* c, const std::vector<SizeGroup>& gs,
+                        const std::vector<RoundJobs>& rj, const PairPlan& L) {
   {
     KScope ks(c, MC_KT_SUM);
-    HIP_TRY(mc::launch_shift_sum_multi(c->spec.as<float2>(), c->spec_stride, n,
-                                       c->jobs.as<int4>(), njobs, c->shifts.as<double2>(), c->st));
+    for (const RoundJobs& q : rj) {
+      const SizeGroup& g = gs[q.gi];
+      HIP_TRY(mc::launch_shift_sum_multi(c->spec.as<float2>() + g.off, g.stride, g.n,
+                                         c->jobs.as<int4>() + q.j0, q.nj, c->shifts.as<double2>(),
+                                         c->st));
+    }
   }
-  return pair_launch(c, np, n, bfactor, box, wnoise, precision);
+  return pairs_launch(c, gs, L);
 }
 
-static int round_graph(mc_context_s* c, RoundGraph& gr, const std::vector<int>& key, int njobs,
-                       int np, int n, float bfactor, int box, int wnoise, int precision) {
+static int round_graph(mc_context_s* c, RoundGraph& gr, const std::vector<int>& key,
+                       const std::vector<SizeGroup>& gs, const std::vector<RoundJobs>& rj,
+                       const PairPlan& L) {
   if (gr.exec && gr.key == key) return MC_OK;
   if (gr.exec) {
     (void)hipGraphExecDestroy(gr.exec);
@@ -1246,7 +1394,7 @@ static int round_graph(mc_context_s* c, RoundGraph& gr, const std::vector<int>& 
   const unsigned kt = c->kt_mask;
   c->kt_mask = 0;  // no event records inside the graph
   HIP_TRY(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-  const int rc = round_launch(c, njobs, np, n, bfactor, box, wnoise, precision);
+  const int rc = round_launch(c, gs, rj, L);
   hipGraph_t graph = nullptr;
   const hipError_t ec = hipStreamEndCapture(c->st, &graph);
   c->kt_mask = kt;
@@ -1267,29 +1415,47 @@ struct GroupTimes {
   float spectra = 0, pairs = 0, solve = 0, iter = 0, sum = 0, total = 0;
 };
 
-// Aligns movies of one transform size, peak box, B-factor and noise window
-// whose spectra fit together in device memory: stage 1 per movie, then one
-// pair-stage launch over every pair of every movie, the host drift model per
-// movie, align-to-sum rounds batched over the movies still iterating (one
-// graph replay per round), and the final sum per movie.
+// Aligns movies with the same peak box, B-factor, noise window and form,
+// of any transform sizes, whose spectra fit together in device memory
+// (callers order them by size): stage 1 per movie into its size group's
+// region, then one pair stage over every pair of every movie (the column
+// pass as one plan-kernel launch sequence across sizes when there are
+// several movies), the host drift model per movie, align-to-sum rounds over
+// the movies still iterating (one graph replay per round), and the final sum
+// per movie.
 static int align_group(mc_context_s* c, std::vector<MovieRun*>& rs, bool use_graph,
                        GroupTimes* tm) {
-  const int n = rs[0]->g.n;
   const int box = rs[0]->box, wnoise = rs[0]->wnoise;
   const float bfactor = rs[0]->p->bfactor;
   const int precision = rs[0]->p->precision;
-  const size_t stride = (size_t)(n / 2 + 1) * n;
-  int nslots = 0, ftot = 0;
-  for (MovieRun* r : rs) {
-    r->base = nslots;
+  std::vector<SizeGroup> gs;
+  std::vector<int> gi_of(rs.size());
+  size_t total = 0;
+  int ftot = 0, nmax = 0;
+  for (size_t i = 0; i < rs.size(); i++) {
+    MovieRun* r = rs[i];
+    size_t k = 0;
+    while (k < gs.size() && gs[k].n != r->g.n) k++;
+    if (k == gs.size()) {
+      gs.emplace_back();
+      gs.back().n = r->g.n;
+      gs.back().stride = (size_t)(r->g.n / 2 + 1) * r->g.n;
+    }
+    gi_of[i] = (int)k;
+    SizeGroup& g = gs[k];
+    r->base = g.nslots;
     r->off2 = (r->m->frames2 || (r->m->corr && r->m->corr->second)) ? r->F : 0;
     r->sum = r->base + r->nslot - 1;
-    nslots += r->nslot;
+    g.nslots += r->nslot;
+    g.rs.push_back(r);
     ftot += r->F;
+    nmax = std::max(nmax, r->g.n);
   }
-  HIP_TRY(c->spec.ensure(sizeof(float2) * stride * nslots));
-  c->n = n;
-  c->spec_stride = stride;
+  for (SizeGroup& g : gs) {
+    g.off = total;
+    total += g.stride * g.nslots;
+  }
+  HIP_TRY(c->spec.ensure(sizeof(float2) * total));
   c->nframe = 0;  // resident single-movie state is rebuilt below
   hipEvent_t* ev = c->gev;
   for (int i = 0; i < 6; i++)
@@ -1300,7 +1466,9 @@ static int align_group(mc_context_s* c, std::vector<MovieRun*>& rs, bool use_gra
   HIP_TRY(c->means.ensure(sizeof(double) * ftot));
   {
     KScope ks(c, MC_KT_SPECTRA);
-    for (MovieRun* r : rs) {
+    for (size_t i = 0; i < rs.size(); i++) {
+      MovieRun* r = rs[i];
+      use_group(c, gs[gi_of[i]]);
       const int first = r->p->nStart - r->m->first_frame;
       MC_TRY(run_stage1(c, r->m->frames, r->m, r->p, r->g, first, r->F, r->base));
       if (r->m->frames2)
@@ -1312,18 +1480,25 @@ static int align_group(mc_context_s* c, std::vector<MovieRun*>& rs, bool use_gra
   }
   {
     int mo = 0;
-    for (MovieRun* r : rs) {
+    for (size_t i = 0; i < rs.size(); i++) {
+      MovieRun* r = rs[i];
+      use_group(c, gs[gi_of[i]]);
+      const int n = r->g.n;
       double* md = c->means.as<double>() + mo;
-      HIP_TRY(mc::launch_dc_gather(slot(c, r->base), stride, r->F, 1.0 / ((double)n * n), md, c->st));
+      HIP_TRY(mc::launch_dc_gather(slot(c, r->base), c->spec_stride, r->F, 1.0 / ((double)n * n), md,
+                                   c->st));
       if (r->out->frame_mean)
         HIP_TRY(hipMemcpyAsync(r->out->frame_mean, md, sizeof(double) * r->F, hipMemcpyDeviceToHost,
                                c->st));
       mo += r->F;
     }
   }
-  HIP_TRY(c->image.ensure(sizeof(float) * (size_t)n * n));
-  for (MovieRun* r : rs) {
+  HIP_TRY(c->image.ensure(sizeof(float) * (size_t)nmax * nmax));
+  for (size_t i = 0; i < rs.size(); i++) {
+    MovieRun* r = rs[i];
     if (!r->out->rawsum_image && !r->out->disp_raw_fft) continue;
+    use_group(c, gs[gi_of[i]]);
+    const int n = r->g.n;
     // DFAlign.cu:808-823,853-857: unaligned sum of frames [nStartSum, nEndSum]
     r->sh.assign(2 * (size_t)r->F, NAN);
     for (int f = 0; f < r->F; f++) {
@@ -1343,18 +1518,30 @@ static int align_group(mc_context_s* c, std::vector<MovieRun*>& rs, bool use_gra
   }
   HIP_TRY(hipEventRecord(ev[1], c->st));
 
-  // 2. pairwise shifts of every movie (DFAlign.cu:875-905): the column pass
-  // movie by movie, the row pass and the peak search over all pairs at once
+  // 2. pairwise shifts of every movie (DFAlign.cu:875-905)
+  PairPlan L;
+  L.bfactor = bfactor;
+  L.box = box;
+  L.wnoise = wnoise;
+  L.precision = precision;
   std::vector<int> slots;
-  for (MovieRun* r : rs)
-    for (int q = 0; q < r->P; q++) {
-      slots.push_back(r->base + r->pairs[2 * q]);
-      slots.push_back(r->base + r->off2 + r->pairs[2 * q + 1]);
+  for (size_t i = 0; i < rs.size(); i++) {
+    MovieRun* r = rs[i];
+    PairSpan q;
+    q.gi = gi_of[i];
+    q.np = r->P;
+    L.sp.push_back(q);
+    for (int k = 0; k < r->P; k++) {
+      slots.push_back(r->base + r->pairs[2 * k]);
+      slots.push_back(r->base + r->off2 + r->pairs[2 * k + 1]);
     }
-  std::vector<double> res;
-  std::vector<int> groups;
-  for (MovieRun* r : rs) groups.push_back(r->P);
-  MC_TRY(pair_stage(c, slots, bfactor, box, wnoise, precision, res, nullptr, &groups));
+  }
+  MC_TRY(pairs_prepare(c, gs, L, slots));
+  MC_TRY(pairs_launch(c, gs, L));
+  std::vector<double> res(3 * L.np);
+  if (L.np)
+    HIP_TRY(hipMemcpyAsync(res.data(), c->peaks.p, sizeof(double) * 3 * L.np, hipMemcpyDeviceToHost,
+                           c->st));
   HIP_TRY(hipEventRecord(ev[2], c->st));
   {
     size_t po = 0;
@@ -1362,56 +1549,76 @@ static int align_group(mc_context_s* c, std::vector<MovieRun*>& rs, bool use_gra
       if (r->out->boxmaps)
         HIP_TRY(hipMemcpyAsync(r->out->boxmaps, c->box.as<float>() + po * box * box,
                                sizeof(float) * (size_t)r->P * box * box, hipMemcpyDeviceToHost, c->st));
+      po += r->P;
+    }
+    HIP_TRY(hipStreamSynchronize(c->st));
+    po = 0;
+    for (MovieRun* r : rs) {
       // 3. least squares, outlier rejection, least squares (DFAlign.cu:914-948)
       MC_TRY(solve_drift(*r, res.data() + 3 * po));
       po += r->P;
     }
-    HIP_TRY(hipStreamSynchronize(c->st));
   }
   HIP_TRY(hipEventRecord(ev[3], c->st));
 
   // 4. iterative alignment to the sum (DFAlign.cu:950-1071), batched
   RoundGraph gr;
   for (;;) {
-    std::vector<MovieRun*> act;
-    for (MovieRun* r : rs)
-      if (r->iterating()) act.push_back(r);
-    if (act.empty()) break;
+    // iterating movies, size group by size group
     std::vector<int> key, jobs, islots;
+    std::vector<RoundJobs> rj;
     std::vector<double> shifts;
-    for (MovieRun* r : act) {
-      key.push_back(r->base);
-      iter_shifts(*r);
-      jobs.insert(jobs.end(), {r->base, r->F, (int)(shifts.size() / 2), r->sum});
-      shifts.insert(shifts.end(), r->sh.begin(), r->sh.end());
-      for (int k = 0; k < r->F; k++) {
-        islots.push_back(r->sum);  // GPUShiftCC(dfft=frame, dsum=sum): sum * conj(frame)
-        islots.push_back(r->base + k);
+    std::vector<MovieRun*> act;
+    PairPlan R;
+    R.bfactor = bfactor;
+    R.box = box;
+    R.wnoise = wnoise;
+    R.precision = precision;
+    for (size_t g = 0; g < gs.size(); g++) {
+      RoundJobs q{(int)g, (int)(jobs.size() / 4), 0};
+      for (MovieRun* r : gs[g].rs) {
+        if (!r->iterating()) continue;
+        act.push_back(r);
+        key.insert(key.end(), {(int)g, r->base});
+        iter_shifts(*r);
+        jobs.insert(jobs.end(), {r->base, r->F, (int)(shifts.size() / 2), r->sum});
+        shifts.insert(shifts.end(), r->sh.begin(), r->sh.end());
+        for (int k = 0; k < r->F; k++) {
+          islots.push_back(r->sum);  // GPUShiftCC(dfft=frame, dsum=sum): sum * conj(frame)
+          islots.push_back(r->base + k);
+        }
+        PairSpan s;
+        s.gi = (int)g;
+        s.np = r->F;
+        R.sp.push_back(s);
+        q.nj++;
       }
+      if (q.nj) rj.push_back(q);
     }
-    const int njobs = (int)act.size(), np = (int)islots.size() / 2;
+    if (act.empty()) break;
     const bool fresh = !(gr.exec && gr.key == key);
     if (fresh) {
-      MC_TRY(pair_reserve(c, np, n, box, precision));
+      MC_TRY(pairs_prepare(c, gs, R, islots));
       HIP_TRY(c->jobs.ensure(sizeof(int) * jobs.size()));
       HIP_TRY(hipMemcpyAsync(c->jobs.p, jobs.data(), sizeof(int) * jobs.size(),
                              hipMemcpyHostToDevice, c->st));
-      HIP_TRY(hipMemcpyAsync(c->pairs_d.p, islots.data(), sizeof(int) * islots.size(),
-                             hipMemcpyHostToDevice, c->st));
       HIP_TRY(c->shifts.ensure(sizeof(double) * shifts.size()));
+    } else {
+      R.np = islots.size() / 2;
     }
     HIP_TRY(hipMemcpyAsync(c->shifts.p, shifts.data(), sizeof(double) * shifts.size(),
                            hipMemcpyHostToDevice, c->st));
     if (use_graph) {
-      MC_TRY(round_graph(c, gr, key, njobs, np, n, bfactor, box, wnoise, precision));
+      MC_TRY(round_graph(c, gr, key, gs, rj, R));
       KScope ks(c, MC_KT_ROUNDS);
       HIP_TRY(hipGraphLaunch(gr.exec, c->st));
       c->graph_replays++;
     } else {
       gr.key = key;  // buffers uploaded for this set
-      MC_TRY(round_launch(c, njobs, np, n, bfactor, box, wnoise, precision));
+      MC_TRY(round_launch(c, gs, rj, R));
     }
-    res.resize(3 * (size_t)np);
+    const size_t np = islots.size() / 2;
+    res.resize(3 * np);
     HIP_TRY(hipMemcpyAsync(res.data(), c->peaks.p, sizeof(double) * 3 * np, hipMemcpyDeviceToHost,
                            c->st));
     size_t po = 0;
@@ -1432,7 +1639,10 @@ static int align_group(mc_context_s* c, std::vector<MovieRun*>& rs, bool use_gra
   HIP_TRY(hipEventRecord(ev[4], c->st));
 
   // 5. final phase-shift-and-sum per movie (DFAlign.cu:1114-1292)
-  for (MovieRun* r : rs) MC_TRY(final_sum(c, *r));
+  for (size_t i = 0; i < rs.size(); i++) {
+    use_group(c, gs[gi_of[i]]);
+    MC_TRY(final_sum(c, *rs[i]));
+  }
   HIP_TRY(hipEventRecord(ev[5], c->st));
   HIP_TRY(hipEventSynchronize(ev[5]));
   tm->spectra = elapsed(ev[0], ev[1]);
@@ -1529,13 +1739,13 @@ extern "C" int mc_align_movie(mc_context ctx, const mc_movie_desc* m, const mc_p
   return MC_OK;
 }
 
-// Device memory the batch may hold for resident spectra: MC_BATCH_MEM_MB, or
-// 60% of the free memory at the call.
+// Device memory a sub-batch may hold for resident spectra and its pair
+// stage: MC_BATCH_MEM_MB, or 60% of the free memory at the call.
 static size_t batch_budget(mc_context_s* c) {
   if (getenv("MC_BATCH_MEM_MB")) return (size_t)atoll(getenv("MC_BATCH_MEM_MB")) << 20;
   size_t fr = 0, tot = 0;
   if (hipMemGetInfo(&fr, &tot) != hipSuccess) return (size_t)8 << 30;
-  return fr / 10 * 6 + c->spec.cap;
+  return fr / 10 * 6 + c->spec.cap + c->tbuf.cap + c->box.cap;
 }
 
 extern "C" int mc_align_batch(mc_context ctx, const mc_batch_item* items, int count,
@@ -1557,19 +1767,22 @@ extern "C" int mc_align_batch(mc_context ctx, const mc_batch_item* items, int co
       return set_err(rc, "batch item %d: %s", i, msg.c_str());
     }
   }
-  // groups: same transform size, box, B-factor and noise window, larger
-  // transforms first; each group split into sub-batches that fit the budget
+  // groups: same box, B-factor, precision form and noise window (any
+  // transform sizes), larger transforms first; each group split into
+  // sub-batches whose spectra fit the budget
   std::vector<int> order(count);
   for (int i = 0; i < count; i++) order[i] = i;
-  auto key_less = [&](int a, int b) {
+  auto group_less = [&](int a, int b) {
     const MovieRun &x = runs[a], &y = runs[b];
-    if (x.g.n != y.g.n) return x.g.n > y.g.n;
     if (x.box != y.box) return x.box < y.box;
     if (x.p->bfactor != y.p->bfactor) return x.p->bfactor < y.p->bfactor;
     if (x.p->precision != y.p->precision) return x.p->precision < y.p->precision;
     return x.wnoise < y.wnoise;
   };
-  std::stable_sort(order.begin(), order.end(), key_less);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    if (group_less(a, b) || group_less(b, a)) return group_less(a, b);
+    return runs[a].g.n > runs[b].g.n;
+  });
   const size_t budget = batch_budget(ctx);
   const int cap0 = ctx->graph_captures, rep0 = ctx->graph_replays;
   int groups = 0, subs = 0;
@@ -1577,15 +1790,18 @@ extern "C" int mc_align_batch(mc_context ctx, const mc_batch_item* items, int co
   size_t i = 0;
   while (i < order.size()) {
     size_t j = i + 1;
-    while (j < order.size() && !key_less(order[i], order[j]) && !key_less(order[j], order[i])) j++;
+    while (j < order.size() && !group_less(order[i], order[j]) && !group_less(order[j], order[i])) j++;
     groups++;
-    const size_t slot_bytes = sizeof(float2) * (size_t)(runs[order[i]].g.n / 2 + 1) * runs[order[i]].g.n;
     size_t k = i;
     while (k < j) {
       std::vector<MovieRun*> rs;
       size_t bytes = 0;
       while (k < j) {
-        const size_t need = slot_bytes * runs[order[k]].nslot;
+        const MovieRun& r = runs[order[k]];
+        // resident spectra, the pair stage's T columns and box maps
+        const size_t H = (size_t)(r.g.n / 2 + 1);
+        const size_t need = sizeof(float2) * H * r.g.n * r.nslot +
+                            sizeof(float2) * H * r.box * r.P + sizeof(float) * r.box * r.box * r.P;
         if (!rs.empty() && bytes + need > budget) break;
         rs.push_back(&runs[order[k]]);
         bytes += need;
@@ -1675,6 +1891,7 @@ extern "C" int mc_autotune(mc_context ctx, const mc_movie_desc* m, const mc_para
   HIP_TRY(c->spec.ensure(sizeof(float2) * stride * (F + 1)));
   c->n = n;
   c->spec_stride = stride;
+  c->spec_off = 0;
   c->nframe = 0;
   const int reps = 3;
   mc_tune_report tr;
@@ -1733,7 +1950,7 @@ extern "C" int mc_autotune(mc_context ctx, const mc_movie_desc* m, const mc_para
       if (v != mc::PAIR_COLS_GENERIC && !(n >= 512 && box <= 128)) continue;
       float ms = 0;
       MC_TRY(time_min(c, reps, &ms, [&] {
-        const hipError_t e = mc::launch_pair_cols(c->spec.as<float2>(), c->spec_stride,
+        const hipError_t e = mc::launch_pair_cols(slot(c, 0), c->spec_stride,
                                                   c->pairs_d.as<int2>(), r.P, n, p->bfactor, box,
                                                   c->tbuf.as<float2>(), c->tmax.as<float>(),
                                                   p->precision, v, c->st);
@@ -1745,6 +1962,10 @@ extern "C" int mc_autotune(mc_context ctx, const mc_movie_desc* m, const mc_para
         bcol = v;
       }
     }
+    // The default variant is the plan kernel that mc_align_batch shares across
+    // movies: it is kept unless another one is more than 2% faster.
+    if (tr.ms_pair_cols[mc::PAIR_COLS_AUTO] > 0 && tr.ms_pair_cols[mc::PAIR_COLS_AUTO] <= 1.02f * best)
+      bcol = mc::PAIR_COLS_AUTO;
     best = 1e30f;
     for (int v = 0; v < mc::PAIR_ROWS_VARIANTS; v++) {
       if (v != mc::PAIR_ROWS_VALU && box > 96) continue;

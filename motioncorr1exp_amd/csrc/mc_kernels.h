@@ -103,6 +103,31 @@ bool pair_form_supported(int precision, int n, int box);
 hipError_t launch_pair_cols(const float2* spectra, size_t spec_stride, const int2* pairs,
                             int npairs, int n, float bfactor, int box, float2* tbuf,
                             float* tmax, int precision, int variant, hipStream_t st);
+// One movie of a batched column pass: its (pair, kx) items are the global
+// indices [item0, item0 + npairs (n/2+1)), kx major; plans are listed in
+// increasing item0.
+struct PairColsPlan {
+  const float2* spectra;  // the movie's slot 0
+  const int2* pairs;      // movie-local slot pairs
+  const float* wxy;       // pair_weights(n, B-factor)
+  const float2* tw;       // twiddles(n)
+  float2* tbuf;           // T columns of the movie's pairs, [pair][kx][box]
+  unsigned long long spec_stride;
+  long long item0;
+  int npairs, box;
+  int body;               // pair_cols_plan_body(n, box, variant)
+};
+// The plan-kernel body launch_pair_cols(n, box, variant) runs at precision 0:
+// the default variant (PAIR_COLS_AUTO) of the M = 128 decimation at n in
+// [512, 4096] and box <= 128; -1 for other variants and sizes, which keep
+// their own kernels.
+int pair_cols_plan_body(int n, int box, int variant);
+// Register class of a body: 0 runs at 4 workgroups per CU, 1 at 2.  A plan
+// table holds the movies of one class; each class is its own launch sequence.
+int pair_cols_plan_class(int body);
+const float* pair_weights(int n, float bfactor);
+hipError_t launch_pair_cols_plans(const PairColsPlan* plans, int nplans, long long items, int cls,
+                                  hipStream_t st);
 size_t pair_rows_part_elems(int npairs, int n, int box);
 hipError_t launch_pair_rows(const float2* tbuf, const float* tmax, int npairs, int n, int box,
                             float* boxmaps, float* part, int precision, int variant,
@@ -111,10 +136,11 @@ hipError_t launch_pair_rows(const float2* tbuf, const float* tmax, int npairs, i
 // Peaks: integer peak + FP64 16x16 -> 512x512 sub-pixel search per box map.
 // out: 3 doubles per pair (sx, sy, cc), followed by per-slice scratch when
 // the maps of few pairs are split over several workgroups:
-// find_peaks_out_elems(npairs) doubles in all.
+// find_peaks_out_elems(npairs) doubles in all.  With `part` the scratch
+// (find_peaks_out_elems(npairs) - 3 npairs doubles) is there instead.
 size_t find_peaks_out_elems(int npairs);
 hipError_t launch_find_peaks(const float* boxmaps, int npairs, int box, int n, int wnoise,
-                             double* out, hipStream_t st);
+                             double* out, hipStream_t st, double* part = nullptr);
 
 // Phase-shift-and-sum: sum[kx][ky] = sum_f include_f * S_f[kx][ky] *
 // exp(i 2 pi (sx_f kx + sy_f ky')/n).  shifts: device double2 per frame;

@@ -47,54 +47,65 @@ typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 // HBM once into that XCD's L2 and reused by all pairs.  Per-workgroup
 // constants (M-point twiddles in LDS; ky weights, combine twiddles in
 // registers) are loaded once.
+//
+// The per-item arithmetic of each column pass lives in a body struct
+// (ColGen, ColM128) shared by the single-movie kernels and the batched plan
+// kernel, so a pair's T column has the same FP32 operations in both.
+//
+// ColGen: generic decimated pass, A = N/M inverse M-point FFTs in LDS
+// (fft_lds_t), combine by lane groups of SP terms summed through LDS.
 template <int N, int M, int NT, int SP>
-__global__ __launch_bounds__(NT) void k_pair_cols(const float2* __restrict__ spectra,
-                                                  size_t spec_stride,
-                                                  const int2* __restrict__ pairs, int npairs,
-                                                  const float* __restrict__ wxy, int box,
-                                                  float2* __restrict__ tbuf,
-                                                  const float2* __restrict__ tw) {
-  constexpr int A = N / M;
-  constexpr int H = N / 2 + 1;
-  constexpr int PB = 4;
-  constexpr int TS = lds_len<PB>(M) | 1;
-  constexpr int EPT = (N + NT - 1) / NT;  // elements per thread
-  constexpr int AS = A / SP;              // combine terms per thread
+struct ColGen {
+  static constexpr int NN = N;
+  static constexpr int A = N / M;
+  static constexpr int H = N / 2 + 1;
+  static constexpr int PB = 4;
+  static constexpr int TS = lds_len<PB>(M) | 1;
+  static constexpr int EPT = (N + NT - 1) / NT;  // elements per thread
+  static constexpr int AS = A / SP;              // combine terms per thread
+  static constexpr int LDS = A * TS + M + SP * 256;  // float2 elements
   static_assert(A * M == N && A % SP == 0, "decimation");
-  __shared__ float2 buf[A * TS];
-  __shared__ float2 twm[M];
-  __shared__ float2 part[SP][256];
-  const int tid = threadIdx.x;
-  for (int i = tid; i < M; i += NT) twm[i] = tw[i * (N / M)];
-  // ky weights (-1)^{y'} exp(m y'^2) / sizec for this thread's elements
+  float2* buf;
+  float2* twm;
+  float2* part;  // [SP][256]
   float wy[EPT];
-#pragma unroll
-  for (int i = 0; i < EPT; i++) {
-    const int e = tid + i * NT;
-    wy[i] = e < N ? wxy[H + e] : 0.f;
-  }
-  // combine: thread -> (output r, term block h)
-  const int ori = (N - box) / 2;
-  const int cr = tid % box, ch = tid / box;
-  const bool cact = tid < box * SP;
-  const int R = ori + cr;
   float2 ctw[AS];
+  int cr, ch, Rm, box, tid;
+  bool cact;
+
+  // Loads the workgroup constants; ends with a barrier.
+  MC_DEV void init(float2* lds, const float* __restrict__ wxy, const float2* __restrict__ tw,
+                   int box_, int tid_) {
+    tid = tid_;
+    box = box_;
+    buf = lds;
+    twm = lds + A * TS;
+    part = twm + M;
+    for (int i = tid; i < M; i += NT) twm[i] = tw[i * (N / M)];
+    // ky weights (-1)^{y'} exp(m y'^2) / sizec for this thread's elements
 #pragma unroll
-  for (int i = 0; i < AS; i++) {
-    const int a = i * SP + ch;
-    ctw[i] = cact ? tw[(a * R) % N] : make_float2(0.f, 0.f);
+    for (int i = 0; i < EPT; i++) {
+      const int e = tid + i * NT;
+      wy[i] = e < N ? wxy[H + e] : 0.f;
+    }
+    // combine: thread -> (output r, term block h)
+    const int ori = (N - box) / 2;
+    cr = tid % box;
+    ch = tid / box;
+    cact = tid < box * SP;
+    const int R = ori + cr;
+#pragma unroll
+    for (int i = 0; i < AS; i++) {
+      const int a = i * SP + ch;
+      ctw[i] = cact ? tw[(a * R) % N] : make_float2(0.f, 0.f);
+    }
+    Rm = R & (M - 1);
+    __syncthreads();
   }
-  const int Rm = R & (M - 1);
-  const int xcd = blockIdx.x % 8, lid = blockIdx.x / 8, nper = gridDim.x / 8;
-  const long long items = (long long)npairs * H;
-  const long long i0 = items * xcd / 8, i1 = items * (xcd + 1) / 8;
-  __syncthreads();
-  for (long long it = i0 + lid; it < i1; it += nper) {
-    const int kx = (int)(it / npairs), p = (int)(it % npairs);
-    const int2 pr = pairs[p];
-    const float2* a = spectra + (size_t)pr.x * spec_stride + (size_t)kx * N;
-    const float2* b = spectra + (size_t)pr.y * spec_stride + (size_t)kx * N;
-    const float wx = wxy[kx];
+  // Item (pair columns a, b at bin kx with column weight wx) -> tout[0, box).
+  // buf and part are rewritten only after the next item's first barrier.
+  MC_DEV void item(const float2* a, const float2* b, const float* __restrict__, float wx,
+                   float2* tout) {
     float2 va[EPT], vb[EPT];
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
@@ -115,16 +126,38 @@ __global__ __launch_bounds__(NT) void k_pair_cols(const float2* __restrict__ spe
       float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
       for (int i = 0; i < AS; i++) acc = cadd(acc, cmul(ctw[i], buf[(i * SP + ch) * TS + lix<PB>(Rm)]));
-      part[ch][cr] = acc;
+      part[ch * 256 + cr] = acc;
     }
     __syncthreads();
     if (tid < box) {
-      float2 acc = part[0][tid];
+      float2 acc = part[tid];
 #pragma unroll
-      for (int h = 1; h < SP; h++) acc = cadd(acc, part[h][tid]);
-      tbuf[((size_t)p * H + kx) * box + tid] = acc;
+      for (int h = 1; h < SP; h++) acc = cadd(acc, part[h * 256 + tid]);
+      tout[tid] = acc;
     }
-    // buf and part are rewritten only after the next item's first barrier
+  }
+};
+
+template <int N, int M, int NT, int SP>
+__global__ __launch_bounds__(NT) void k_pair_cols(const float2* __restrict__ spectra,
+                                                  size_t spec_stride,
+                                                  const int2* __restrict__ pairs, int npairs,
+                                                  const float* __restrict__ wxy, int box,
+                                                  float2* __restrict__ tbuf,
+                                                  const float2* __restrict__ tw) {
+  typedef ColGen<N, M, NT, SP> B;
+  __shared__ float2 lds[B::LDS];
+  B b;
+  b.init(lds, wxy, tw, box, threadIdx.x);
+  const int xcd = blockIdx.x % 8, lid = blockIdx.x / 8, nper = gridDim.x / 8;
+  const long long items = (long long)npairs * B::H;
+  const long long i0 = items * xcd / 8, i1 = items * (xcd + 1) / 8;
+  for (long long it = i0 + lid; it < i1; it += nper) {
+    const int kx = (int)(it / npairs), p = (int)(it % npairs);
+    const int2 pr = pairs[p];
+    b.item(spectra + (size_t)pr.x * spec_stride + (size_t)kx * N,
+           spectra + (size_t)pr.y * spec_stride + (size_t)kx * N, wxy, wxy[kx],
+           tbuf + ((size_t)p * B::H + kx) * box);
   }
 }
 
@@ -137,74 +170,69 @@ __global__ __launch_bounds__(NT) void k_pair_cols(const float2* __restrict__ spe
 //    only on tid % 16 and stay in registers;
 //  combine: SP adjacent lanes share one output row and reduce with xor
 //    shuffles (no LDS round trip).
-// A launch covers the bins kx in [kxb, kxb + nkx) of every pair.
-template <int N, int SP, bool PF, int MINW>
-__global__ __launch_bounds__(256, (N <= 4096 ? MINW : 1)) void k_pair_cols_m128(const float2* __restrict__ spectra,
-                                                        size_t spec_stride,
-                                                        const int2* __restrict__ pairs,
-                                                        int npairs,
-                                                        const float* __restrict__ wxy, int box,
-                                                        float2* __restrict__ tbuf,
-                                                        const float2* __restrict__ tw,
-                                                        int kxb, int nkx) {
-  constexpr int NT = 256;
-  constexpr int M = 128;
-  constexpr int A = N / M;
-  constexpr int H = N / 2 + 1;
-  constexpr int PB = 4;
-  constexpr int TS = lds_len<PB>(M) | 1;
-  constexpr int NB0 = A * 8;                    // stage-0 butterflies
-  constexpr int P0 = (NB0 + NT - 1) / NT;
-  constexpr int NB1 = A * 16;                   // stage-1 butterflies
-  constexpr int P1 = (NB1 + NT - 1) / NT;
-  constexpr int AS = A / SP;
+// WYG (for the 128-VGPR cap of 4 workgroups per CU): the stage-0 weights are
+// re-read per item (L1-resident, coalesced) instead of held in 16 registers,
+// and the stage-1 twiddles come from a 128-entry LDS table, which frees 16
+// more VGPRs.  The values, and so the results, are the same either way.
+template <int N, int SP, bool WYG>
+struct ColM128 {
+  static constexpr int NN = N;
+  static constexpr int NT = 256;
+  static constexpr int M = 128;
+  static constexpr int A = N / M;
+  static constexpr int H = N / 2 + 1;
+  static constexpr int PB = 4;
+  static constexpr int TS = lds_len<PB>(M) | 1;
+  static constexpr int NB0 = A * 8;  // stage-0 butterflies
+  static constexpr int P0 = (NB0 + NT - 1) / NT;
+  static constexpr int NB1 = A * 16;  // stage-1 butterflies
+  static constexpr int P1 = (NB1 + NT - 1) / NT;
+  static constexpr int AS = A / SP;
+  static constexpr int LDS = A * TS + (WYG ? M : 0);  // float2 elements
   static_assert(NT % 16 == 0 && A % SP == 0, "shape");
-  // Under the 128-VGPR cap of MINW = 4 the stage-0 weights are re-read per
-  // item (L1-resident, coalesced) instead of held in 16 registers.
-  constexpr bool WYG = MINW >= 4;
-  __shared__ float2 buf[A * TS];
-  const int tid = threadIdx.x;
-  // stage-0 element weights (fixed per thread)
+  float2* buf;
+  float2* tw128;
   float wy[P0][16];
-#pragma unroll
-  for (int p = 0; p < P0; p++) {
-    const int q = tid + p * NT;
-    const int a = q % A, j = q / A;
-#pragma unroll
-    for (int r = 0; r < 16; r++) wy[p][r] = (!WYG && q < NB0) ? wxy[H + a + A * (j + 8 * r)] : 0.f;
-  }
-  // stage-1 twiddles W_128^{k r}, k = tid % 16 (inverse sign): in registers,
-  // or (WYG) read from a 128-entry LDS table, which frees 16 more VGPRs
-  __shared__ float2 tw128[WYG ? M : 1];
   float2 t1[8];
-  if constexpr (WYG) {
-    for (int i = tid; i < M; i += NT) tw128[i] = tw[i * (N / M)];
-  } else {
-#pragma unroll
-    for (int r = 0; r < 8; r++) t1[r] = tw[((tid % 16) * r * (N / M)) % N];
-  }
-  // combine: lane group (SP lanes) per output row
-  const int ori = (N - box) / 2;
-  const int cr = tid / SP, ch = tid % SP;
-  const bool cact = cr < box;
-  const int R = ori + (cact ? cr : 0);
-  const int Rm = R & (M - 1);
   float2 ctw[AS];
+  int cr, ch, Rm, tid;
+  bool cact;
+
+  // Loads the workgroup constants (the LDS twiddle table is complete after
+  // the first barrier of the first item).
+  MC_DEV void init(float2* lds, const float* __restrict__ wxy, const float2* __restrict__ tw,
+                   int box, int tid_) {
+    tid = tid_;
+    buf = lds;
+    tw128 = lds + A * TS;
+    // stage-0 element weights (fixed per thread)
 #pragma unroll
-  for (int i = 0; i < AS; i++) ctw[i] = tw[((i * SP + ch) * R) % N];
-  const int xcd = blockIdx.x % 8, lid = blockIdx.x / 8, nper = gridDim.x / 8;
-  const long long items = (long long)npairs * nkx;
-  const long long i0 = items * xcd / 8, i1 = items * (xcd + 1) / 8;
-  // With PF (software pipeline) the raw spectrum values of the next item are
-  // loaded into registers while the current item goes through stage 1 and
-  // the combine, so the global-load latency overlaps LDS work (at the cost
-  // of 64 more VGPRs, i.e. one workgroup less per CU).
-  float2 ra[P0][16], rb[P0][16];
-  auto fetch = [&](long long it) {
-    const int kx = kxb + (int)(it / npairs), p = (int)(it % npairs);
-    const int2 pr = pairs[p];
-    const float2* sa = spectra + (size_t)pr.x * spec_stride + (size_t)kx * N;
-    const float2* sb = spectra + (size_t)pr.y * spec_stride + (size_t)kx * N;
+    for (int p = 0; p < P0; p++) {
+      const int q = tid + p * NT;
+      const int a = q % A, j = q / A;
+#pragma unroll
+      for (int r = 0; r < 16; r++) wy[p][r] = (!WYG && q < NB0) ? wxy[H + a + A * (j + 8 * r)] : 0.f;
+    }
+    // stage-1 twiddles W_128^{k r}, k = tid % 16 (inverse sign)
+    if constexpr (WYG) {
+      for (int i = tid; i < M; i += NT) tw128[i] = tw[i * (N / M)];
+    } else {
+#pragma unroll
+      for (int r = 0; r < 8; r++) t1[r] = tw[((tid % 16) * r * (N / M)) % N];
+    }
+    // combine: lane group (SP lanes) per output row
+    const int ori = (N - box) / 2;
+    cr = tid / SP;
+    ch = tid % SP;
+    cact = cr < box;
+    const int R = ori + (cact ? cr : 0);
+    Rm = R & (M - 1);
+#pragma unroll
+    for (int i = 0; i < AS; i++) ctw[i] = tw[((i * SP + ch) * R) % N];
+  }
+  // Raw stage-0 inputs of one item: the two spectrum columns at bin kx.
+  MC_DEV void fetch(float2 (&ra)[P0][16], float2 (&rb)[P0][16], const float2* sa,
+                    const float2* sb) const {
 #pragma unroll
     for (int pp = 0; pp < P0; pp++) {
       const int q = tid + pp * NT;
@@ -218,13 +246,10 @@ __global__ __launch_bounds__(256, (N <= 4096 ? MINW : 1)) void k_pair_cols_m128(
         }
       }
     }
-  };
-  if (PF && i0 + lid < i1) fetch(i0 + lid);
-  for (long long it = i0 + lid; it < i1; it += nper) {
-    const int kx = kxb + (int)(it / npairs), p = (int)(it % npairs);
-    const float wx = wxy[kx];
-    if (!PF) fetch(it);
-    // ---- stage 0 from the prefetched registers
+  }
+  // Stage 0 from the fetched registers into LDS (no barrier).
+  MC_DEV void stage0(const float2 (&ra)[P0][16], const float2 (&rb)[P0][16],
+                     const float* __restrict__ wxy, float wx) {
 #pragma unroll
     for (int pp = 0; pp < P0; pp++) {
       const int q = tid + pp * NT;
@@ -241,7 +266,10 @@ __global__ __launch_bounds__(256, (N <= 4096 ? MINW : 1)) void k_pair_cols_m128(
         for (int r = 0; r < 16; r++) buf[a * TS + lix_at<PB, 1, 16>(lix<PB>(16 * j), 16 * j, r)] = v[r];
       }
     }
-    if (PF && it + nper < i1) fetch(it + nper);
+  }
+  // Stage 1 and the combine into tout[0, box); ends with a barrier (buf is
+  // rewritten by the next item's stage 0).
+  MC_DEV void finish(float2* tout) {
     __syncthreads();
     // ---- stage 1 in place (radix 8, Ns = 16)
 #pragma unroll
@@ -270,8 +298,172 @@ __global__ __launch_bounds__(256, (N <= 4096 ? MINW : 1)) void k_pair_cols_m128(
       acc.x += __shfl_xor(acc.x, o);
       acc.y += __shfl_xor(acc.y, o);
     }
-    if (cact && ch == 0) tbuf[((size_t)p * H + kx) * box + cr] = acc;
-    __syncthreads();  // buf is rewritten by the next item's stage 0
+    if (cact && ch == 0) tout[cr] = acc;
+    __syncthreads();
+  }
+  // One whole item (no prefetch).
+  MC_DEV void item(const float2* a, const float2* b, const float* __restrict__ wxy, float wx,
+                   float2* tout) {
+    float2 ra[P0][16], rb[P0][16];
+    fetch(ra, rb, a, b);
+    stage0(ra, rb, wxy, wx);
+    finish(tout);
+  }
+};
+
+// A launch covers the bins kx in [kxb, kxb + nkx) of every pair.  With PF
+// (software pipeline) the raw spectrum values of the next item are loaded
+// into registers while the current item goes through stage 1 and the
+// combine, so the global-load latency overlaps LDS work (at the cost of 64
+// more VGPRs, i.e. one workgroup less per CU).
+template <int N, int SP, bool PF, int MINW>
+__global__ __launch_bounds__(256, (N <= 4096 ? MINW : 1)) void k_pair_cols_m128(const float2* __restrict__ spectra,
+                                                        size_t spec_stride,
+                                                        const int2* __restrict__ pairs,
+                                                        int npairs,
+                                                        const float* __restrict__ wxy, int box,
+                                                        float2* __restrict__ tbuf,
+                                                        const float2* __restrict__ tw,
+                                                        int kxb, int nkx) {
+  typedef ColM128<N, SP, (MINW >= 4)> B;
+  __shared__ float2 lds[B::LDS];
+  B b;
+  b.init(lds, wxy, tw, box, threadIdx.x);
+  const int xcd = blockIdx.x % 8, lid = blockIdx.x / 8, nper = gridDim.x / 8;
+  const long long items = (long long)npairs * nkx;
+  const long long i0 = items * xcd / 8, i1 = items * (xcd + 1) / 8;
+  float2 ra[B::P0][16], rb[B::P0][16];
+  auto fetch = [&](long long it) {
+    const int kx = kxb + (int)(it / npairs), p = (int)(it % npairs);
+    const int2 pr = pairs[p];
+    b.fetch(ra, rb, spectra + (size_t)pr.x * spec_stride + (size_t)kx * N,
+            spectra + (size_t)pr.y * spec_stride + (size_t)kx * N);
+  };
+  if (PF && i0 + lid < i1) fetch(i0 + lid);
+  for (long long it = i0 + lid; it < i1; it += nper) {
+    const int kx = kxb + (int)(it / npairs), p = (int)(it % npairs);
+    if (!PF) fetch(it);
+    b.stage0(ra, rb, wxy, wxy[kx]);
+    if (PF && it + nper < i1) fetch(it + nper);
+    b.finish(tbuf + ((size_t)p * B::H + kx) * box);
+  }
+}
+
+// ------------------------------------------- batched (plan) column pass
+// One persistent launch over the column-pass items of one movie or of many
+// movies of different transform sizes (the default column pass of the M = 128
+// decimation).  Movie m's items are the global indices [item0_m, item0_m +
+// npairs_m (n_m/2+1)), kx major, pair minor; a plan table lists the movies in
+// increasing item0 (one movie: the plan is passed by value).  A launch covers
+// the global range [g0, g1), split over the XCDs as in the single-movie
+// kernels; a workgroup runs consecutive items of one movie with that movie's
+// body (ColM128 at its n) and sets the body up again when its next item
+// belongs to the next movie.  Bodies are keyed n | SP << 14.  A movie alone
+// and a movie in a batch run the same machine code for its items, so its T
+// columns are bitwise the same (two builds of one body may contract
+// multiply-adds differently, so this is a property of the code, not of the
+// source).
+// The items [it, stop) (step nper) of one movie.  The movie's pointers are
+// parameters with __restrict__ so that, as in the single-movie kernels, its
+// uniform reads (pair slots, column weight) are scalar loads.
+template <class B>
+MC_DEV long long plan_items(const float2* __restrict__ spectra, size_t spec_stride,
+                            const int2* __restrict__ pairs, int np, const float* __restrict__ wxy,
+                            const float2* __restrict__ tw, float2* __restrict__ tbuf, long long it,
+                            long long li0, long long stop, int nper, int box, float2* lds,
+                            int tid) {
+  constexpr int N = B::NN;
+  constexpr int H = N / 2 + 1;
+  B b;
+  __syncthreads();  // the previous body's LDS reads are done
+  b.init(lds, wxy, tw, box, tid);
+  // a movie has fewer than 2^31 items: kx and pair advance incrementally
+  const int li = (int)li0;
+  int kx = li / np, p = li - kx * np;
+  const int dkx = nper / np, dp = nper - dkx * np;
+  for (; it < stop; it += nper) {
+    const int2 pr = pairs[p];
+    b.item(spectra + (size_t)pr.x * spec_stride + (size_t)kx * N,
+           spectra + (size_t)pr.y * spec_stride + (size_t)kx * N, wxy, wxy[kx],
+           tbuf + ((size_t)p * H + kx) * box);
+    kx += dkx;
+    p += dp;
+    if (p >= np) {
+      p -= np;
+      kx++;
+    }
+  }
+  return it;
+}
+
+constexpr int plan_key(int n, int sp, bool gen) { return n | (sp << 14) | (gen ? 1 << 20 : 0); }
+constexpr int kPlanLds = ColM128<4096, 2, true>::LDS;  // the largest body arena (float2)
+// The LDS arena of the plan kernels, shared by all bodies.
+__shared__ float2 plan_lds[kPlanLds];
+
+// Runs the items [it, stop) of plan pl with body B.  The thread index comes
+// in through an opaque move made here, once per plan: the values a body
+// derives from it are then computed per plan rather than hoisted to the
+// kernel entry, where those of every body would be live at once.  The box
+// comes from the plan record for the same reason.
+template <class B>
+MC_DEV long long plan_run(const PairColsPlan& pl, long long it, long long stop, int nper) {
+  int tid = threadIdx.x;
+  if constexpr (B::NN != 4096) asm volatile("v_mov_b32 %0, %1" : "=v"(tid) : "v"(tid));
+  return plan_items<B>(pl.spectra, pl.spec_stride, pl.pairs, pl.npairs, pl.wxy, pl.tw, pl.tbuf, it,
+                       it - pl.item0, stop, nper, pl.box, plan_lds, tid);
+}
+
+// Two register classes: ColM128 at n <= 1536 and at 4096 fits the 128-VGPR
+// cap of 4 workgroups per CU and runs in k_pair_cols_plan<4>; at 1920..3840
+// it needs 150-180 VGPRs and runs in k_pair_cols_plan<2>.
+constexpr bool plan_wide(int n) { return n >= 1920 && n < 4096; }
+
+template <int MINW>
+__global__ __launch_bounds__(256, MINW) void k_pair_cols_plan(PairColsPlan one,
+                                                              const PairColsPlan* __restrict__ plans,
+                                                              int nplans, long long g0,
+                                                              long long g1) {
+  const int xcd = blockIdx.x % 8, lid = blockIdx.x / 8, nper = gridDim.x / 8;
+  const long long tot = g1 - g0;
+  const long long x1 = g0 + tot * (xcd + 1) / 8;
+  long long it = g0 + tot * xcd / 8 + lid;
+  if (it >= x1) return;
+  if (!plans) nplans = 1;
+  auto plan = [&](int i) -> PairColsPlan { return plans ? plans[i] : one; };
+  // the movie of the first item (binary search), then forward as `it` grows
+  int lo = 0, hi = nplans - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (plan(mid).item0 <= it) lo = mid;
+    else hi = mid - 1;
+  }
+  int pi = lo;
+  while (it < x1) {
+    while (pi + 1 < nplans && plan(pi + 1).item0 <= it) pi++;
+    const PairColsPlan pl = plan(pi);
+    const long long end = pi + 1 < nplans ? plan(pi + 1).item0 : x1;
+    const long long stop = end < x1 ? end : x1;
+    switch (pl.body) {
+// A key of the other class is never in this class's table; it is skipped.
+#define MC_PLAN_M(NN, SPV)                                                                   \
+  case plan_key(NN, SPV, false):                                                             \
+    if constexpr (plan_wide(NN) == (MINW == 2))                                              \
+      it = plan_run<ColM128<NN, SPV, true>>(pl, it, stop, nper);                            \
+    else                                                                                     \
+      it += (stop - it + nper - 1) / nper * nper;                                            \
+    break;
+      MC_PLAN_M(512, 4) MC_PLAN_M(512, 2) MC_PLAN_M(768, 2) MC_PLAN_M(1024, 4) MC_PLAN_M(1024, 2)
+      MC_PLAN_M(1280, 2) MC_PLAN_M(1536, 4) MC_PLAN_M(1536, 2) MC_PLAN_M(1920, 1)
+      MC_PLAN_M(2048, 4) MC_PLAN_M(2048, 2) MC_PLAN_M(2560, 4) MC_PLAN_M(2560, 2)
+      MC_PLAN_M(3072, 4) MC_PLAN_M(3072, 2) MC_PLAN_M(3840, 2) MC_PLAN_M(4096, 4)
+      MC_PLAN_M(4096, 2)
+#undef MC_PLAN_M
+      default:  // not reached: the host admits only the keys above
+        it += (stop - it + nper - 1) / nper * nper;
+        break;
+    }
+    // `it` is now the workgroup's first item at or past `stop`
   }
 }
 
@@ -1126,6 +1318,9 @@ bool pair_form_supported(int precision, int n, int box) {
   return false;
 }
 
+static hipError_t plan_launches(const PairColsPlan& one, const PairColsPlan* plans, int nplans,
+                                long long items, int cls, hipStream_t st);
+
 hipError_t launch_pair_cols(const float2* spectra, size_t spec_stride, const int2* pairs,
                             int npairs, int n, float bfactor, int box, float2* tbuf,
                             float* tmax, int precision, int variant, hipStream_t st) {
@@ -1148,6 +1343,23 @@ hipError_t launch_pair_cols(const float2* spectra, size_t spec_stride, const int
   }
   if (!pair_cols_decimated(n))
     return launch_pair_cols_czt(spectra, spec_stride, pairs, npairs, wxy, n, box, tbuf, st);
+  const int body = pair_cols_plan_body(n, box, variant);
+  if (body >= 0) {  // the default: the plan kernel with this one movie
+    PairColsPlan one{};
+    one.spectra = spectra;
+    one.pairs = pairs;
+    one.wxy = wxy;
+    one.tw = twiddles(n);
+    if (!one.tw) return hipErrorOutOfMemory;
+    one.tbuf = tbuf;
+    one.spec_stride = spec_stride;
+    one.item0 = 0;
+    one.npairs = npairs;
+    one.box = box;
+    one.body = body;
+    return plan_launches(one, nullptr, 1, (long long)npairs * (n / 2 + 1),
+                         pair_cols_plan_class(body), st);
+  }
   switch (n) {
     case 64: return pair_cols_n<64>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
     case 128: return pair_cols_n<128>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
@@ -1168,6 +1380,45 @@ hipError_t launch_pair_cols(const float2* spectra, size_t spec_stride, const int
     case 3840: return pair_cols_n<3840>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, variant, st);
     default: return hipErrorInvalidValue;
   }
+}
+
+int pair_cols_plan_body(int n, int box, int variant) {
+  if (variant != PAIR_COLS_AUTO) return -1;
+  if (box < 1 || box > 128 || n < 512 || n > 4096 || n % 128 != 0 || !pair_cols_decimated(n))
+    return -1;
+  const int A = n / 128;
+  return plan_key(n, (A % 4 == 0 && box <= 64) ? 4 : (A % 2 == 0 ? 2 : 1), false);
+}
+
+int pair_cols_plan_class(int body) { return plan_wide(body & 0x3fff) ? 1 : 0; }
+
+const float* pair_weights(int n, float bfactor) { return weights(n, bfactor); }
+
+// Launches of about 128 items per workgroup (see col_segments) over a plan
+// table (plans != nullptr) or the one plan `one`.
+static hipError_t plan_launches(const PairColsPlan& one, const PairColsPlan* plans, int nplans,
+                                long long items, int cls, hipStream_t st) {
+  if (items <= 0) return hipSuccess;
+  static const int g4 = persistent_grid(k_pair_cols_plan<4>, 256);
+  static const int g2 = persistent_grid(k_pair_cols_plan<2>, 256);
+  const int grid = cls ? g2 : g4;
+  const long long per_wg = items / grid;
+  const long long S = std::max(1LL, (per_wg + 64) / 128);
+  for (long long s = 0; s < S; s++) {
+    const long long a = items * s / S, b = items * (s + 1) / S;
+    if (cls)
+      hipLaunchKernelGGL(k_pair_cols_plan<2>, dim3(grid), dim3(256), 0, st, one, plans, nplans, a, b);
+    else
+      hipLaunchKernelGGL(k_pair_cols_plan<4>, dim3(grid), dim3(256), 0, st, one, plans, nplans, a, b);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_cols_plans(const PairColsPlan* plans, int nplans, long long items, int cls,
+                                  hipStream_t st) {
+  if (nplans <= 0) return hipSuccess;
+  PairColsPlan none{};
+  return plan_launches(none, plans, nplans, items, cls, st);
 }
 
 hipError_t launch_pair_rows(const float2* tbuf, const float* tmax, int npairs, int n, int box,

@@ -255,10 +255,10 @@ size_t find_peaks_out_elems(int npairs) {
 }
 
 hipError_t launch_find_peaks(const float* boxmaps, int npairs, int box, int n, int wnoise,
-                             double* out, hipStream_t st) {
+                             double* out, hipStream_t st, double* part) {
   if (npairs <= 0) return hipSuccess;
   if (box < 16) return hipErrorInvalidValue;
-  double* part = out + (size_t)npairs * 3;
+  if (!part) part = out + (size_t)npairs * 3;
   const unsigned fg = (unsigned)((npairs + 63) / 64);
   switch (peak_slices(npairs)) {
     case 1:

@@ -55,7 +55,9 @@ def batch(ctx, movies):
 
 def test_batch_bit_identical_to_single(ctx, movies, batch):
     res, br = batch
-    assert br["groups"] == len(SPECS)
+    # one group and one sub-batch: the column pass of all six sizes runs in
+    # the batched plan kernel, one graph replay per round
+    assert br["groups"] == 1 and br["sub_batches"] == 1
     for m, r in zip(movies, res):
         s = ctx.align_movie(m["dev"], m["frames"].shape, m["frames"].dtype, m["params"],
                             want=WANT)

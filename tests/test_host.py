@@ -103,6 +103,8 @@ def test_shipped_tuning_table_is_well_formed():
         assert e["nu"] in sizes and e["n"] in sizes and e["n"] <= e["nu"]
         assert e["dtype"] in mc.MC_DTYPE.values() and 16 <= e["box"] <= 256
         assert e["precision"] in (0, 1)
-        assert 1 <= (e["r2c"] & 7) <= 3 and (e["r2c"] & ~15) == 0
+        # row shape (bits 0-2), persistent column pass (bit 3), column radix
+        # plan and LDS padding (bits 4-6, fewer than 6 plans)
+        assert 1 <= (e["r2c"] & 7) <= 3 and 0 <= (e["r2c"] >> 4) < 6
         assert e["chunk"] in (1, 2, 4, 8)
         assert 0 <= e["cols"] <= 3 and 0 <= e["rows"] <= 2
