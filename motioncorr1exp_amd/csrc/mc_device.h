@@ -141,20 +141,52 @@ template <int S> MC_DEV void dft5(float2* v) {
   v[3] = csub(r2, i2);
 }
 
+// X[k] = sum_n v[n] W7^{S n k}: the pairs v[m] +- v[7-m] (m = 1, 2, 3) with
+// the cosines and sines of 2 pi m k / 7.
+template <int S> MC_DEV void dft7(float2* v) {
+  const float c1 = 0.62348980185873353053f;   // cos(2 pi/7)
+  const float c2 = -0.22252093395631440429f;  // cos(4 pi/7)
+  const float c3 = -0.90096886790241912624f;  // cos(6 pi/7)
+  const float s1 = 0.78183148246802980871f;   // sin(2 pi/7)
+  const float s2 = 0.97492791218182360702f;   // sin(4 pi/7)
+  const float s3 = 0.43388373911755812048f;   // sin(6 pi/7)
+  const float2 a = v[0];
+  const float2 p1 = cadd(v[1], v[6]), m1 = csub(v[1], v[6]);
+  const float2 p2 = cadd(v[2], v[5]), m2 = csub(v[2], v[5]);
+  const float2 p3 = cadd(v[3], v[4]), m3 = csub(v[3], v[4]);
+  const float2 r1 = make_float2(a.x + c1 * p1.x + c2 * p2.x + c3 * p3.x, a.y + c1 * p1.y + c2 * p2.y + c3 * p3.y);
+  const float2 r2 = make_float2(a.x + c2 * p1.x + c3 * p2.x + c1 * p3.x, a.y + c2 * p1.y + c3 * p2.y + c1 * p3.y);
+  const float2 r3 = make_float2(a.x + c3 * p1.x + c1 * p2.x + c2 * p3.x, a.y + c3 * p1.y + c1 * p2.y + c2 * p3.y);
+  const float2 q1 = make_float2(s1 * m1.x + s2 * m2.x + s3 * m3.x, s1 * m1.y + s2 * m2.y + s3 * m3.y);
+  const float2 q2 = make_float2(s2 * m1.x - s3 * m2.x - s1 * m3.x, s2 * m1.y - s3 * m2.y - s1 * m3.y);
+  const float2 q3 = make_float2(s3 * m1.x - s1 * m2.x + s2 * m3.x, s3 * m1.y - s1 * m2.y + s2 * m3.y);
+  // S i q
+  const float2 i1 = make_float2(-S * q1.y, S * q1.x), i2 = make_float2(-S * q2.y, S * q2.x),
+               i3 = make_float2(-S * q3.y, S * q3.x);
+  v[0] = cadd(a, cadd(p1, cadd(p2, p3)));
+  v[1] = cadd(r1, i1);
+  v[6] = csub(r1, i1);
+  v[2] = cadd(r2, i2);
+  v[5] = csub(r2, i2);
+  v[3] = cadd(r3, i3);
+  v[4] = csub(r3, i3);
+}
+
 template <int R, int S> MC_DEV void dft(float2* v) {
   if constexpr (R == 2) dft2<S>(v);
   else if constexpr (R == 3) dft3<S>(v);
   else if constexpr (R == 4) dft4<S>(v);
   else if constexpr (R == 5) dft5<S>(v);
+  else if constexpr (R == 7) dft7<S>(v);
   else if constexpr (R == 8) dft8<S>(v);
   else if constexpr (R == 16) dft16<S>(v);
-  else static_assert(R == 2 || R == 3 || R == 4 || R == 5 || R == 8 || R == 16, "radix");
+  else static_assert(R == 2 || R == 3 || R == 4 || R == 5 || R == 7 || R == 8 || R == 16, "radix");
 }
 
 // ---------------------------------------------------- Stockham stage plans
-// Lengths N = 2^k * m, m in {1, 3, 5, 15}.  Power-of-two plans are
-// balanced; the odd factor adds a radix-3 and/or a radix-5 stage after the
-// plan of 2^k.
+// Lengths N = 2^k * m, m a product of distinct factors 3, 5 and 7.
+// Power-of-two plans are balanced; the odd factors add a radix-3, radix-5
+// and/or radix-7 stage after the plan of 2^k.
 __host__ __device__ constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
 __host__ __device__ constexpr bool is_pow2c(int n) { return n > 0 && (n & (n - 1)) == 0; }
 __host__ __device__ constexpr int pow2_stage_count(int N) {
@@ -200,16 +232,22 @@ __host__ __device__ constexpr int plan_p2_radix(int N, int s, int P) {
        : pow2_stage_radix(N, s);
 }
 __host__ __device__ constexpr int odd_part(int N) { return N % 2 ? N : odd_part(N / 2); }
-// Stages of a length N = 2^k * {1, 3, 5, 15} under plan P: the power-of-two
-// stages, then a radix-3 and/or a radix-5 stage.
+// Stages of a length N = 2^k * m under plan P: the power-of-two stages, then
+// one stage per odd factor of m, in the order 3, 5, 7.
+__host__ __device__ constexpr int odd_radix(int m, int i) {
+  // the i-th of the factors 3, 5, 7 that divide m
+  return i == 0 ? (m % 3 == 0 ? 3 : (m % 5 == 0 ? 5 : 7))
+       : i == 1 ? (m % 3 == 0 ? (m % 5 == 0 ? 5 : 7) : 7)
+                : 7;
+}
 __host__ __device__ constexpr int stage_count(int N, int P = 0) {
   return plan_p2_count(N / odd_part(N), P) + (odd_part(N) % 3 == 0 ? 1 : 0) +
-         (odd_part(N) % 5 == 0 ? 1 : 0);
+         (odd_part(N) % 5 == 0 ? 1 : 0) + (odd_part(N) % 7 == 0 ? 1 : 0);
 }
 __host__ __device__ constexpr int stage_radix(int N, int s, int P = 0) {
   return s < plan_p2_count(N / odd_part(N), P)
              ? plan_p2_radix(N / odd_part(N), s, P)
-             : ((s == plan_p2_count(N / odd_part(N), P) && odd_part(N) % 3 == 0) ? 3 : 5);
+             : odd_radix(odd_part(N), s - plan_p2_count(N / odd_part(N), P));
 }
 // Table index modulo the table size (any size; a power of two masks).
 MC_DEV int tw_mod(int e, int ntw) { return (ntw & (ntw - 1)) == 0 ? (e & (ntw - 1)) : (e % ntw); }

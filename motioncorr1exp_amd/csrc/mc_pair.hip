@@ -53,7 +53,8 @@ typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 // kernel, so a pair's T column has the same FP32 operations in both.
 //
 // ColGen: generic decimated pass, A = N/M inverse M-point FFTs in LDS
-// (fft_lds_t), combine by lane groups of SP terms summed through LDS.
+// (fft_lds_t; M any Stockham length), combine by lane groups of SP lanes
+// whose lane h takes the terms a = h, h + SP, ... < A, summed through LDS.
 template <int N, int M, int NT, int SP>
 struct ColGen {
   static constexpr int NN = N;
@@ -62,9 +63,9 @@ struct ColGen {
   static constexpr int PB = 4;
   static constexpr int TS = lds_len<PB>(M) | 1;
   static constexpr int EPT = (N + NT - 1) / NT;  // elements per thread
-  static constexpr int AS = A / SP;              // combine terms per thread
+  static constexpr int AS = (A + SP - 1) / SP;   // combine terms per thread (at most)
   static constexpr int LDS = A * TS + M + SP * 256;  // float2 elements
-  static_assert(A * M == N && A % SP == 0, "decimation");
+  static_assert(A * M == N, "decimation");
   float2* buf;
   float2* twm;
   float2* part;  // [SP][256]
@@ -97,9 +98,9 @@ struct ColGen {
 #pragma unroll
     for (int i = 0; i < AS; i++) {
       const int a = i * SP + ch;
-      ctw[i] = cact ? tw[(a * R) % N] : make_float2(0.f, 0.f);
+      ctw[i] = cact && a < A ? tw[(a * R) % N] : make_float2(0.f, 0.f);
     }
-    Rm = R & (M - 1);
+    Rm = R % M;
     __syncthreads();
   }
   // Item (pair columns a, b at bin kx with column weight wx) -> tout[0, box).
@@ -125,7 +126,9 @@ struct ColGen {
     if (cact) {
       float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
-      for (int i = 0; i < AS; i++) acc = cadd(acc, cmul(ctw[i], buf[(i * SP + ch) * TS + lix<PB>(Rm)]));
+      for (int i = 0; i < AS; i++)
+        if (A % SP == 0 || i * SP + ch < A)
+          acc = cadd(acc, cmul(ctw[i], buf[(i * SP + ch) * TS + lix<PB>(Rm)]));
       part[ch * 256 + cr] = acc;
     }
     __syncthreads();
@@ -1341,6 +1344,10 @@ hipError_t launch_pair_cols(const float2* spectra, size_t spec_stride, const int
       default: return hipErrorInvalidValue;
     }
   }
+  if (n == 3710 && box <= 128)
+    // the default square crop of a 3838 x 3710 K2 frame: n = 70 * 53,
+    // decimated into 53 inverse 70-point FFTs (radix 2, 5, 7)
+    return pair_cols_t<3710, 70, 2>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
   if (!pair_cols_decimated(n))
     return launch_pair_cols_czt(spectra, spec_stride, pairs, npairs, wxy, n, box, tbuf, st);
   const int body = pair_cols_plan_body(n, box, variant);
