@@ -399,7 +399,7 @@ MC_DEV long long plan_items(const float2* __restrict__ spectra, size_t spec_stri
   return it;
 }
 
-constexpr int plan_key(int n, int sp, bool gen) { return n | (sp << 14) | (gen ? 1 << 20 : 0); }
+constexpr int plan_key(int n, int sp) { return n | (sp << 14); }
 constexpr int kPlanLds = ColM128<4096, 2, true>::LDS;  // the largest body arena (float2)
 // The LDS arena of the plan kernels, shared by all bodies.
 __shared__ float2 plan_lds[kPlanLds];
@@ -450,7 +450,7 @@ __global__ __launch_bounds__(256, MINW) void k_pair_cols_plan(PairColsPlan one,
     switch (pl.body) {
 // A key of the other class is never in this class's table; it is skipped.
 #define MC_PLAN_M(NN, SPV)                                                                   \
-  case plan_key(NN, SPV, false):                                                             \
+  case plan_key(NN, SPV):                                                             \
     if constexpr (plan_wide(NN) == (MINW == 2))                                              \
       it = plan_run<ColM128<NN, SPV, true>>(pl, it, stop, nper);                            \
     else                                                                                     \
@@ -1394,7 +1394,7 @@ int pair_cols_plan_body(int n, int box, int variant) {
   if (box < 1 || box > 128 || n < 512 || n > 4096 || n % 128 != 0 || !pair_cols_decimated(n))
     return -1;
   const int A = n / 128;
-  return plan_key(n, (A % 4 == 0 && box <= 64) ? 4 : (A % 2 == 0 ? 2 : 1), false);
+  return plan_key(n, (A % 4 == 0 && box <= 64) ? 4 : (A % 2 == 0 ? 2 : 1));
 }
 
 int pair_cols_plan_class(int body) { return plan_wide(body & 0x3fff) ? 1 : 0; }

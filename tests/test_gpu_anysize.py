@@ -55,6 +55,31 @@ def test_level0_fft_any_even_size(n):
     d.free()
 
 
+@pytest.mark.parametrize("box", [32, 64, 96, 128, 160])
+def test_box_maps_at_3710_match_oracle(box):
+    """The pair column pass at n = 3710 runs decimated (53 inverse 70-point
+    FFTs, radix 2, 5, 7, and a 53-term combine split 27 + 26 over two lanes
+    per row) for boxes up to 128, and as the chirp-z pass above.  Box maps of
+    random spectra against the FP64 oracle's cross power and pruned inverse
+    (cufunc.cu:343-370, DFAlign.cu:897, FindShift's read-back
+    cufunc.cu:396-424)."""
+    n, F = 3710, 4
+    rng = np.random.default_rng(3710 + box)
+    spec = np.fft.rfft2(rng.standard_normal((F, n, n))).astype(np.complex64)
+    ctx = mc.Context(0)
+    ctx.set_spectra(spec.view(np.float32))
+    pairs = np.array([(0, 1), (0, 3), (1, 2), (2, 3)], np.int32)
+    _, _, boxes = ctx.pair_shifts(pairs, 150.0, box, want_boxes=True)
+    ctx.close()
+    o = Oracle("f64")
+    worst = 0.0
+    for q, (a, b) in enumerate(pairs):
+        ref = o.pruned_box(o.cross_power(spec[a], spec[b], n, 150.0), box)
+        worst = max(worst, float(np.abs(boxes[q] - ref).max() / np.abs(ref).max()))
+    print("n 3710 box %d: max error / max %.3g" % (box, worst), flush=True)
+    assert worst < 1e-5
+
+
 def _off(a, b):
     d = np.abs(np.asarray(a) - np.asarray(b))
     d = d[np.isfinite(d)]
