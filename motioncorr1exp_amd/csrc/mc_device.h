@@ -26,6 +26,23 @@ template <int S> MC_DEV float2 mul_si(float2 a) {
   return S > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
 }
 
+// ------------------------------------------------------- buffer loads
+// A raw buffer resource over `bytes` bytes at p (uniform).  A load through it
+// takes a per-lane 32-bit byte offset and a scalar byte offset, so the base is
+// added by the address unit instead of by 64-bit vector adds per load, and an
+// offset at or past `bytes` reads zero.  Word 3 (0x00020000) is the gfx9
+// family's 32-bit data format with no swizzle, as raw loads need.
+MC_DEV __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, bytes, 0x00020000);
+}
+// (The 64-bit result is reinterpreted whole: taking its two 32-bit elements
+// one by one made hipcc emit a 32-bit load and reuse the first word.)
+MC_DEV float2 buf_ld2(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  const f2v v = __builtin_bit_cast(f2v, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  return make_float2(v.x, v.y);
+}
+
 // twiddle e^{S * 2 pi i e / N} from a table t[e] = e^{+2 pi i e / N}
 template <int S> MC_DEV float2 twid(const float2* t, int e) {
   float2 w = t[e];

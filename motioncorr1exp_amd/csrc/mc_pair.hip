@@ -107,19 +107,21 @@ struct ColGen {
   // buf and part are rewritten only after the next item's first barrier.
   MC_DEV void item(const float2* a, const float2* b, const float* __restrict__, float wx,
                    float2* tout) {
+    const auto qa = buf_rsrc(a, N * 8), qb = buf_rsrc(b, N * 8);
     float2 va[EPT], vb[EPT];
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
       const int e = tid + i * NT;
       if (e < N) {
-        va[i] = a[e];
-        vb[i] = b[e];
+        va[i] = buf_ld2(qa, 8 * tid, 8 * NT * i);
+        vb[i] = buf_ld2(qb, 8 * tid, 8 * NT * i);
       }
     }
+    // the column weight wx is applied to the combined outputs
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
       const int e = tid + i * NT;
-      if (e < N) buf[(e % A) * TS + lix<PB>(e / A)] = cscale(cmulc(va[i], vb[i]), wx * wy[i]);
+      if (e < N) buf[(e % A) * TS + lix<PB>(e / A)] = cscale(cmulc(va[i], vb[i]), wy[i]);
     }
     __syncthreads();
     fft_lds_t<M, NT, 1, A, TS, TwTable, PB>(buf, TwTable{twm, M}, M);
@@ -136,7 +138,7 @@ struct ColGen {
       float2 acc = part[tid];
 #pragma unroll
       for (int h = 1; h < SP; h++) acc = cadd(acc, part[h * 256 + tid]);
-      tout[tid] = acc;
+      tout[tid] = cscale(acc, wx);
     }
   }
 };
@@ -233,26 +235,31 @@ struct ColM128 {
 #pragma unroll
     for (int i = 0; i < AS; i++) ctw[i] = tw[((i * SP + ch) * R) % N];
   }
-  // Raw stage-0 inputs of one item: the two spectrum columns at bin kx.
+  // Raw stage-0 inputs of one item: the two spectrum columns at bin kx
+  // (uniform pointers), as buffer loads: element a + A (j + 8 r) is at the
+  // lane's fixed byte offset 8 (a + A j) plus the scalar offset 64 A r.
   MC_DEV void fetch(float2 (&ra)[P0][16], float2 (&rb)[P0][16], const float2* sa,
                     const float2* sb) const {
+    const auto qa = buf_rsrc(sa, N * 8), qb = buf_rsrc(sb, N * 8);
 #pragma unroll
     for (int pp = 0; pp < P0; pp++) {
       const int q = tid + pp * NT;
       if (NB0 % NT == 0 || q < NB0) {
         const int a = q % A, j = q / A;
+        const int vo = 8 * (a + A * j);
 #pragma unroll
         for (int r = 0; r < 16; r++) {
-          const int e = a + A * (j + 8 * r);
-          ra[pp][r] = sa[e];
-          rb[pp][r] = sb[e];
+          ra[pp][r] = buf_ld2(qa, vo, 64 * A * r);
+          rb[pp][r] = buf_ld2(qb, vo, 64 * A * r);
         }
       }
     }
   }
-  // Stage 0 from the fetched registers into LDS (no barrier).
+  // Stage 0 from the fetched registers into LDS (no barrier).  The column
+  // weight is left to finish(): the transform is linear, so scaling its
+  // box outputs by wx replaces 16 products per lane.
   MC_DEV void stage0(const float2 (&ra)[P0][16], const float2 (&rb)[P0][16],
-                     const float* __restrict__ wxy, float wx) {
+                     const float* __restrict__ wxy) {
 #pragma unroll
     for (int pp = 0; pp < P0; pp++) {
       const int q = tid + pp * NT;
@@ -262,7 +269,7 @@ struct ColM128 {
 #pragma unroll
         for (int r = 0; r < 16; r++) {
           const float w = WYG ? wxy[H + a + A * (j + 8 * r)] : wy[pp][r];
-          v[r] = cscale(cmulc(ra[pp][r], rb[pp][r]), wx * w);
+          v[r] = cscale(cmulc(ra[pp][r], rb[pp][r]), w);
         }
         dft16<1>(v);
 #pragma unroll
@@ -270,9 +277,9 @@ struct ColM128 {
       }
     }
   }
-  // Stage 1 and the combine into tout[0, box); ends with a barrier (buf is
-  // rewritten by the next item's stage 0).
-  MC_DEV void finish(float2* tout) {
+  // Stage 1 and the combine into tout[0, box), scaled by the column weight
+  // wx; ends with a barrier (buf is rewritten by the next item's stage 0).
+  MC_DEV void finish(float2* tout, float wx) {
     __syncthreads();
     // ---- stage 1 in place (radix 8, Ns = 16)
 #pragma unroll
@@ -301,7 +308,7 @@ struct ColM128 {
       acc.x += __shfl_xor(acc.x, o);
       acc.y += __shfl_xor(acc.y, o);
     }
-    if (cact && ch == 0) tout[cr] = acc;
+    if (cact && ch == 0) tout[cr] = cscale(acc, wx);
     __syncthreads();
   }
   // One whole item (no prefetch).
@@ -309,8 +316,8 @@ struct ColM128 {
                    float2* tout) {
     float2 ra[P0][16], rb[P0][16];
     fetch(ra, rb, a, b);
-    stage0(ra, rb, wxy, wx);
-    finish(tout);
+    stage0(ra, rb, wxy);
+    finish(tout, wx);
   }
 };
 
@@ -346,9 +353,9 @@ __global__ __launch_bounds__(256, (N <= 4096 ? MINW : 1)) void k_pair_cols_m128(
   for (long long it = i0 + lid; it < i1; it += nper) {
     const int kx = kxb + (int)(it / npairs), p = (int)(it % npairs);
     if (!PF) fetch(it);
-    b.stage0(ra, rb, wxy, wxy[kx]);
+    b.stage0(ra, rb, wxy);
     if (PF && it + nper < i1) fetch(it + nper);
-    b.finish(tbuf + ((size_t)p * B::H + kx) * box);
+    b.finish(tbuf + ((size_t)p * B::H + kx) * box, wxy[kx]);
   }
 }
 
