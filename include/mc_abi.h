@@ -303,7 +303,8 @@ int mc_align_batch(mc_context ctx, const mc_batch_item *items, int count,
  * "cols", "rows"}); mc_tune_load reads the lines of the same device type
  * back (later lines win).  Without tuning, the defaults are 4x512 (4x256
  * below 4096), 192 MB of row-pass scratch per launch, the M=128 column
- * kernel without prefetch and the streaming FP32 MFMA row kernel. */
+ * pass (the batched plan kernel up to n = 3840, the 4-wave kernel at 4096,
+ * the plain one above) and the streaming FP32 MFMA row kernel. */
 typedef struct {
   int r2c_variant;        /* 1: 4x256, 2: 2x256, 3: 4x512; +8: persistent
                              column pass with register twiddles           */

@@ -407,7 +407,7 @@ MC_DEV long long plan_items(const float2* __restrict__ spectra, size_t spec_stri
 }
 
 constexpr int plan_key(int n, int sp) { return n | (sp << 14); }
-constexpr int kPlanLds = ColM128<4096, 2, true>::LDS;  // the largest body arena (float2)
+constexpr int kPlanLds = ColM128<3840, 2, true>::LDS;  // the largest body arena (float2)
 // The LDS arena of the plan kernels, shared by all bodies.
 __shared__ float2 plan_lds[kPlanLds];
 
@@ -419,14 +419,16 @@ __shared__ float2 plan_lds[kPlanLds];
 template <class B>
 MC_DEV long long plan_run(const PairColsPlan& pl, long long it, long long stop, int nper) {
   int tid = threadIdx.x;
-  if constexpr (B::NN != 4096) asm volatile("v_mov_b32 %0, %1" : "=v"(tid) : "v"(tid));
+  asm volatile("v_mov_b32 %0, %1" : "=v"(tid) : "v"(tid));
   return plan_items<B>(pl.spectra, pl.spec_stride, pl.pairs, pl.npairs, pl.wxy, pl.tw, pl.tbuf, it,
                        it - pl.item0, stop, nper, pl.box, plan_lds, tid);
 }
 
-// Two register classes: ColM128 at n <= 1536 and at 4096 fits the 128-VGPR
-// cap of 4 workgroups per CU and runs in k_pair_cols_plan<4>; at 1920..3840
-// it needs 150-180 VGPRs and runs in k_pair_cols_plan<2>.
+// Two register classes: ColM128 at n <= 1536 fits the 128-VGPR cap of 4
+// workgroups per CU and runs in k_pair_cols_plan<4>; at 1920..3840 it needs
+// 150-180 VGPRs and runs in k_pair_cols_plan<2>.  At n = 4096 the body needs
+// 120-125 VGPRs alone and spills inside the plan kernel, so movies of that
+// size take k_pair_cols_m128<4096, SP, false, 4> (single and batched alike).
 constexpr bool plan_wide(int n) { return n >= 1920 && n < 4096; }
 
 template <int MINW>
@@ -466,8 +468,7 @@ __global__ __launch_bounds__(256, MINW) void k_pair_cols_plan(PairColsPlan one,
       MC_PLAN_M(512, 4) MC_PLAN_M(512, 2) MC_PLAN_M(768, 2) MC_PLAN_M(1024, 4) MC_PLAN_M(1024, 2)
       MC_PLAN_M(1280, 2) MC_PLAN_M(1536, 4) MC_PLAN_M(1536, 2) MC_PLAN_M(1920, 1)
       MC_PLAN_M(2048, 4) MC_PLAN_M(2048, 2) MC_PLAN_M(2560, 4) MC_PLAN_M(2560, 2)
-      MC_PLAN_M(3072, 4) MC_PLAN_M(3072, 2) MC_PLAN_M(3840, 2) MC_PLAN_M(4096, 4)
-      MC_PLAN_M(4096, 2)
+      MC_PLAN_M(3072, 4) MC_PLAN_M(3072, 2) MC_PLAN_M(3840, 2)
 #undef MC_PLAN_M
       default:  // not reached: the host admits only the keys above
         it += (stop - it + nper - 1) / nper * nper;
@@ -1271,7 +1272,9 @@ static hipError_t pair_cols_n(const float2* spectra, size_t spec_stride, const i
                               hipStream_t st) {
   if constexpr (N >= 512) {
     if (variant != PAIR_COLS_GENERIC) {
-      const int mode = variant == PAIR_COLS_M128_PF ? 1 : (variant == PAIR_COLS_M128_W4 ? 2 : 0);
+      // the default at 4096 (no plan body there) is the 4-workgroup kernel
+      const bool w4 = variant == PAIR_COLS_M128_W4 || (variant == PAIR_COLS_AUTO && N == 4096);
+      const int mode = variant == PAIR_COLS_M128_PF ? 1 : (w4 ? 2 : 0);
       if constexpr ((N / 128) % 4 == 0) {
         if (box <= 64)
           return pair_cols_m128<N, 4>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, mode, st);
@@ -1398,7 +1401,7 @@ hipError_t launch_pair_cols(const float2* spectra, size_t spec_stride, const int
 
 int pair_cols_plan_body(int n, int box, int variant) {
   if (variant != PAIR_COLS_AUTO) return -1;
-  if (box < 1 || box > 128 || n < 512 || n > 4096 || n % 128 != 0 || !pair_cols_decimated(n))
+  if (box < 1 || box > 128 || n < 512 || n >= 4096 || n % 128 != 0 || !pair_cols_decimated(n))
     return -1;
   const int A = n / 128;
   return plan_key(n, (A % 4 == 0 && box <= 64) ? 4 : (A % 2 == 0 ? 2 : 1));
