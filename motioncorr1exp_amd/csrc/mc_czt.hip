@@ -159,6 +159,12 @@ constexpr int CZT_PB = 4;
 // Threads per workgroup: one radix-16 butterfly of stage 0 per thread at
 // the largest size (the 16384-point stages spill at 512 threads).
 template <int M> constexpr int czt_nt() { return M >= 16384 ? 1024 : (M >= 8192 ? 512 : 256); }
+// Minimum waves per SIMD the kernels are built for.  With 512 threads (M =
+// 8192) the LDS buffer admits two workgroups per CU, 4 waves per SIMD, which
+// needs at most 128 VGPRs: uncapped the kernels took 130-170, which is one
+// workgroup per CU.  With 256 threads (M <= 4096) the cap to 128 costs
+// spills and measured no faster than 3 workgroups per CU, so none is set.
+template <int NT> constexpr int czt_waves() { return NT >= 512 ? 4 : 1; }
 
 // Length-L DFT with sign S of buf[lix(j)], j < L (buf holds lds_len(M)
 // entries), in place; entries from L on are overwritten.  Caller
@@ -186,24 +192,43 @@ MC_DEV void bluestein(float2* buf, const TwSplit& tw, const float2* __restrict__
   __syncthreads();
 }
 
-// Stage-1 row pass of one canvas row y < h (any even nu): the row packed
-// into nu/2 complex values, a Bluestein DFT of length nu/2, unpack to the
-// kept bins kx < nkx, stored as RT[kx/4][y][kx%4] (the layout of the
-// power-of-two row pass; the column pass reads it the same way).
-template <int M, int NT, int DT>
-__global__ __launch_bounds__(NT) void k_r2c_rows_czt(R2CGeom g, int f0, int nkx, float2* rt,
-                                                     size_t rt_fstride, float* rowsum,
-                                                     const float2* __restrict__ gtw,
-                                                     const float2* __restrict__ chirpv,
-                                                     const float2* __restrict__ filt,
-                                                     const float2* __restrict__ twnu) {
+// The same with the M-point FFT twiddles of the calling thread held in
+// registers (forward ones; the inverse FFT uses their conjugates), for
+// persistent kernels whose threads keep their butterflies from item to item.
+template <int M, int NT>
+MC_DEV void bluestein_rw(float2* buf, const FftTw<M, NT, -1, 1>& ftw,
+                         const float2* __restrict__ chirp, const float2* __restrict__ filt, int L) {
+  constexpr int PB = CZT_PB;
+  for (int j = threadIdx.x; j < M; j += NT) {
+    const int lj = lix<PB>(j);
+    buf[lj] = j < L ? cmul(buf[lj], chirp[j]) : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_lds_rw<M, NT, -1, 1, M, PB>(buf, ftw);
+  for (int k = threadIdx.x; k < M; k += NT) {
+    const int lk = lix<PB>(k);
+    buf[lk] = cmul(buf[lk], filt[k]);
+  }
+  __syncthreads();
+  fft_lds_rw<M, NT, 1, 1, M, PB>(buf, ftw);
+  for (int k = threadIdx.x; k < L; k += NT) {
+    const int lk = lix<PB>(k);
+    buf[lk] = cmul(buf[lk], chirp[k]);
+  }
+  __syncthreads();
+}
+
+// Stage-1 row pass of one canvas row y < h of frame f (any even nu): the
+// row packed into nu/2 complex values, a Bluestein DFT of length nu/2 (bs),
+// unpack to the kept bins kx < nkx, stored as RT[kx/4][y][kx%4] (the layout
+// of the power-of-two row pass; the column pass reads it the same way).
+// Caller synchronises before; buf is read up to the return.
+template <int M, int NT, int DT, class BS>
+MC_DEV void row_czt(const R2CGeom& g, int f0, int f, int y, int nkx, float2* rt,
+                    size_t rt_fstride, float* rowsum, const float2* __restrict__ twnu,
+                    float2* buf, const BS& bs) {
   typedef typename Elem<DT>::T T;
   constexpr int PB = CZT_PB;
-  constexpr int LB = tw_lbits(M);
-  __shared__ float2 buf[lds_len<PB>(M)];
-  __shared__ float2 tlo[1 << LB], thi[M >> LB];
-  tw_split_load(tlo, thi, gtw, M, LB, threadIdx.x, NT);
-  const int f = blockIdx.y, y = blockIdx.x;
   const int NU = g.nu, N = NU / 2;
   const char* fbase = g.src + (size_t)(f0 + f) * g.frame_pitch;
   const bool corr = g.dark || g.gain || g.xs < 1.f || g.ys < 1.f;
@@ -218,7 +243,7 @@ __global__ __launch_bounds__(NT) void k_r2c_rows_czt(R2CGeom g, int f0, int nkx,
   };
   for (int m = threadIdx.x; m < N; m += NT) buf[lix<PB>(m)] = make_float2(px(2 * m), px(2 * m + 1));
   __syncthreads();
-  bluestein<M, NT, -1>(buf, TwSplit{tlo, thi, LB, M}, chirpv, filt, N);
+  bs(buf, N);
   // unpack, one thread per bin pair (k, N-k): X[k] = E[k] + e^{-2 pi i k/NU} O[k]
   for (int k = threadIdx.x; k <= N / 2; k += NT) {
     const int lk = lix<PB>(k), lm = lix<PB>(k ? N - k : 0);
@@ -249,12 +274,49 @@ __global__ __launch_bounds__(NT) void k_r2c_rows_czt(R2CGeom g, int f0, int nkx,
   }
 }
 
+// One workgroup per row, twiddles from the split LDS tables.
+template <int M, int NT, int DT>
+__global__ __launch_bounds__(NT, czt_waves<NT>()) void k_r2c_rows_czt(
+    R2CGeom g, int f0, int nkx, float2* rt, size_t rt_fstride, float* rowsum,
+    const float2* __restrict__ gtw, const float2* __restrict__ chirpv,
+    const float2* __restrict__ filt, const float2* __restrict__ twnu) {
+  constexpr int PB = CZT_PB;
+  constexpr int LB = tw_lbits(M);
+  __shared__ float2 buf[lds_len<PB>(M)];
+  __shared__ float2 tlo[1 << LB], thi[M >> LB];
+  tw_split_load(tlo, thi, gtw, M, LB, threadIdx.x, NT);
+  const TwSplit tw{tlo, thi, LB, M};
+  row_czt<M, NT, DT>(g, f0, blockIdx.y, blockIdx.x, nkx, rt, rt_fstride, rowsum, twnu, buf,
+                     [&](float2* b, int L) { bluestein<M, NT, -1>(b, tw, chirpv, filt, L); });
+}
+
+// Persistent form: each workgroup walks a contiguous range of the (frame,
+// row) items owned by its XCD with the FFT twiddles in registers.
+template <int M, int NT, int DT>
+__global__ __launch_bounds__(NT, czt_waves<NT>()) void k_r2c_rows_czt_p(
+    R2CGeom g, int f0, int nframes, int nkx, float2* rt, size_t rt_fstride, float* rowsum,
+    const float2* __restrict__ gtw, const float2* __restrict__ chirpv,
+    const float2* __restrict__ filt, const float2* __restrict__ twnu) {
+  constexpr int PB = CZT_PB;
+  __shared__ float2 buf[lds_len<PB>(M)];
+  FftTw<M, NT, -1, 1> ftw;
+  ftw.init(TwTable{gtw, M}, M);
+  const int nitems = g.h * nframes;
+  const int xcd = blockIdx.x % 8, lid = blockIdx.x / 8, nper = gridDim.x / 8;
+  const int i0 = (int)((long long)nitems * xcd / 8), i1 = (int)((long long)nitems * (xcd + 1) / 8);
+  for (int L = i0 + lid; L < i1; L += nper) {
+    row_czt<M, NT, DT>(g, f0, L / g.h, L % g.h, nkx, rt, rt_fstride, rowsum, twnu, buf,
+                       [&](float2* b, int n) { bluestein_rw<M, NT>(b, ftw, chirpv, filt, n); });
+    __syncthreads();  // buf is rewritten by the next row
+  }
+}
+
 // Stage-1 column pass of column kx (any even nu): rows [0, h) from the
 // row-pass scratch (the rest of the canvas is zero, or the fill in pad
 // mode, added as fill x the spectrum P of the fill mask), a Bluestein DFT
 // of length nu, and the rows of the Fourier crop to n written as S[kx][ky].
 template <int M, int NT>
-__global__ __launch_bounds__(NT) void k_c2c_cols_czt(const float2* rt, size_t rt_fstride, int nkx,
+__global__ __launch_bounds__(NT, czt_waves<NT>()) void k_c2c_cols_czt(const float2* rt, size_t rt_fstride, int nkx,
                                                      int NU, int h_data, const float* fill,
                                                      const float2* __restrict__ padP, int n,
                                                      float2* spec, size_t spec_stride,
@@ -294,13 +356,60 @@ __global__ __launch_bounds__(NT) void k_c2c_cols_czt(const float2* rt, size_t rt
   }
 }
 
+// Persistent form of k_c2c_cols_czt: each workgroup walks a contiguous
+// range of the (frame, kx) columns owned by its XCD (kx minor, so the four
+// columns of one row-pass line group are in flight together), with the FFT
+// twiddles in registers.  The LDS buffer allows two workgroups per CU at
+// M = 8192, so the registers the twiddles take cost no occupancy there.
+template <int M, int NT>
+__global__ __launch_bounds__(NT, czt_waves<NT>()) void k_c2c_cols_czt_p(
+    const float2* rt, size_t rt_fstride, int nkx, int nitems, int NU, int h_data,
+    const float* fill, const float2* __restrict__ padP, int n, float2* spec, size_t spec_stride,
+    const float2* __restrict__ gtw, const float2* __restrict__ chirpv,
+    const float2* __restrict__ filt) {
+  constexpr int PB = CZT_PB;
+  __shared__ float2 buf[lds_len<PB>(M)];
+  FftTw<M, NT, -1, 1> ftw;
+  ftw.init(TwTable{gtw, M}, M);
+  const int xcd = blockIdx.x % 8, lid = blockIdx.x / 8, nper = gridDim.x / 8;
+  const int i0 = (int)((long long)nitems * xcd / 8), i1 = (int)((long long)nitems * (xcd + 1) / 8);
+  for (int L = i0 + lid; L < i1; L += nper) {
+    const int kx = L % nkx, f = L / nkx;
+    const float2* src = rt + (size_t)f * rt_fstride + (size_t)(kx >> 2) * NU * 4 + (kx & 3);
+    float fv = 0.f;
+    float2 P = make_float2(0.f, 0.f);
+    if (fill) {
+      fv = fill[f];
+      P = padP[kx];
+    }
+    for (int i = threadIdx.x; i < NU; i += NT) {
+      float2 v = make_float2(0.f, 0.f);
+      if (i < h_data) {
+        v = src[(size_t)i * 4];
+        if (fill) v = cadd(v, cscale(P, fv));
+      } else if (fill && kx == 0) {
+        v.x = fv * (float)NU;
+      }
+      buf[lix<PB>(i)] = v;
+    }
+    __syncthreads();
+    bluestein_rw<M, NT>(buf, ftw, chirpv, filt, NU);
+    float2* dst = spec + (size_t)f * spec_stride + (size_t)kx * n;
+    for (int i = threadIdx.x; i < n; i += NT) {
+      const int si = i < n / 2 ? i : NU - n + i;
+      dst[i] = buf[lix<PB>(si)];
+    }
+    __syncthreads();  // buf is rewritten by the next column
+  }
+}
+
 // Pair-stage column pass (any even n): per (pair, kx) item, the weighted
 // cross power X[ky] = F_a[kx][ky] conj(F_b[kx][ky]) wx[kx] wy[ky], then the
 // pruned chirp-z inverse to the box rows, T[p][kx][u].  Persistent: each
 // XCD walks a contiguous range of items (kx-major, so the frame columns of
 // one kx are read from HBM once and reused from L2 by all pairs).
 template <int M, int NT>
-__global__ __launch_bounds__(NT) void k_pair_cols_czt(const float2* __restrict__ spectra,
+__global__ __launch_bounds__(NT, czt_waves<NT>()) void k_pair_cols_czt(const float2* __restrict__ spectra,
                                                       size_t spec_stride,
                                                       const int2* __restrict__ pairs, int npairs,
                                                       const float* __restrict__ wxy, int n,
@@ -349,7 +458,7 @@ __global__ __launch_bounds__(NT) void k_pair_cols_czt(const float2* __restrict__
 // Sum-image C2R, column pass (any even n): inverse DFT of S[kx][:] of
 // length n, stored as tmp[kx/4][ky][kx%4].
 template <int M, int NT>
-__global__ __launch_bounds__(NT) void k_cols_inv_czt(const float2* spec, int n, float2* tmp,
+__global__ __launch_bounds__(NT, czt_waves<NT>()) void k_cols_inv_czt(const float2* spec, int n, float2* tmp,
                                                      const float2* __restrict__ gtw,
                                                      const float2* __restrict__ chirpv,
                                                      const float2* __restrict__ filt) {
@@ -371,7 +480,7 @@ __global__ __launch_bounds__(NT) void k_cols_inv_czt(const float2* spec, int n, 
 // DC/Nyquist projection, packed into n/2 complex values, a Bluestein
 // inverse DFT of length n/2, x *= scale.
 template <int M, int NT>
-__global__ __launch_bounds__(NT) void k_c2r_rows_czt(const float2* tmp, int n, float scale,
+__global__ __launch_bounds__(NT, czt_waves<NT>()) void k_c2r_rows_czt(const float2* tmp, int n, float scale,
                                                      float* img, size_t pitch,
                                                      const float2* __restrict__ gtw,
                                                      const float2* __restrict__ chirpv,
@@ -419,6 +528,24 @@ static hipError_t rows_czt_t(const R2CGeom& g, int f0, int nframes, int nkx, flo
   constexpr int NT = czt_nt<M>();
   const float2* gtw = twiddles(M);
   if (!gtw) return hipErrorOutOfMemory;
+  if constexpr (M == 4096) {
+    // persistent, register twiddles
+    const int nitems = g.h * nframes;
+    switch (g.dtype) {
+#define MC_RCP(D)                                                                                  \
+  case D: {                                                                                        \
+    static const int grid = czt_resident_grid(k_r2c_rows_czt_p<M, NT, D>, NT, 1 << 30);           \
+    const int gr = std::min(grid, std::max(8, (nitems + 7) / 8 * 8));                             \
+    hipLaunchKernelGGL((k_r2c_rows_czt_p<M, NT, D>), dim3(gr), dim3(NT), 0, st, g, f0, nframes,   \
+                       nkx, rt, rt_fstride, rs, gtw, t.chirp, t.filt, twnu);                      \
+    break;                                                                                         \
+  }
+      MC_RCP(DT_INT8) MC_RCP(DT_UINT8) MC_RCP(DT_INT16) MC_RCP(DT_UINT16) MC_RCP(DT_FLOAT32)
+#undef MC_RCP
+      default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
   dim3 gr(g.h, nframes);
   switch (g.dtype) {
 #define MC_RC(D) case D: hipLaunchKernelGGL((k_r2c_rows_czt<M, NT, D>), gr, dim3(NT), 0, st, g, f0, nkx, rt, rt_fstride, rs, gtw, t.chirp, t.filt, twnu); break;
@@ -429,6 +556,21 @@ static hipError_t rows_czt_t(const R2CGeom& g, int f0, int nframes, int nkx, flo
   return hipGetLastError();
 }
 
+// Resident grid of a persistent kernel: CUs x workgroups per CU, a multiple
+// of 8 (one share per XCD), at most one workgroup per item.
+template <class K>
+static int czt_resident_grid(K kernel, int nt, int nitems) {
+  int dev = 0, cus = 256, per = 1;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, nt, 0) != hipSuccess || per < 1)
+    per = 1;
+  int g = (cus * per) / 8 * 8;
+  const int need = (nitems + 7) / 8 * 8;
+  if (g > need) g = need;
+  return g < 8 ? 8 : g;
+}
+
 template <int M>
 static hipError_t cols_czt_t(const R2CGeom& g, int nframes, int nkx, const float2* rt,
                              size_t rt_fstride, const float* fill, const float2* padP,
@@ -436,6 +578,16 @@ static hipError_t cols_czt_t(const R2CGeom& g, int nframes, int nkx, const float
   constexpr int NT = czt_nt<M>();
   const float2* gtw = twiddles(M);
   if (!gtw) return hipErrorOutOfMemory;
+  if constexpr (M == 8192) {
+    // persistent, register twiddles (two workgroups per CU: the LDS limit)
+    const int nitems = nkx * nframes;
+    static const int grid = czt_resident_grid(k_c2c_cols_czt_p<M, NT>, NT, 1 << 30);
+    const int gr = std::min(grid, std::max(8, (nitems + 7) / 8 * 8));
+    hipLaunchKernelGGL((k_c2c_cols_czt_p<M, NT>), dim3(gr), dim3(NT), 0, st, rt, rt_fstride,
+                       nkx, nitems, g.nu, g.h, fill, padP, g.n, spectra, spec_stride, gtw,
+                       t.chirp, t.filt);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL((k_c2c_cols_czt<M, NT>), dim3(nkx * nframes), dim3(NT), 0, st, rt,
                      rt_fstride, nkx, g.nu, g.h, fill, padP, g.n, spectra, spec_stride, gtw,
                      t.chirp, t.filt);

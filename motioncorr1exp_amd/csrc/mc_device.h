@@ -423,7 +423,7 @@ struct StageTw<N, NT, S, NB, STAGE, true> {
   }
 };
 
-template <int N, int NT, int S, int NB, int STAGE, int TS, int PB, class WT>
+template <int N, int NT, int S, int NB, int STAGE, int TS, int PB, class WT, bool CONJ = false>
 MC_DEV void fft_lds_stage_rw(float2* buf, const WT& tw) {
   constexpr int R = stage_radix(N, STAGE);
   constexpr int Ns = stage_ns(N, STAGE);
@@ -442,7 +442,8 @@ MC_DEV void fft_lds_stage_rw(float2* buf, const WT& tw) {
       for (int r = 0; r < R; r++) v[p][r] = src[lix_at<PB, N / R>(lj, j, r)];
       if constexpr (Ns > 1) {
 #pragma unroll
-        for (int r = 1; r < R; r++) v[p][r] = cmul(v[p][r], tw.w[p][r]);
+        for (int r = 1; r < R; r++)
+          v[p][r] = CONJ ? cmulc(v[p][r], tw.w[p][r]) : cmul(v[p][r], tw.w[p][r]);
       }
       dft<R, S>(v[p]);
     }
@@ -478,12 +479,19 @@ struct FftTw {
   }
 };
 
-template <int N, int NT, int S, int NB, int TS, int PB>
-MC_DEV void fft_lds_rw(float2* buf, const FftTw<N, NT, S, NB>& tw) {
-  fft_lds_stage_rw<N, NT, S, NB, 0, TS, PB>(buf, tw.s0);
-  if constexpr (stage_count(N) > 1) fft_lds_stage_rw<N, NT, S, NB, 1, TS, PB>(buf, tw.s1);
-  if constexpr (stage_count(N) > 2) fft_lds_stage_rw<N, NT, S, NB, 2, TS, PB>(buf, tw.s2);
-  if constexpr (stage_count(N) > 3) fft_lds_stage_rw<N, NT, S, NB, 3, TS, PB>(buf, tw.s3);
+// Sign S with twiddles held for sign SW: when they differ, the held
+// twiddles are conjugated on use (the inverse of the same plan from the
+// forward twiddles).
+template <int N, int NT, int S, int NB, int TS, int PB, int SW>
+MC_DEV void fft_lds_rw(float2* buf, const FftTw<N, NT, SW, NB>& tw) {
+  constexpr bool CJ = S != SW;
+  using W1 = StageTw<N, NT, SW, NB, 1>;
+  using W2 = StageTw<N, NT, SW, NB, 2>;
+  using W3 = StageTw<N, NT, SW, NB, 3>;
+  fft_lds_stage_rw<N, NT, S, NB, 0, TS, PB, StageTw<N, NT, SW, NB, 0>, CJ>(buf, tw.s0);
+  if constexpr (stage_count(N) > 1) fft_lds_stage_rw<N, NT, S, NB, 1, TS, PB, W1, CJ>(buf, tw.s1);
+  if constexpr (stage_count(N) > 2) fft_lds_stage_rw<N, NT, S, NB, 2, TS, PB, W2, CJ>(buf, tw.s2);
+  if constexpr (stage_count(N) > 3) fft_lds_stage_rw<N, NT, S, NB, 3, TS, PB, W3, CJ>(buf, tw.s3);
 }
 
 // -------------------------------------------- float results rounded exactly
