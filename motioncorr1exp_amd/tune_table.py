@@ -6,7 +6,8 @@ read back by mc_tune_load / Context.tune_load).
 Each geometry is tuned by mc_autotune on a synthetic movie of its shape
 generated in device memory (random-wave phantom, Poisson counts).  The
 default list covers the benchmark workloads: 3838x3710x40 float32 padded to
-4096; 4096x4096x24 uint8 with 2x binning; uint8 movies padded to every
+4096 and at its default square crop (n = 3710, chirp-z stage 1); 4096x4096x24
+uint8 with 2x binning; uint8 movies padded to every
 canvas size of the batched form (1024 ... 4096); 512x512x10 float32.
 """
 import argparse
@@ -20,6 +21,7 @@ from motioncorr1exp_amd.synth import make_movie_gpu
 # (nx, ny, frames, dtype, pad, bin, fod, box)
 GEOMETRIES = [
     (3838, 3710, 40, np.float32, 4096, 1.0, 2, 96),
+    (3838, 3710, 40, np.float32, 0, 1.0, 2, 96),
     (4096, 4096, 24, np.uint8, 0, 2.0, 11, 96),
     (4000, 4000, 30, np.uint8, 4096, 1.0, 2, 96),
     (3800, 3800, 30, np.uint8, 3840, 1.0, 2, 96),
