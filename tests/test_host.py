@@ -94,9 +94,12 @@ def test_shipped_tuning_table_is_well_formed():
     import os
 
     path = os.path.join(os.path.dirname(mc.__file__), "tune_mi355x.jsonl")
-    sizes = {1 << k for k in range(6, 14)} | {768, 1280, 1536, 1920, 2560, 3072, 3840, 5120, 6144}
+    # transform sizes: any even size from 64 to 8192 (Stockham plans for
+    # 2^k * {1, 3, 5, 15}, chirp-z for the rest, such as the 3710 crop)
+    sizes = set(range(64, 8193, 2))
     lines = open(path).read().splitlines()
     assert lines
+    assert any(json.loads(line)["n"] == 3710 for line in lines)
     for line in lines:
         e = json.loads(line)
         assert e["arch"] == "gfx950" and e["cus"] > 0
