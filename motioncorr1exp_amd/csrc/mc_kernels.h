@@ -119,8 +119,8 @@ struct PairColsPlan {
 };
 // The plan-kernel body launch_pair_cols(n, box, variant) runs at precision 0:
 // the default variant (PAIR_COLS_AUTO) of the M = 128 decimation at n in
-// [512, 4096] and box <= 128; -1 for other variants and sizes, which keep
-// their own kernels.
+// [512, 3840] and box <= 128; -1 for other variants and sizes, which keep
+// their own kernels (the default at 4096 is the 4-wave M = 128 kernel).
 int pair_cols_plan_body(int n, int box, int variant);
 // Register class of a body: 0 runs at 4 workgroups per CU, 1 at 2.  A plan
 // table holds the movies of one class; each class is its own launch sequence.
