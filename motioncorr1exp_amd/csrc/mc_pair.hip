@@ -55,17 +55,26 @@ typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 // ColGen: generic decimated pass, A = N/M inverse M-point FFTs in LDS
 // (fft_lds_t; M any Stockham length), combine by lane groups of SP lanes
 // whose lane h takes the terms a = h, h + SP, ... < A, summed through LDS.
-template <int N, int M, int NT, int SP>
+// F0 (for a small first radix R0, such as M = 70 = 2 5 7): the first
+// Stockham stage runs in registers on the loaded products.  Its butterfly
+// (a, j) takes the elements e + r N/R0 (e = a + A j < N/R0, r < R0), and
+// writes row a at j R0 + r, so the products skip one LDS round trip and two
+// barriers.
+template <int N, int M, int NT, int SP, bool F0 = false>
 struct ColGen {
   static constexpr int NN = N;
   static constexpr int A = N / M;
   static constexpr int H = N / 2 + 1;
   static constexpr int PB = 4;
   static constexpr int TS = lds_len<PB>(M) | 1;
-  static constexpr int EPT = (N + NT - 1) / NT;  // elements per thread
+  static constexpr int R0 = stage_radix(M, 0);
+  static constexpr int E0 = N / R0;              // first-stage butterflies (all rows)
+  static constexpr int EP0 = (E0 + NT - 1) / NT;  // ... per thread
+  static constexpr int EPT = F0 ? EP0 * R0 : (N + NT - 1) / NT;  // elements per thread
   static constexpr int AS = (A + SP - 1) / SP;   // combine terms per thread (at most)
   static constexpr int LDS = A * TS + M + SP * 256;  // float2 elements
   static_assert(A * M == N, "decimation");
+  static_assert(!F0 || (M % R0 == 0 && stage_count(M) >= 2), "first stage");
   float2* buf;
   float2* twm;
   float2* part;  // [SP][256]
@@ -86,8 +95,8 @@ struct ColGen {
     // ky weights (-1)^{y'} exp(m y'^2) / sizec for this thread's elements
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
-      const int e = tid + i * NT;
-      wy[i] = e < N ? wxy[H + e] : 0.f;
+      const int e = F0 ? tid + (i / R0) * NT + (i % R0) * E0 : tid + i * NT;
+      wy[i] = (F0 ? tid + (i / R0) * NT < E0 : e < N) ? wxy[H + e] : 0.f;
     }
     // combine: thread -> (output r, term block h)
     const int ori = (N - box) / 2;
@@ -109,22 +118,56 @@ struct ColGen {
                    float2* tout) {
     const auto qa = buf_rsrc(a, N * 8), qb = buf_rsrc(b, N * 8);
     float2 va[EPT], vb[EPT];
+    if constexpr (F0) {
 #pragma unroll
-    for (int i = 0; i < EPT; i++) {
-      const int e = tid + i * NT;
-      if (e < N) {
-        va[i] = buf_ld2(qa, 8 * tid, 8 * NT * i);
-        vb[i] = buf_ld2(qb, 8 * tid, 8 * NT * i);
+      for (int p = 0; p < EP0; p++) {
+        if (E0 % NT == 0 || tid + p * NT < E0) {
+#pragma unroll
+          for (int r = 0; r < R0; r++) {
+            va[p * R0 + r] = buf_ld2(qa, 8 * tid, 8 * (p * NT + r * E0));
+            vb[p * R0 + r] = buf_ld2(qb, 8 * tid, 8 * (p * NT + r * E0));
+          }
+        }
       }
-    }
-    // the column weight wx is applied to the combined outputs
+      // products (column weight wx applied to the combined outputs), then
+      // the first radix-R0 stage (Ns = 1: no twiddles) into the LDS rows
 #pragma unroll
-    for (int i = 0; i < EPT; i++) {
-      const int e = tid + i * NT;
-      if (e < N) buf[(e % A) * TS + lix<PB>(e / A)] = cscale(cmulc(va[i], vb[i]), wy[i]);
+      for (int p = 0; p < EP0; p++) {
+        const int e = tid + p * NT;
+        if (E0 % NT == 0 || e < E0) {
+          float2 v[R0];
+#pragma unroll
+          for (int r = 0; r < R0; r++) v[r] = cscale(cmulc(va[p * R0 + r], vb[p * R0 + r]), wy[p * R0 + r]);
+          dft<R0, 1>(v);
+          float2* row = buf + (e % A) * TS;
+          const int j = e / A;
+#pragma unroll
+          for (int r = 0; r < R0; r++) row[lix<PB>(j * R0 + r)] = v[r];
+        }
+      }
+      __syncthreads();
+      const TwTable twt{twm, M};
+      fft_lds_stage<M, NT, 1, A, 1, TS, TwTable, PB>(buf, twt, M);
+      if constexpr (stage_count(M) > 2) fft_lds_stage<M, NT, 1, A, 2, TS, TwTable, PB>(buf, twt, M);
+      if constexpr (stage_count(M) > 3) fft_lds_stage<M, NT, 1, A, 3, TS, TwTable, PB>(buf, twt, M);
+    } else {
+#pragma unroll
+      for (int i = 0; i < EPT; i++) {
+        const int e = tid + i * NT;
+        if (e < N) {
+          va[i] = buf_ld2(qa, 8 * tid, 8 * NT * i);
+          vb[i] = buf_ld2(qb, 8 * tid, 8 * NT * i);
+        }
+      }
+      // the column weight wx is applied to the combined outputs
+#pragma unroll
+      for (int i = 0; i < EPT; i++) {
+        const int e = tid + i * NT;
+        if (e < N) buf[(e % A) * TS + lix<PB>(e / A)] = cscale(cmulc(va[i], vb[i]), wy[i]);
+      }
+      __syncthreads();
+      fft_lds_t<M, NT, 1, A, TS, TwTable, PB>(buf, TwTable{twm, M}, M);
     }
-    __syncthreads();
-    fft_lds_t<M, NT, 1, A, TS, TwTable, PB>(buf, TwTable{twm, M}, M);
     if (cact) {
       float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
@@ -143,14 +186,14 @@ struct ColGen {
   }
 };
 
-template <int N, int M, int NT, int SP>
+template <int N, int M, int NT, int SP, bool F0 = false>
 __global__ __launch_bounds__(NT) void k_pair_cols(const float2* __restrict__ spectra,
                                                   size_t spec_stride,
                                                   const int2* __restrict__ pairs, int npairs,
                                                   const float* __restrict__ wxy, int box,
                                                   float2* __restrict__ tbuf,
                                                   const float2* __restrict__ tw) {
-  typedef ColGen<N, M, NT, SP> B;
+  typedef ColGen<N, M, NT, SP, F0> B;
   __shared__ float2 lds[B::LDS];
   B b;
   b.init(lds, wxy, tw, box, threadIdx.x);
@@ -1200,15 +1243,15 @@ static int persistent_grid(K kernel, int nt) {
   return std::max(8, (cus * per) / 8 * 8);
 }
 
-template <int N, int M, int SP>
+template <int N, int M, int SP, bool F0 = false>
 static hipError_t pair_cols_t(const float2* spectra, size_t spec_stride, const int2* pairs,
                               int npairs, const float* wxy, int box, float2* tbuf,
                               hipStream_t st) {
   constexpr int NT = 256;
   const float2* tw = twiddles(N);
   if (!tw) return hipErrorOutOfMemory;
-  static const int grid = persistent_grid(k_pair_cols<N, M, NT, SP>, NT);
-  hipLaunchKernelGGL((k_pair_cols<N, M, NT, SP>), dim3(grid), dim3(NT), 0, st, spectra,
+  static const int grid = persistent_grid(k_pair_cols<N, M, NT, SP, F0>, NT);
+  hipLaunchKernelGGL((k_pair_cols<N, M, NT, SP, F0>), dim3(grid), dim3(NT), 0, st, spectra,
                      spec_stride, pairs, npairs, wxy, box, tbuf, tw);
   return hipGetLastError();
 }
@@ -1356,8 +1399,9 @@ hipError_t launch_pair_cols(const float2* spectra, size_t spec_stride, const int
   }
   if (n == 3710 && box <= 128)
     // the default square crop of a 3838 x 3710 K2 frame: n = 70 * 53,
-    // decimated into 53 inverse 70-point FFTs (radix 2, 5, 7)
-    return pair_cols_t<3710, 70, 2>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
+    // decimated into 53 inverse 70-point FFTs (radix 2, 5, 7), the radix-2
+    // stage in registers on the loaded products
+    return pair_cols_t<3710, 70, 2, true>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
   if (!pair_cols_decimated(n))
     return launch_pair_cols_czt(spectra, spec_stride, pairs, npairs, wxy, n, box, tbuf, st);
   const int body = pair_cols_plan_body(n, box, variant);
