@@ -195,6 +195,8 @@ MC_DEV void bluestein(float2* buf, const TwSplit& tw, const float2* __restrict__
 // The same with the M-point FFT twiddles of the calling thread held in
 // registers (forward ones; the inverse FFT uses their conjugates), for
 // persistent kernels whose threads keep their butterflies from item to item.
+// It stops before the final product with chirp[k]: the caller applies it as
+// it reads the result (one LDS pass and one barrier fewer).
 template <int M, int NT>
 MC_DEV void bluestein_rw(float2* buf, const FftTw<M, NT, -1, 1>& ftw,
                          const float2* __restrict__ chirp, const float2* __restrict__ filt, int L) {
@@ -211,11 +213,6 @@ MC_DEV void bluestein_rw(float2* buf, const FftTw<M, NT, -1, 1>& ftw,
   }
   __syncthreads();
   fft_lds_rw<M, NT, 1, 1, M, PB>(buf, ftw);
-  for (int k = threadIdx.x; k < L; k += NT) {
-    const int lk = lix<PB>(k);
-    buf[lk] = cmul(buf[lk], chirp[k]);
-  }
-  __syncthreads();
 }
 
 // Stage-1 row pass of one canvas row y < h of frame f (any even nu): the
@@ -226,7 +223,7 @@ MC_DEV void bluestein_rw(float2* buf, const FftTw<M, NT, -1, 1>& ftw,
 template <int M, int NT, int DT, class BS>
 MC_DEV void row_czt(const R2CGeom& g, int f0, int f, int y, int nkx, float2* rt,
                     size_t rt_fstride, float* rowsum, const float2* __restrict__ twnu,
-                    float2* buf, const BS& bs) {
+                    float2* buf, const BS& bs, const float2* __restrict__ post = nullptr) {
   typedef typename Elem<DT>::T T;
   constexpr int PB = CZT_PB;
   const int NU = g.nu, N = NU / 2;
@@ -247,7 +244,11 @@ MC_DEV void row_czt(const R2CGeom& g, int f0, int f, int y, int nkx, float2* rt,
   // unpack, one thread per bin pair (k, N-k): X[k] = E[k] + e^{-2 pi i k/NU} O[k]
   for (int k = threadIdx.x; k <= N / 2; k += NT) {
     const int lk = lix<PB>(k), lm = lix<PB>(k ? N - k : 0);
-    const float2 zk = buf[lk], zm = buf[lm];
+    float2 zk = buf[lk], zm = buf[lm];
+    if (post) {  // the final chirp product of the Bluestein DFT
+      zk = cmul(zk, post[k]);
+      zm = cmul(zm, post[k ? N - k : 0]);
+    }
     {
       const float2 zc = cconj(zm);
       const float2 E = cscale(cadd(zk, zc), 0.5f);
@@ -306,7 +307,8 @@ __global__ __launch_bounds__(NT, czt_waves<NT>()) void k_r2c_rows_czt_p(
   const int i0 = (int)((long long)nitems * xcd / 8), i1 = (int)((long long)nitems * (xcd + 1) / 8);
   for (int L = i0 + lid; L < i1; L += nper) {
     row_czt<M, NT, DT>(g, f0, L / g.h, L % g.h, nkx, rt, rt_fstride, rowsum, twnu, buf,
-                       [&](float2* b, int n) { bluestein_rw<M, NT>(b, ftw, chirpv, filt, n); });
+                       [&](float2* b, int n) { bluestein_rw<M, NT>(b, ftw, chirpv, filt, n); },
+                       chirpv);
     __syncthreads();  // buf is rewritten by the next row
   }
 }
@@ -397,7 +399,7 @@ __global__ __launch_bounds__(NT, czt_waves<NT>()) void k_c2c_cols_czt_p(
     float2* dst = spec + (size_t)f * spec_stride + (size_t)kx * n;
     for (int i = threadIdx.x; i < n; i += NT) {
       const int si = i < n / 2 ? i : NU - n + i;
-      dst[i] = buf[lix<PB>(si)];
+      dst[i] = cmul(buf[lix<PB>(si)], chirpv[si]);  // with the final chirp product
     }
     __syncthreads();  // buf is rewritten by the next column
   }
