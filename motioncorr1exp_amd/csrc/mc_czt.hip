@@ -195,17 +195,14 @@ MC_DEV void bluestein(float2* buf, const TwSplit& tw, const float2* __restrict__
 // The same with the M-point FFT twiddles of the calling thread held in
 // registers (forward ones; the inverse FFT uses their conjugates), for
 // persistent kernels whose threads keep their butterflies from item to item.
-// It stops before the final product with chirp[k]: the caller applies it as
-// it reads the result (one LDS pass and one barrier fewer).
+// The caller applies both chirp products as it writes the input and reads
+// the result (two LDS passes and two barriers fewer): buf holds x[j] chirp[j]
+// for j < L and zeros up to M, synchronised; on return buf[k] chirp[k] is the
+// DFT, k < L.
 template <int M, int NT>
 MC_DEV void bluestein_rw(float2* buf, const FftTw<M, NT, -1, 1>& ftw,
-                         const float2* __restrict__ chirp, const float2* __restrict__ filt, int L) {
+                         const float2* __restrict__ filt) {
   constexpr int PB = CZT_PB;
-  for (int j = threadIdx.x; j < M; j += NT) {
-    const int lj = lix<PB>(j);
-    buf[lj] = j < L ? cmul(buf[lj], chirp[j]) : make_float2(0.f, 0.f);
-  }
-  __syncthreads();
   fft_lds_rw<M, NT, -1, 1, M, PB>(buf, ftw);
   for (int k = threadIdx.x; k < M; k += NT) {
     const int lk = lix<PB>(k);
@@ -223,7 +220,7 @@ MC_DEV void bluestein_rw(float2* buf, const FftTw<M, NT, -1, 1>& ftw,
 template <int M, int NT, int DT, class BS>
 MC_DEV void row_czt(const R2CGeom& g, int f0, int f, int y, int nkx, float2* rt,
                     size_t rt_fstride, float* rowsum, const float2* __restrict__ twnu,
-                    float2* buf, const BS& bs, const float2* __restrict__ post = nullptr) {
+                    float2* buf, const BS& bs, const float2* __restrict__ chirp = nullptr) {
   typedef typename Elem<DT>::T T;
   constexpr int PB = CZT_PB;
   const int NU = g.nu, N = NU / 2;
@@ -238,16 +235,22 @@ MC_DEV void row_czt(const R2CGeom& g, int f0, int f, int y, int nkx, float2* rt,
     const size_t idx = (size_t)sp.y * g.nx + sp.x;
     return correct(g, v, g.dark ? g.dark[idx] : 0.f, g.gain ? g.gain[idx] : 1.f);
   };
-  for (int m = threadIdx.x; m < N; m += NT) buf[lix<PB>(m)] = make_float2(px(2 * m), px(2 * m + 1));
+  if (chirp) {  // the first chirp product of the Bluestein DFT, zeros up to M
+    for (int m = threadIdx.x; m < M; m += NT)
+      buf[lix<PB>(m)] = m < N ? cmul(make_float2(px(2 * m), px(2 * m + 1)), chirp[m])
+                              : make_float2(0.f, 0.f);
+  } else {
+    for (int m = threadIdx.x; m < N; m += NT) buf[lix<PB>(m)] = make_float2(px(2 * m), px(2 * m + 1));
+  }
   __syncthreads();
   bs(buf, N);
   // unpack, one thread per bin pair (k, N-k): X[k] = E[k] + e^{-2 pi i k/NU} O[k]
   for (int k = threadIdx.x; k <= N / 2; k += NT) {
     const int lk = lix<PB>(k), lm = lix<PB>(k ? N - k : 0);
     float2 zk = buf[lk], zm = buf[lm];
-    if (post) {  // the final chirp product of the Bluestein DFT
-      zk = cmul(zk, post[k]);
-      zm = cmul(zm, post[k ? N - k : 0]);
+    if (chirp) {  // the final chirp product of the Bluestein DFT
+      zk = cmul(zk, chirp[k]);
+      zm = cmul(zm, chirp[k ? N - k : 0]);
     }
     {
       const float2 zc = cconj(zm);
@@ -307,8 +310,7 @@ __global__ __launch_bounds__(NT, czt_waves<NT>()) void k_r2c_rows_czt_p(
   const int i0 = (int)((long long)nitems * xcd / 8), i1 = (int)((long long)nitems * (xcd + 1) / 8);
   for (int L = i0 + lid; L < i1; L += nper) {
     row_czt<M, NT, DT>(g, f0, L / g.h, L % g.h, nkx, rt, rt_fstride, rowsum, twnu, buf,
-                       [&](float2* b, int n) { bluestein_rw<M, NT>(b, ftw, chirpv, filt, n); },
-                       chirpv);
+                       [&](float2* b, int) { bluestein_rw<M, NT>(b, ftw, filt); }, chirpv);
     __syncthreads();  // buf is rewritten by the next row
   }
 }
@@ -384,18 +386,19 @@ __global__ __launch_bounds__(NT, czt_waves<NT>()) void k_c2c_cols_czt_p(
       fv = fill[f];
       P = padP[kx];
     }
-    for (int i = threadIdx.x; i < NU; i += NT) {
+    // with the first chirp product of the Bluestein DFT, zeros up to M
+    for (int i = threadIdx.x; i < M; i += NT) {
       float2 v = make_float2(0.f, 0.f);
       if (i < h_data) {
         v = src[(size_t)i * 4];
         if (fill) v = cadd(v, cscale(P, fv));
-      } else if (fill && kx == 0) {
+      } else if (fill && kx == 0 && i < NU) {
         v.x = fv * (float)NU;
       }
-      buf[lix<PB>(i)] = v;
+      buf[lix<PB>(i)] = i < NU ? cmul(v, chirpv[i]) : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    bluestein_rw<M, NT>(buf, ftw, chirpv, filt, NU);
+    bluestein_rw<M, NT>(buf, ftw, filt);
     float2* dst = spec + (size_t)f * spec_stride + (size_t)kx * n;
     for (int i = threadIdx.x; i < n; i += NT) {
       const int si = i < n / 2 ? i : NU - n + i;
