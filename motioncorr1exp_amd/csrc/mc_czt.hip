@@ -539,7 +539,7 @@ static hipError_t rows_czt_t(const R2CGeom& g, int f0, int nframes, int nkx, flo
     switch (g.dtype) {
 #define MC_RCP(D)                                                                                  \
   case D: {                                                                                        \
-    static const int grid = czt_resident_grid(k_r2c_rows_czt_p<M, NT, D>, NT, 1 << 30);           \
+    static const int grid = persistent_grid(k_r2c_rows_czt_p<M, NT, D>, NT);                      \
     const int gr = std::min(grid, std::max(8, (nitems + 7) / 8 * 8));                             \
     hipLaunchKernelGGL((k_r2c_rows_czt_p<M, NT, D>), dim3(gr), dim3(NT), 0, st, g, f0, nframes,   \
                        nkx, rt, rt_fstride, rs, gtw, t.chirp, t.filt, twnu);                      \
@@ -561,21 +561,6 @@ static hipError_t rows_czt_t(const R2CGeom& g, int f0, int nframes, int nkx, flo
   return hipGetLastError();
 }
 
-// Resident grid of a persistent kernel: CUs x workgroups per CU, a multiple
-// of 8 (one share per XCD), at most one workgroup per item.
-template <class K>
-static int czt_resident_grid(K kernel, int nt, int nitems) {
-  int dev = 0, cus = 256, per = 1;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, nt, 0) != hipSuccess || per < 1)
-    per = 1;
-  int g = (cus * per) / 8 * 8;
-  const int need = (nitems + 7) / 8 * 8;
-  if (g > need) g = need;
-  return g < 8 ? 8 : g;
-}
-
 template <int M>
 static hipError_t cols_czt_t(const R2CGeom& g, int nframes, int nkx, const float2* rt,
                              size_t rt_fstride, const float* fill, const float2* padP,
@@ -586,7 +571,7 @@ static hipError_t cols_czt_t(const R2CGeom& g, int nframes, int nkx, const float
   if constexpr (M == 8192) {
     // persistent, register twiddles (two workgroups per CU: the LDS limit)
     const int nitems = nkx * nframes;
-    static const int grid = czt_resident_grid(k_c2c_cols_czt_p<M, NT>, NT, 1 << 30);
+    static const int grid = persistent_grid(k_c2c_cols_czt_p<M, NT>, NT);
     const int gr = std::min(grid, std::max(8, (nitems + 7) / 8 * 8));
     hipLaunchKernelGGL((k_c2c_cols_czt_p<M, NT>), dim3(gr), dim3(NT), 0, st, rt, rt_fstride,
                        nkx, nitems, g.nu, g.h, fill, padP, g.n, spectra, spec_stride, gtw,
@@ -641,13 +626,7 @@ static hipError_t pair_cols_czt_t(const float2* spectra, size_t spec_stride, con
   constexpr int NT = czt_nt<M>();
   const float2* gtw = twiddles(M);
   if (!gtw) return hipErrorOutOfMemory;
-  int dev = 0, cus = 256, per = 1;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pair_cols_czt<M, NT>, NT, 0) !=
-          hipSuccess || per < 1)
-    per = 1;
-  const int grid = std::max(8, (cus * per) / 8 * 8);
+  const int grid = persistent_grid(k_pair_cols_czt<M, NT>, NT);
   const int H = n / 2 + 1;
   hipLaunchKernelGGL((k_pair_cols_czt<M, NT>), dim3(grid), dim3(NT), 0, st, spectra, spec_stride,
                      pairs, npairs, wxy, n, box, tbuf, gtw, t.pre, t.filt, t.post, 0, H);

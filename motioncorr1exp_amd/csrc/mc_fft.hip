@@ -405,19 +405,6 @@ __global__ __launch_bounds__(NT) void k_c2c_cols_p(const float2* rt, size_t rt_f
   }
 }
 
-template <class K>
-static int resident_grid(K kernel, int nt, int nitems) {
-  int dev = 0, cus = 256, per = 1;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, nt, 0) != hipSuccess || per < 1)
-    per = 1;
-  int g = (cus * per) / 8 * 8;
-  const int need = (nitems + 7) / 8 * 8;
-  if (g > need) g = need;
-  return g < 8 ? 8 : g;
-}
-
 template <int NU, int ROWS, int NT>
 static hipError_t launch_rows(const R2CGeom& g, int f0, int nframes, int nkx, float2* rt,
                               size_t rt_fstride, float* rs, const float2* tw, hipStream_t st) {

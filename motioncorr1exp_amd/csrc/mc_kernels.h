@@ -6,6 +6,24 @@
 
 namespace mc {
 
+// Grid of a persistent kernel on the current device: CUs x resident
+// workgroups per CU (occupancy query), a multiple of 8 so that every XCD
+// gets the same share (workgroup b runs on XCD b % 8), at least 8.
+template <class K> inline int persistent_grid(K kernel, int nt) {
+  int dev = 0, cus = 256, per = 1;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, nt, 0) != hipSuccess || per < 1)
+    per = 1;
+  const int g = (cus * per) / 8 * 8;
+  return g < 8 ? 8 : g;
+}
+// The same, with at most one workgroup per item (rounded up to 8).
+template <class K> inline int resident_grid(K kernel, int nt, int nitems) {
+  const int g = persistent_grid(kernel, nt), need = (nitems + 7) / 8 * 8;
+  return need < 8 ? 8 : (g < need ? g : need);
+}
+
 // Element type codes of frame data (match MC_DTYPE_* in include/mc_abi.h).
 enum { DT_INT8 = 0, DT_INT16 = 1, DT_FLOAT32 = 2, DT_UINT16 = 6, DT_UINT8 = 100 };
 

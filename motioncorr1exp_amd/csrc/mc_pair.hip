@@ -1231,17 +1231,6 @@ size_t pair_rows_part_elems(int npairs, int n, int box) {
   return ks > 1 ? (size_t)npairs * ks * box * box : 0;
 }
 
-// Persistent grid: every workgroup resident at once (the item ranges are
-// static), a multiple of 8 so that each XCD gets the same number.
-template <class K>
-static int persistent_grid(K kernel, int nt) {
-  int dev = 0, cus = 256, per = 1;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, nt, 0) != hipSuccess || per < 1)
-    per = 1;
-  return std::max(8, (cus * per) / 8 * 8);
-}
 
 template <int N, int M, int SP, bool F0 = false>
 static hipError_t pair_cols_t(const float2* spectra, size_t spec_stride, const int2* pairs,
