@@ -55,26 +55,27 @@ typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 // ColGen: generic decimated pass, A = N/M inverse M-point FFTs in LDS
 // (fft_lds_t; M any Stockham length), combine by lane groups of SP lanes
 // whose lane h takes the terms a = h, h + SP, ... < A, summed through LDS.
-// F0 (for a small first radix R0, such as M = 70 = 2 5 7): the first
-// Stockham stage runs in registers on the loaded products.  Its butterfly
-// (a, j) takes the elements e + r N/R0 (e = a + A j < N/R0, r < R0), and
-// writes row a at j R0 + r, so the products skip one LDS round trip and two
-// barriers.
-template <int N, int M, int NT, int SP, bool F0 = false>
+// F0 (for a small first radix R0, such as M = 70 = 2 5 7, or M = 128 under
+// stage plan PL = 1, 8 16): the first Stockham stage runs in registers on
+// the loaded products.  Its butterfly (a, j) takes the elements e + r N/R0
+// (e = a + A j < N/R0, r < R0), and writes row a at j R0 + r, so the
+// products skip one LDS round trip and two barriers.
+template <int N, int M, int NT, int SP, bool F0 = false, int PL = 0>
 struct ColGen {
   static constexpr int NN = N;
   static constexpr int A = N / M;
   static constexpr int H = N / 2 + 1;
   static constexpr int PB = 4;
   static constexpr int TS = lds_len<PB>(M) | 1;
-  static constexpr int R0 = stage_radix(M, 0);
+  static constexpr int R0 = stage_radix(M, 0, PL);
   static constexpr int E0 = N / R0;              // first-stage butterflies (all rows)
   static constexpr int EP0 = (E0 + NT - 1) / NT;  // ... per thread
   static constexpr int EPT = F0 ? EP0 * R0 : (N + NT - 1) / NT;  // elements per thread
   static constexpr int AS = (A + SP - 1) / SP;   // combine terms per thread (at most)
   static constexpr int LDS = A * TS + M + SP * 256;  // float2 elements
   static_assert(A * M == N, "decimation");
-  static_assert(!F0 || (M % R0 == 0 && stage_count(M) >= 2), "first stage");
+  static_assert(!F0 || (M % R0 == 0 && stage_count(M, PL) >= 2), "first stage");
+  static_assert(F0 || PL == 0, "plan");
   float2* buf;
   float2* twm;
   float2* part;  // [SP][256]
@@ -147,9 +148,9 @@ struct ColGen {
       }
       __syncthreads();
       const TwTable twt{twm, M};
-      fft_lds_stage<M, NT, 1, A, 1, TS, TwTable, PB>(buf, twt, M);
-      if constexpr (stage_count(M) > 2) fft_lds_stage<M, NT, 1, A, 2, TS, TwTable, PB>(buf, twt, M);
-      if constexpr (stage_count(M) > 3) fft_lds_stage<M, NT, 1, A, 3, TS, TwTable, PB>(buf, twt, M);
+      fft_lds_stage<M, NT, 1, A, 1, TS, TwTable, PB, PL>(buf, twt, M);
+      if constexpr (stage_count(M, PL) > 2) fft_lds_stage<M, NT, 1, A, 2, TS, TwTable, PB, PL>(buf, twt, M);
+      if constexpr (stage_count(M, PL) > 3) fft_lds_stage<M, NT, 1, A, 3, TS, TwTable, PB, PL>(buf, twt, M);
     } else {
 #pragma unroll
       for (int i = 0; i < EPT; i++) {
@@ -186,14 +187,14 @@ struct ColGen {
   }
 };
 
-template <int N, int M, int NT, int SP, bool F0 = false>
+template <int N, int M, int NT, int SP, bool F0 = false, int PL = 0>
 __global__ __launch_bounds__(NT) void k_pair_cols(const float2* __restrict__ spectra,
                                                   size_t spec_stride,
                                                   const int2* __restrict__ pairs, int npairs,
                                                   const float* __restrict__ wxy, int box,
                                                   float2* __restrict__ tbuf,
                                                   const float2* __restrict__ tw) {
-  typedef ColGen<N, M, NT, SP, F0> B;
+  typedef ColGen<N, M, NT, SP, F0, PL> B;
   __shared__ float2 lds[B::LDS];
   B b;
   b.init(lds, wxy, tw, box, threadIdx.x);
@@ -1232,15 +1233,15 @@ size_t pair_rows_part_elems(int npairs, int n, int box) {
 }
 
 
-template <int N, int M, int SP, bool F0 = false>
+template <int N, int M, int SP, bool F0 = false, int PL = 0>
 static hipError_t pair_cols_t(const float2* spectra, size_t spec_stride, const int2* pairs,
                               int npairs, const float* wxy, int box, float2* tbuf,
                               hipStream_t st) {
   constexpr int NT = 256;
   const float2* tw = twiddles(N);
   if (!tw) return hipErrorOutOfMemory;
-  static const int grid = persistent_grid(k_pair_cols<N, M, NT, SP, F0>, NT);
-  hipLaunchKernelGGL((k_pair_cols<N, M, NT, SP, F0>), dim3(grid), dim3(NT), 0, st, spectra,
+  static const int grid = persistent_grid(k_pair_cols<N, M, NT, SP, F0, PL>, NT);
+  hipLaunchKernelGGL((k_pair_cols<N, M, NT, SP, F0, PL>), dim3(grid), dim3(NT), 0, st, spectra,
                      spec_stride, pairs, npairs, wxy, box, tbuf, tw);
   return hipGetLastError();
 }
@@ -1320,17 +1321,19 @@ static hipError_t pair_cols_n(const float2* spectra, size_t spec_stride, const i
       }
     }
   }
+  // the generic pass at M = 128: first stage (radix 8 of plan 1, 8 x 16) in
+  // registers on the loaded products
   if constexpr (N % 512 == 0) {
     if (box <= 64)
-      return pair_cols_t<N, 128, 4>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
+      return pair_cols_t<N, 128, 4, true, 1>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
   }
   if constexpr (N % 256 == 0) {
     if (box <= 128)
-      return pair_cols_t<N, 128, 2>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
+      return pair_cols_t<N, 128, 2, true, 1>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
   }
   if constexpr (N % 128 == 0) {
     if (box <= 128)
-      return pair_cols_t<N, 128, 1>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
+      return pair_cols_t<N, 128, 1, true, 1>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
   }
   if constexpr (N % 256 == 0) {
     if (box <= 256)
