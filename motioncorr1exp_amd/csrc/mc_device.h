@@ -189,21 +189,59 @@ template <int S> MC_DEV void dft7(float2* v) {
   v[4] = csub(r3, i3);
 }
 
+// X[k] = sum_n v[n] W11^{S n k}: the pairs v[m] +- v[11-m] (m = 1..5) with
+// the cosines and sines of 2 pi j / 11, j = m k mod 11 (cos(2 pi (11-j)/11)
+// = cos(2 pi j/11), sin(2 pi (11-j)/11) = -sin(2 pi j/11)).
+template <int S> MC_DEV void dft11(float2* v) {
+  const float C[6] = {1.f, 0.84125353283118116886f, 0.41541501300188642553f,
+                      -0.14231483827328514044f, -0.65486073394528506406f,
+                      -0.95949297361449738989f};
+  const float SN[6] = {0.f, 0.54064081745559758211f, 0.90963199535451837141f,
+                       0.98982144188093273238f, 0.75574957435425828377f,
+                       0.28173255684142969771f};
+  float2 p[6], m[6];
+  const float2 a = v[0];
+  float2 x0 = a;
+#pragma unroll
+  for (int j = 1; j <= 5; j++) {
+    p[j] = cadd(v[j], v[11 - j]);
+    m[j] = csub(v[j], v[11 - j]);
+    x0 = cadd(x0, p[j]);
+  }
+#pragma unroll
+  for (int k = 1; k <= 5; k++) {
+    float2 r = a, q = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int j = 1; j <= 5; j++) {
+      const int e = (j * k) % 11;
+      const float c = C[e <= 5 ? e : 11 - e];
+      const float sn = e <= 5 ? SN[e] : -SN[11 - e];
+      r = make_float2(r.x + c * p[j].x, r.y + c * p[j].y);
+      q = make_float2(q.x + sn * m[j].x, q.y + sn * m[j].y);
+    }
+    const float2 iq = make_float2(-S * q.y, S * q.x);  // S i q
+    v[k] = cadd(r, iq);
+    v[11 - k] = csub(r, iq);
+  }
+  v[0] = x0;
+}
+
 template <int R, int S> MC_DEV void dft(float2* v) {
   if constexpr (R == 2) dft2<S>(v);
   else if constexpr (R == 3) dft3<S>(v);
   else if constexpr (R == 4) dft4<S>(v);
   else if constexpr (R == 5) dft5<S>(v);
   else if constexpr (R == 7) dft7<S>(v);
+  else if constexpr (R == 11) dft11<S>(v);
   else if constexpr (R == 8) dft8<S>(v);
   else if constexpr (R == 16) dft16<S>(v);
-  else static_assert(R == 2 || R == 3 || R == 4 || R == 5 || R == 7 || R == 8 || R == 16, "radix");
+  else static_assert(R == 2 || R == 3 || R == 4 || R == 5 || R == 7 || R == 8 || R == 11 || R == 16, "radix");
 }
 
 // ---------------------------------------------------- Stockham stage plans
-// Lengths N = 2^k * m, m a product of distinct factors 3, 5 and 7.
-// Power-of-two plans are balanced; the odd factors add a radix-3, radix-5
-// and/or radix-7 stage after the plan of 2^k.
+// Lengths N = 2^k * m, m a product of distinct factors 3, 5, 7 and 11.
+// Power-of-two plans are balanced; the odd factors add a radix-3, radix-5,
+// radix-7 and/or radix-11 stage after the plan of 2^k.
 __host__ __device__ constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
 __host__ __device__ constexpr bool is_pow2c(int n) { return n > 0 && (n & (n - 1)) == 0; }
 __host__ __device__ constexpr int pow2_stage_count(int N) {
@@ -250,22 +288,33 @@ __host__ __device__ constexpr int plan_p2_radix(int N, int s, int P) {
 }
 __host__ __device__ constexpr int odd_part(int N) { return N % 2 ? N : odd_part(N / 2); }
 // Stages of a length N = 2^k * m under plan P: the power-of-two stages, then
-// one stage per odd factor of m, in the order 3, 5, 7.
-__host__ __device__ constexpr int odd_radix(int m, int i) {
-  // the i-th of the factors 3, 5, 7 that divide m
-  return i == 0 ? (m % 3 == 0 ? 3 : (m % 5 == 0 ? 5 : 7))
-       : i == 1 ? (m % 3 == 0 ? (m % 5 == 0 ? 5 : 7) : 7)
-                : 7;
+// one stage per odd factor of m, in the order 3, 5, 7, 11.
+__host__ __device__ constexpr int odd_factor(int k) {
+  return k == 0 ? 3 : (k == 1 ? 5 : (k == 2 ? 7 : 11));
 }
+__host__ __device__ constexpr int odd_radix_from(int m, int i, int k) {
+  return k > 3 ? 11
+       : m % odd_factor(k) != 0 ? odd_radix_from(m, i, k + 1)
+       : i == 0 ? odd_factor(k)
+                : odd_radix_from(m, i - 1, k + 1);
+}
+// the i-th of the factors 3, 5, 7, 11 that divide m
+__host__ __device__ constexpr int odd_radix(int m, int i) { return odd_radix_from(m, i, 0); }
 __host__ __device__ constexpr int stage_count(int N, int P = 0) {
   return plan_p2_count(N / odd_part(N), P) + (odd_part(N) % 3 == 0 ? 1 : 0) +
-         (odd_part(N) % 5 == 0 ? 1 : 0) + (odd_part(N) % 7 == 0 ? 1 : 0);
+         (odd_part(N) % 5 == 0 ? 1 : 0) + (odd_part(N) % 7 == 0 ? 1 : 0) +
+         (odd_part(N) % 11 == 0 ? 1 : 0);
 }
 __host__ __device__ constexpr int stage_radix(int N, int s, int P = 0) {
   return s < plan_p2_count(N / odd_part(N), P)
              ? plan_p2_radix(N / odd_part(N), s, P)
              : odd_radix(odd_part(N), s - plan_p2_count(N / odd_part(N), P));
 }
+static_assert(stage_radix(70, 0) == 2 && stage_radix(70, 1) == 5 && stage_radix(70, 2) == 7 &&
+                  stage_radix(3840, 2) == 3 && stage_radix(3840, 3) == 5 &&
+                  stage_count(132) == 3 && stage_radix(132, 0) == 4 &&
+                  stage_radix(132, 1) == 3 && stage_radix(132, 2) == 11,
+              "stage plans");
 // Table index modulo the table size (any size; a power of two masks).
 MC_DEV int tw_mod(int e, int ntw) { return (ntw & (ntw - 1)) == 0 ? (e & (ntw - 1)) : (e % ntw); }
 __host__ __device__ constexpr int stage_ns(int N, int s, int P = 0) {

@@ -1394,6 +1394,10 @@ hipError_t launch_pair_cols(const float2* spectra, size_t spec_stride, const int
     // decimated into 53 inverse 70-point FFTs (radix 2, 5, 7), the radix-2
     // stage in registers on the loaded products
     return pair_cols_t<3710, 70, 2, true>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
+  if (n == 4092 && box <= 128)
+    // the default square crop of a 5760 x 4092 K3 frame: n = 132 * 31,
+    // decimated into 31 inverse 132-point FFTs (radix 4, 3, 11)
+    return pair_cols_t<4092, 132, 2, true>(spectra, spec_stride, pairs, npairs, wxy, box, tbuf, st);
   if (!pair_cols_decimated(n))
     return launch_pair_cols_czt(spectra, spec_stride, pairs, npairs, wxy, n, box, tbuf, st);
   const int body = pair_cols_plan_body(n, box, variant);

@@ -55,16 +55,19 @@ def test_level0_fft_any_even_size(n):
     d.free()
 
 
-@pytest.mark.parametrize("box", [32, 64, 96, 128, 160])
-def test_box_maps_at_3710_match_oracle(box):
-    """The pair column pass at n = 3710 runs decimated (53 inverse 70-point
-    FFTs, radix 2, 5, 7, and a 53-term combine split 27 + 26 over two lanes
-    per row) for boxes up to 128, and as the chirp-z pass above.  Box maps of
+@pytest.mark.parametrize("n,box", [(3710, 32), (3710, 64), (3710, 96), (3710, 128), (3710, 160),
+                                   (4092, 64), (4092, 96), (4092, 128)])
+def test_box_maps_at_3710_match_oracle(n, box):
+    """The pair column pass at the default crops of K2 (n = 3710) and K3 (n =
+    4092) frames runs decimated for boxes up to 128: at 3710, 53 inverse
+    70-point FFTs (radix 2, 5, 7) and a 53-term combine split 27 + 26 over
+    two lanes per row; at 4092, 31 inverse 132-point FFTs (radix 4, 3, 11)
+    and a 31-term combine.  Larger boxes run the chirp-z pass.  Box maps of
     random spectra against the FP64 oracle's cross power and pruned inverse
     (cufunc.cu:343-370, DFAlign.cu:897, FindShift's read-back
     cufunc.cu:396-424)."""
-    n, F = 3710, 4
-    rng = np.random.default_rng(3710 + box)
+    F = 4
+    rng = np.random.default_rng(n + box)
     spec = np.fft.rfft2(rng.standard_normal((F, n, n))).astype(np.complex64)
     ctx = mc.Context(0)
     ctx.set_spectra(spec.view(np.float32))
@@ -76,7 +79,7 @@ def test_box_maps_at_3710_match_oracle(box):
     for q, (a, b) in enumerate(pairs):
         ref = o.pruned_box(o.cross_power(spec[a], spec[b], n, 150.0), box)
         worst = max(worst, float(np.abs(boxes[q] - ref).max() / np.abs(ref).max()))
-    print("n 3710 box %d: max error / max %.3g" % (box, worst), flush=True)
+    print("n %d box %d: max error / max %.3g" % (n, box, worst), flush=True)
     assert worst < 1e-5
 
 
