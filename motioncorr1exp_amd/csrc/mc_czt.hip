@@ -533,8 +533,8 @@ static hipError_t rows_czt_t(const R2CGeom& g, int f0, int nframes, int nkx, flo
   constexpr int NT = czt_nt<M>();
   const float2* gtw = twiddles(M);
   if (!gtw) return hipErrorOutOfMemory;
-  if constexpr (M == 4096) {
-    // persistent, register twiddles
+  if constexpr (M == 4096 || M == 8192) {
+    // persistent, register twiddles (packed rows of 2050-8192 pixels)
     const int nitems = g.h * nframes;
     switch (g.dtype) {
 #define MC_RCP(D)                                                                                  \

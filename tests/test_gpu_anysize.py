@@ -95,6 +95,9 @@ CASES = [
     ("k3_bin2_2046", 4200, 4092, 6, dict(fod=1, iter=1, bin=2.0), 2046, 502),
     ("bin1.5_640", 1000, 960, 8, dict(fod=2, iter=2, bin=1.5), 640, 503),
     ("crop1854", 1900, 1854, 8, dict(fod=2, iter=1), 1854, 504),
+    # canvas 4200 binned 2x: packed rows of 2100 through the 8192-point
+    # persistent chirp-z row kernel, columns through the 16384-point one
+    ("crop4200_bin2_2100", 4300, 4200, 4, dict(fod=1, iter=1, bin=2.0), 2100, 505),
 ]
 
 
