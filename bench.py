@@ -564,7 +564,8 @@ def main():
 
     if not args.no_refgeom:
         # the reference's default geometry: square crop min(nx, ny) = 3710 =
-        # 2*5*7*53, no padding (chirp-z transforms)
+        # 2*5*7*53, no padding (stage 1 in two levels, 35 x 53 rows and 70 x 53
+        # columns; chirp-z for the image)
         pr = dict(params, pad=0)
         step(pr)
         mc.check(L.mc_sync())
@@ -579,7 +580,7 @@ def main():
         ctx.kernel_timing(False)
         out["reference_geometry"] = {
             "workload": "the same movie at the reference's default square crop, n = %d "
-                        "(no -pad; chirp-z transforms), itr 0" % rr["report"]["n"],
+                        "(no -pad; two-level stage-1 transforms), itr 0" % rr["report"]["n"],
             "n": rr["report"]["n"], "movies_per_s": kr / dtr, "ms_per_movie": dtr * 1e3 / kr,
             "kernels_ms_per_movie": kernr}
         log("reference geometry n=%d: %.3f ms/movie %s" % (rr["report"]["n"], dtr * 1e3 / kr,

@@ -154,6 +154,29 @@ static const CztTab* czt_tables(int n, int box) {
   return &(cache[key] = t);
 }
 
+// Twiddles of the two-level column transform n = Q P:
+// tw[k2 Q + a] = e^{-2 pi i ((a k2) mod n)/n}, k2 < P, a < Q.
+static const float2* qp_twiddles(int Q, int P) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int>, const float2*> cache;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(mu);
+  auto key = std::make_tuple(dev, Q, P);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  const int n = Q * P;
+  std::vector<float2> h((size_t)n);
+  for (int k2 = 0; k2 < P; k2++)
+    for (int a = 0; a < Q; a++) {
+      const double t = -2.0 * M_PI * (double)((a * k2) % n) / (double)n;
+      h[(size_t)k2 * Q + a] = make_float2((float)std::cos(t), (float)std::sin(t));
+    }
+  const float2* d = upload(h);
+  if (d) cache[key] = d;
+  return d;
+}
+
 // ------------------------------------------------------------ device part
 constexpr int CZT_PB = 4;
 // Threads per workgroup: one radix-16 butterfly of stage 0 per thread at
@@ -212,6 +235,19 @@ MC_DEV void bluestein_rw(float2* buf, const FftTw<M, NT, -1, 1>& ftw,
   fft_lds_rw<M, NT, 1, 1, M, PB>(buf, ftw);
 }
 
+// Pixel x of canvas row y of a frame: decoded, corrected, zero outside the
+// data region.
+template <int DT>
+MC_DEV float row_px(const R2CGeom& g, const char* fbase, bool corr, int x, int y) {
+  typedef typename Elem<DT>::T T;
+  if (x >= g.w) return 0.f;
+  const int2 sp = stretch_src(g, g.offx + x, g.offy + y);
+  const float v = (float)reinterpret_cast<const T*>(fbase + (size_t)sp.y * g.row_pitch)[sp.x];
+  if (!corr) return v;
+  const size_t idx = (size_t)sp.y * g.nx + sp.x;
+  return correct(g, v, g.dark ? g.dark[idx] : 0.f, g.gain ? g.gain[idx] : 1.f);
+}
+
 // Stage-1 row pass of one canvas row y < h of frame f (any even nu): the
 // row packed into nu/2 complex values, a Bluestein DFT of length nu/2 (bs),
 // unpack to the kept bins kx < nkx, stored as RT[kx/4][y][kx%4] (the layout
@@ -221,20 +257,11 @@ template <int M, int NT, int DT, class BS>
 MC_DEV void row_czt(const R2CGeom& g, int f0, int f, int y, int nkx, float2* rt,
                     size_t rt_fstride, float* rowsum, const float2* __restrict__ twnu,
                     float2* buf, const BS& bs, const float2* __restrict__ chirp = nullptr) {
-  typedef typename Elem<DT>::T T;
   constexpr int PB = CZT_PB;
   const int NU = g.nu, N = NU / 2;
   const char* fbase = g.src + (size_t)(f0 + f) * g.frame_pitch;
   const bool corr = g.dark || g.gain || g.xs < 1.f || g.ys < 1.f;
-  // pixel x of the row: decoded, corrected, zero outside the data region
-  auto px = [&](int x) -> float {
-    if (x >= g.w) return 0.f;
-    const int2 sp = stretch_src(g, g.offx + x, g.offy + y);
-    const float v = (float)reinterpret_cast<const T*>(fbase + (size_t)sp.y * g.row_pitch)[sp.x];
-    if (!corr) return v;
-    const size_t idx = (size_t)sp.y * g.nx + sp.x;
-    return correct(g, v, g.dark ? g.dark[idx] : 0.f, g.gain ? g.gain[idx] : 1.f);
-  };
+  auto px = [&](int x) -> float { return row_px<DT>(g, fbase, corr, x, y); };
   if (chirp) {  // the first chirp product of the Bluestein DFT, zeros up to M
     for (int m = threadIdx.x; m < M; m += NT)
       buf[lix<PB>(m)] = m < N ? cmul(make_float2(px(2 * m), px(2 * m + 1)), chirp[m])
@@ -406,6 +433,162 @@ __global__ __launch_bounds__(NT, czt_waves<NT>()) void k_c2c_cols_czt_p(
     }
     __syncthreads();  // buf is rewritten by the next column
   }
+}
+
+// Stage-1 row pass at nu = 2 Q P (packed length N = Q P, Q with a Stockham
+// plan, P odd), for the rows of the 3710 crop (N = 1855 = 35 x 53): the
+// packing, N-point DFT and unpack of row_czt, with the DFT in two levels as
+// in k_c2c_cols_qp below.  Lane a < Q reads the packed values z[a + Q b]
+// (pixels 2 (a + Q b) and 2 (a + Q b) + 1) straight from the frame and runs
+// the P-point direct DFT into LDS row k2, element a; after the twiddles and
+// the P Q-point FFTs, Z[k] is row k % P, element k / P (and the unpack puts
+// X[N] in the spare element Q of row 0).  One row per 64-lane workgroup;
+// the rows are numbered XCD-major, so the four rows that share a line of
+// the row-pass scratch are written through one L2.
+template <int Q, int P, int R, int NT, int DT>
+__global__ __launch_bounds__(NT) void k_r2c_rows_qp(R2CGeom g, int f0, int nframes, int nkx,
+                                                    float2* __restrict__ rt, size_t rt_fstride,
+                                                    float* __restrict__ rowsum,
+                                                    const float2* __restrict__ gtw,
+                                                    const float2* __restrict__ twqp,
+                                                    const float2* __restrict__ twnu) {
+  constexpr int N = Q * P;
+  constexpr int NH = N / 2 + 1;
+  constexpr int PB = 4;
+  constexpr int TS = lds_len<PB>(Q + 1) | 1;
+  constexpr int RS = P * TS;  // one row's LDS block
+  static_assert(NT >= R * Q && NT % 64 == 0, "one lane per P-point DFT");
+  __shared__ float2 buf[R * RS];
+  __shared__ float2 twm[Q];
+  const int nitems = g.h * nframes;
+  const int L0 = R * ((int)(blockIdx.x % 8) * (int)(gridDim.x / 8) + (int)(blockIdx.x / 8));
+  if (L0 >= nitems) return;
+  const int nr = nitems - L0 < R ? nitems - L0 : R;  // rows of this workgroup
+  for (int i = threadIdx.x; i < Q; i += NT) twm[i] = gtw[i * P];  // e^{+2 pi i i/Q}
+  const bool corr = g.dark || g.gain || g.xs < 1.f || g.ys < 1.f;
+  {
+    const int r = threadIdx.x / Q, a = threadIdx.x - r * Q;
+    if (r < nr) {
+      const int L = L0 + r, y = L % g.h, f = L / g.h;
+      const char* fbase = g.src + (size_t)(f0 + f) * g.frame_pitch;
+      float2 x[P];
+      if (!corr && g.vec) {
+        // the pixel pairs as single aligned loads (the row is all data)
+        typedef typename Elem<DT>::T T;
+        struct alignas(2 * sizeof(T)) Pair {
+          T a, b;
+        };
+        const Pair* row = reinterpret_cast<const Pair*>(
+            reinterpret_cast<const T*>(fbase + (size_t)(y + g.offy) * g.row_pitch) + g.offx);
+#pragma unroll
+        for (int b = 0; b < P; b++) {
+          const Pair v = row[a + Q * b];
+          x[b] = make_float2((float)v.a, (float)v.b);
+        }
+      } else {
+#pragma unroll
+        for (int b = 0; b < P; b++) {
+          const int m = a + Q * b;
+          x[b] = make_float2(row_px<DT>(g, fbase, corr, 2 * m, y),
+                             row_px<DT>(g, fbase, corr, 2 * m + 1, y));
+        }
+      }
+      float2* rb = buf + r * RS;
+      odd_dft<P, -1>(x, [&](int k, float2 v) { rb[k * TS + lix<PB>(a)] = v; });
+    }
+  }
+  __syncthreads();
+  // twiddles e^{-2 pi i a k2/N} (row k2 = 0 has none)
+  for (int e = threadIdx.x; e < R * (N - Q); e += NT) {
+    const int r = e / (N - Q), j = e - r * (N - Q) + Q;
+    const int k2 = j / Q, i = j - k2 * Q;
+    float2* p = buf + r * RS + k2 * TS + lix<PB>(i);
+    *p = cmul(*p, twqp[j]);
+  }
+  __syncthreads();
+  fft_lds_t<Q, NT, -1, R * P, TS, TwTable, PB>(buf, TwTable{twm, Q}, Q);
+  auto at = [&](int k) -> int { return (k % P) * TS + lix<PB>(k / P); };
+  // unpack, one thread per bin pair (k, N-k): X[k] = E[k] + e^{-2 pi i k/2N} O[k]
+  for (int t = threadIdx.x; t < R * NH; t += NT) {
+    const int r = t / NH, k = t - r * NH;
+    float2* rb = buf + r * RS;
+    const int lk = at(k), lm = at(k ? N - k : 0);
+    const float2 zk = rb[lk], zm = rb[lm];
+    {
+      const float2 zc = cconj(zm);
+      const float2 E = cscale(cadd(zk, zc), 0.5f);
+      const float2 D = csub(zk, zc);
+      const float2 O = make_float2(0.5f * D.y, -0.5f * D.x);
+      if (k == 0) rb[at(N)] = csub(E, O);
+      rb[lk] = cadd(E, cmul(cconj(twnu[k]), O));
+    }
+    if (k > 0 && k < N - k) {
+      const float2 zc = cconj(zk);
+      const float2 E = cscale(cadd(zm, zc), 0.5f);
+      const float2 D = csub(zm, zc);
+      const float2 O = make_float2(0.5f * D.y, -0.5f * D.x);
+      rb[lm] = cadd(E, cmul(cconj(twnu[N - k]), O));
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < nr * nkx; t += NT) {
+    const int r = t / nkx, k = t - r * nkx;
+    const int L = L0 + r, y = L % g.h, f = L / g.h;
+    const float2 X = buf[r * RS + at(k)];
+    rt[(size_t)f * rt_fstride + ((size_t)(k >> 2) * (2 * N) + y) * 4 + (k & 3)] = X;
+    if (k == 0 && rowsum) rowsum[(size_t)f * (2 * N) + y] = X.x;
+  }
+}
+
+// Stage-1 column pass at n = Q P without fill rows (Q with a Stockham plan,
+// P odd), for the 3710 = 70 x 53 crop of a 3838 x 3710 frame:
+//   X[k2 + P k1] = sum_a e^{-2 pi i a k1/Q} (e^{-2 pi i a k2/n}
+//                  sum_b x[a + Q b] e^{-2 pi i b k2/P}).
+// Lane a < Q loads x[a + Q b] (b < P) and runs the direct P-point DFT
+// (odd_dft) in registers into LDS row k2, element a; a second pass applies
+// the twiddles; then P Q-point FFTs run in LDS, and row k2 element k1 is
+// X[k2 + P k1].  About 0.4x the arithmetic of the 8192-point chirp-z
+// convolution, in a 33 KB LDS buffer.  One column per workgroup (a
+// persistent loop would hoist the LDS addresses of every FFT stage out of
+// the loop and hold them in registers); the columns are numbered XCD-major,
+// so the four columns of one row-pass line group share an L2.
+template <int Q, int P, int NT, int W>
+__global__ __launch_bounds__(NT, W) void k_c2c_cols_qp(const float2* __restrict__ rt,
+                                                       size_t rt_fstride, int nkx, int nitems,
+                                                       float2* __restrict__ spec,
+                                                       size_t spec_stride,
+                                                       const float2* __restrict__ gtw,
+                                                       const float2* __restrict__ twqp) {
+  constexpr int N = Q * P;
+  constexpr int PB = 4;
+  constexpr int TS = lds_len<PB>(Q) | 1;
+  static_assert(NT >= Q && NT % 64 == 0, "one lane per P-point DFT");
+  __shared__ float2 buf[P * TS];
+  __shared__ float2 twm[Q];
+  const int L = (int)(blockIdx.x % 8) * (int)(gridDim.x / 8) + (int)(blockIdx.x / 8);
+  if (L >= nitems) return;
+  for (int i = threadIdx.x; i < Q; i += NT) twm[i] = gtw[i * P];  // e^{+2 pi i i/Q}
+  const int a = threadIdx.x;
+  const int kx = L % nkx, f = L / nkx;
+  const float2* src = rt + (size_t)f * rt_fstride + (size_t)(kx >> 2) * N * 4 + (kx & 3);
+  if (a < Q) {
+    const auto q = buf_rsrc(src, N * 32);
+    float2 x[P];
+#pragma unroll
+    for (int b = 0; b < P; b++) x[b] = buf_ld2(q, 32 * a, 32 * Q * b);
+    odd_dft<P, -1>(x, [&](int k, float2 v) { buf[k * TS + lix<PB>(a)] = v; });
+  }
+  __syncthreads();
+  // twiddles e^{-2 pi i a k2/n} (row k2 = 0 has none)
+  for (int e = Q + threadIdx.x; e < N; e += NT) {
+    const int k2 = e / Q, i = e - k2 * Q;
+    float2* p = buf + k2 * TS + lix<PB>(i);
+    *p = cmul(*p, twqp[e]);
+  }
+  __syncthreads();
+  fft_lds_t<Q, NT, -1, P, TS, TwTable, PB>(buf, TwTable{twm, Q}, Q);
+  float2* dst = spec + (size_t)f * spec_stride + (size_t)kx * N;
+  for (int o = threadIdx.x; o < N; o += NT) dst[o] = buf[(o % P) * TS + lix<PB>(o / P)];
 }
 
 // Pair-stage column pass (any even n): per (pair, kx) item, the weighted
@@ -584,6 +767,41 @@ static hipError_t cols_czt_t(const R2CGeom& g, int nframes, int nkx, const float
   return hipGetLastError();
 }
 
+template <int Q, int P>
+static hipError_t rows_qp_t(const R2CGeom& g, int f0, int nframes, int nkx, float2* rt,
+                            size_t rt_fstride, float* rs, const float2* twnu, hipStream_t st) {
+  constexpr int R = 1, NT = 64;
+  const float2* gtw = twiddles(Q * P);
+  const float2* twqp = qp_twiddles(Q, P);
+  if (!gtw || !twqp) return hipErrorOutOfMemory;
+  // pixel pairs load as one element when the pairs are aligned
+  const size_t es = g.dtype == DT_FLOAT32 ? 4 : (g.dtype == DT_INT16 || g.dtype == DT_UINT16) ? 2 : 1;
+  R2CGeom gq = g;
+  gq.vec = ((uintptr_t)g.src % (2 * es) == 0 && g.row_pitch % (2 * es) == 0 &&
+            g.frame_pitch % (2 * es) == 0 && (g.offx & 1) == 0) ? 1 : 0;
+  const int grid = ((g.h * nframes + R - 1) / R + 7) / 8 * 8;
+  switch (g.dtype) {
+#define MC_RQ(D) case D: hipLaunchKernelGGL((k_r2c_rows_qp<Q, P, R, NT, D>), dim3(grid), dim3(NT), 0, st, gq, f0, nframes, nkx, rt, rt_fstride, rs, gtw, twqp, twnu); break;
+    MC_RQ(DT_INT8) MC_RQ(DT_UINT8) MC_RQ(DT_INT16) MC_RQ(DT_UINT16) MC_RQ(DT_FLOAT32)
+#undef MC_RQ
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int Q, int P>
+static hipError_t cols_qp_t(int nframes, int nkx, const float2* rt, size_t rt_fstride,
+                            float2* spectra, size_t spec_stride, hipStream_t st) {
+  constexpr int NT = 128, W = 2;
+  const float2* gtw = twiddles(Q * P);
+  const float2* twqp = qp_twiddles(Q, P);
+  if (!gtw || !twqp) return hipErrorOutOfMemory;
+  const int grid = (nkx * nframes + 7) / 8 * 8;
+  hipLaunchKernelGGL((k_c2c_cols_qp<Q, P, NT, W>), dim3(grid), dim3(NT), 0, st, rt, rt_fstride, nkx,
+                     nkx * nframes, spectra, spec_stride, gtw, twqp);
+  return hipGetLastError();
+}
+
 hipError_t launch_frames_to_spectra_czt(const R2CGeom& g, int f0, int nframes, float2* spectra,
                                         size_t spec_stride, float2* rt, float* rowsum,
                                         float* fill, const float2* padP, hipStream_t st) {
@@ -598,10 +816,17 @@ hipError_t launch_frames_to_spectra_czt(const R2CGeom& g, int f0, int nframes, f
   const float2* twnu = twiddles(g.nu);
   if (!tr || !tc || !twnu) return hipErrorOutOfMemory;
   hipError_t e = hipErrorInvalidValue;
-  switch (tr->M) {
+  // the square 3710 crop (no fill, no binning): two-level transforms,
+  // 35 x 53 for the packed rows and 70 x 53 for the columns
+  const bool qp = !g.pad_mode && g.nu == 3710 && g.n == g.nu && g.h == g.nu && g.w == g.nu;
+  if (qp) {
+    e = rows_qp_t<35, 53>(gv, f0, nframes, nkx, rt, rt_fstride, rs, twnu, st);
+  } else {
+    switch (tr->M) {
 #define MC_R(MM) case MM: e = rows_czt_t<MM>(gv, f0, nframes, nkx, rt, rt_fstride, rs, *tr, twnu, st); break;
-    MC_CZT_SIZES(MC_R)
+      MC_CZT_SIZES(MC_R)
 #undef MC_R
+    }
   }
   if (e != hipSuccess) return e;
   if (g.pad_mode) {
@@ -610,10 +835,14 @@ hipError_t launch_frames_to_spectra_czt(const R2CGeom& g, int f0, int nframes, f
   }
   e = hipErrorInvalidValue;
   const float* fl = g.pad_mode ? fill : nullptr;
-  switch (tc->M) {
+  if (qp) {
+    e = cols_qp_t<70, 53>(nframes, nkx, rt, rt_fstride, spectra, spec_stride, st);
+  } else {
+    switch (tc->M) {
 #define MC_C(MM) case MM: e = cols_czt_t<MM>(g, nframes, nkx, rt, rt_fstride, fl, padP, spectra, spec_stride, *tc, st); break;
-    MC_CZT_SIZES(MC_C)
+      MC_CZT_SIZES(MC_C)
 #undef MC_C
+    }
   }
   if (e != hipSuccess) return e;
   return hipGetLastError();

@@ -238,8 +238,85 @@ template <int R, int S> MC_DEV void dft(float2* v) {
   else static_assert(R == 2 || R == 3 || R == 4 || R == 5 || R == 7 || R == 8 || R == 11 || R == 16, "radix");
 }
 
+// ------------------------------------------- direct DFT of an odd length
+// cos and sin of 2 pi m/P evaluated at compile time: Taylor series in
+// double on the angle reduced to [-pi, pi] (40 terms: the truncation error
+// is far below one double ulp), rounded once to float.
+__host__ __device__ constexpr double cx_cos(double x) {
+  double term = 1.0, sum = 1.0;
+  for (int i = 1; i < 40; i++) {
+    term *= -x * x / ((2.0 * i - 1.0) * (2.0 * i));
+    sum += term;
+  }
+  return sum;
+}
+__host__ __device__ constexpr double cx_sin(double x) {
+  double term = x, sum = x;
+  for (int i = 1; i < 40; i++) {
+    term *= -x * x / ((2.0 * i) * (2.0 * i + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+template <int P> struct OddTrig {
+  float c[P], s[P];
+  __host__ __device__ constexpr OddTrig() : c{}, s{} {
+    for (int m = 0; m < P; m++) {
+      const int mm = 2 * m < P ? m : m - P;
+      const double t = 2.0 * 3.14159265358979323846 * mm / P;
+      c[m] = (float)cx_cos(t);
+      s[m] = (float)cx_sin(t);
+    }
+  }
+};
+
+// DFT of odd length P in registers, X_k = sum_n x_n e^{S 2 pi i n k/P}, by
+// pairing the inputs n and P - n (s_n = x_n + x_{P-n}, d_n = x_n - x_{P-n})
+// and the outputs k and P - k:
+//   X_k, X_{P-k} = x_0 + sum_n s_n cos(2 pi n k/P) -/+ i S' sum_n d_n sin(...)
+// that is (P - 1)^2 real FMAs instead of the 4 P^2 of the plain sum.  Every
+// lane uses the same coefficient at the same time, so with the loops
+// unrolled the coefficients are literal operands.  x is overwritten;
+// out(k, X_k) is called once for every k.
+template <int P, int S, class OUT>
+MC_DEV void odd_dft(float2 (&x)[P], OUT&& out) {
+  static_assert(P % 2 == 1 && P >= 3, "odd length");
+  constexpr int H = (P - 1) / 2;
+  constexpr OddTrig<P> T{};
+  const float2 x0 = x[0];
+  float2 sum = x0;
+#pragma unroll
+  for (int n = 1; n <= H; n++) {
+    const float2 a = x[n], b = x[P - n];
+    x[n] = cadd(a, b);
+    x[P - n] = csub(a, b);
+    sum = cadd(sum, x[n]);
+  }
+  out(0, sum);
+#pragma unroll
+  for (int k = 1; k <= H; k++) {
+    float2 A = x0, B = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int n = 1; n <= H; n++) {
+      const float c = T.c[(n * k) % P], s = T.s[(n * k) % P];
+      A.x = fmaf(x[n].x, c, A.x);
+      A.y = fmaf(x[n].y, c, A.y);
+      B.x = fmaf(x[P - n].x, s, B.x);
+      B.y = fmaf(x[P - n].y, s, B.y);
+    }
+    // A -/+ i B: S = -1 gives X_k = A - i B, X_{P-k} = A + i B
+    const float2 m = make_float2(A.x + B.y, A.y - B.x), p = make_float2(A.x - B.y, A.y + B.x);
+    out(k, S < 0 ? m : p);
+    out(P - k, S < 0 ? p : m);
+    // one output pair at a time: interleaving them keeps 4 accumulators per
+    // pair live and spills
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
 // ---------------------------------------------------- Stockham stage plans
-// Lengths N = 2^k * m, m a product of distinct factors 3, 5, 7 and 11.
+// Lengths N = 2^k * m, m a product of distinct factors 3, 5, 7 and 11 (k = 0
+// included: no power-of-two stage).
 // Power-of-two plans are balanced; the odd factors add a radix-3, radix-5,
 // radix-7 and/or radix-11 stage after the plan of 2^k.
 __host__ __device__ constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
@@ -279,7 +356,7 @@ __host__ __device__ constexpr int r8_stage_radix(int N, int s) {
        : 8;
 }
 __host__ __device__ constexpr int plan_p2_count(int N, int P) {
-  return P == 2 ? r8_stage_count(N) : pow2_stage_count(N);
+  return N == 1 ? 0 : P == 2 ? r8_stage_count(N) : pow2_stage_count(N);
 }
 __host__ __device__ constexpr int plan_p2_radix(int N, int s, int P) {
   return P == 2 ? r8_stage_radix(N, s)
@@ -313,7 +390,8 @@ __host__ __device__ constexpr int stage_radix(int N, int s, int P = 0) {
 static_assert(stage_radix(70, 0) == 2 && stage_radix(70, 1) == 5 && stage_radix(70, 2) == 7 &&
                   stage_radix(3840, 2) == 3 && stage_radix(3840, 3) == 5 &&
                   stage_count(132) == 3 && stage_radix(132, 0) == 4 &&
-                  stage_radix(132, 1) == 3 && stage_radix(132, 2) == 11,
+                  stage_radix(132, 1) == 3 && stage_radix(132, 2) == 11 &&
+                  stage_count(35) == 2 && stage_radix(35, 0) == 5 && stage_radix(35, 1) == 7,
               "stage plans");
 // Table index modulo the table size (any size; a power of two masks).
 MC_DEV int tw_mod(int e, int ntw) { return (ntw & (ntw - 1)) == 0 ? (e & (ntw - 1)) : (e % ntw); }

@@ -94,19 +94,21 @@ def test_device_corrections_match_host(ctx, dtype, two, stretch):
     assert np.array_equal(s_dev, ctx.get_spectra(F, 512))
 
 
-def test_device_corrections_match_host_chirp_z(ctx):
-    """The same at a square crop without a Stockham plan (2470 = 2 5 13 19),
-    where stage 1 runs the persistent chirp-z row kernel (packed length 1235,
-    4096-point convolution) and column kernel (8192-point): the corrections
-    are applied in its row loads."""
+@pytest.mark.parametrize("nx,ny,crop", [(2500, 2480, 2470), (3730, 3720, 3710)])
+def test_device_corrections_match_host_chirp_z(ctx, nx, ny, crop):
+    """The same at square crops without a Stockham plan.  At 2470 = 2 5 13 19
+    stage 1 runs the persistent chirp-z row kernel (packed length 1235,
+    4096-point convolution) and column kernel (8192-point); at 3710 = 70 x 53
+    the two-level row (35 x 53) and column (70 x 53) kernels.  The
+    corrections are applied in their row loads."""
     F = 3
-    frames, _ = make_movie(2500, 2480, F, dose=2.0, step=0.8, seed=72)
+    frames, _ = make_movie(nx, ny, F, dose=2.0, step=0.8, seed=72)
     raw = np.clip(frames, 0, 127).astype(np.uint16)
     nz, ny, nx = raw.shape
     dark, gain, dark2, gain2 = _refs(ny, nx, 6, True)
     xs, ys = 1 / 1.02, 1 / 1.01
     b1, b2 = host_correct(raw, dark, gain, dark2, gain2, xs, ys)
-    prm = mc.default_params(F, FrameDistOffset=1, CCPeakSearchDim=64, crop_nsam=2470)
+    prm = mc.default_params(F, FrameDistOffset=1, CCPeakSearchDim=64, crop_nsam=crop)
     want = ("pair_b", "frame_shift", "sum_image")
     h = ctx.align_movie(mc.DeviceArray(b1), b1.shape, np.float32, prm,
                         dev_frames2=mc.DeviceArray(b2), want=want)
@@ -117,9 +119,9 @@ def test_device_corrections_match_host_chirp_z(ctx):
     assert np.array_equal(d["frame_shift"], h["frame_shift"])
     assert np.array_equal(d["sum_image"], h["sum_image"])
     ctx.load_frames(mc.DeviceArray(raw), raw.shape, np.uint16, prm, corr=corr)
-    s_dev = ctx.get_spectra(F, 2470)
+    s_dev = ctx.get_spectra(F, crop)
     ctx.load_frames(mc.DeviceArray(b1), b1.shape, np.float32, prm)
-    assert np.array_equal(s_dev, ctx.get_spectra(F, 2470))
+    assert np.array_equal(s_dev, ctx.get_spectra(F, crop))
 
 
 def test_corrections_reject_bad_arguments(ctx):
