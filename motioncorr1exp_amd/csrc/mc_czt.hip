@@ -442,7 +442,10 @@ __global__ __launch_bounds__(NT, czt_waves<NT>()) void k_c2c_cols_czt_p(
 // (pixels 2 (a + Q b) and 2 (a + Q b) + 1) straight from the frame and runs
 // the P-point direct DFT into LDS row k2, element a; after the twiddles and
 // the P Q-point FFTs, Z[k] is row k % P, element k / P (and the unpack puts
-// X[N] in the spare element Q of row 0).  One row per 64-lane workgroup;
+// X[N] in the spare element Q of row 0).  The LDS rows are unpadded (odd
+// stride 37): 16 KB per row fits 10 workgroups per CU against 9 with the
+// padded layout, 0.5 ms less per 40-frame movie for the two passes.  One
+// row per 64-lane workgroup;
 // the rows are numbered XCD-major, so the four rows that share a line of
 // the row-pass scratch are written through one L2.
 template <int Q, int P, int R, int NT, int DT>
@@ -454,7 +457,7 @@ __global__ __launch_bounds__(NT) void k_r2c_rows_qp(R2CGeom g, int f0, int nfram
                                                     const float2* __restrict__ twnu) {
   constexpr int N = Q * P;
   constexpr int NH = N / 2 + 1;
-  constexpr int PB = 4;
+  constexpr int PB = 0;  // unpadded rows (odd stride): more workgroups per CU
   constexpr int TS = lds_len<PB>(Q + 1) | 1;
   constexpr int RS = P * TS;  // one row's LDS block
   static_assert(NT >= R * Q && NT % 64 == 0, "one lane per P-point DFT");
@@ -548,7 +551,8 @@ __global__ __launch_bounds__(NT) void k_r2c_rows_qp(R2CGeom g, int f0, int nfram
 // (odd_dft) in registers into LDS row k2, element a; a second pass applies
 // the twiddles; then P Q-point FFTs run in LDS, and row k2 element k1 is
 // X[k2 + P k1].  About 0.4x the arithmetic of the 8192-point chirp-z
-// convolution, in a 33 KB LDS buffer.  One column per workgroup (a
+// convolution, in a 31 KB LDS buffer (unpadded rows of odd stride 71, 5
+// workgroups per CU).  One column per workgroup (a
 // persistent loop would hoist the LDS addresses of every FFT stage out of
 // the loop and hold them in registers); the columns are numbered XCD-major,
 // so the four columns of one row-pass line group share an L2.
@@ -560,7 +564,7 @@ __global__ __launch_bounds__(NT, W) void k_c2c_cols_qp(const float2* __restrict_
                                                        const float2* __restrict__ gtw,
                                                        const float2* __restrict__ twqp) {
   constexpr int N = Q * P;
-  constexpr int PB = 4;
+  constexpr int PB = 0;  // as in k_r2c_rows_qp
   constexpr int TS = lds_len<PB>(Q) | 1;
   static_assert(NT >= Q && NT % 64 == 0, "one lane per P-point DFT");
   __shared__ float2 buf[P * TS];
