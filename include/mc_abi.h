@@ -74,7 +74,8 @@ int mc_mem_bin_d2h(float *dst_host, const float *src, int dst_n, int src_n);
 /* GPUFFTPlan / GPUIFFTPlan (cufunc.cu:202-215): n x n in-place plans.
  * Supported n: any even size from 64 to 8192 (Stockham plans for powers of
  * two and 2^k * 3, 5 or 15; chirp-z transforms for the others, except the
- * forward transform at 3710, which runs in two levels, 35 x 53 and 70 x 53). */
+ * forward transforms at 3710 and 4092, which run in two levels: 35 x 53 and
+ * 70 x 53, 66 x 31 and 132 x 31). */
 int mc_fft_plan(mc_plan *plan, int n);
 int mc_ifft_plan(mc_plan *plan, int n);
 /* GPUFFT2d / GPUIFFT2d (cufunc.cu:223-233): unnormalised, in place. */

@@ -436,16 +436,17 @@ __global__ __launch_bounds__(NT, czt_waves<NT>()) void k_c2c_cols_czt_p(
 }
 
 // Stage-1 row pass at nu = 2 Q P (packed length N = Q P, Q with a Stockham
-// plan, P odd), for the rows of the 3710 crop (N = 1855 = 35 x 53): the
+// plan, P odd), for the rows of the 3710 and 4092 crops (N = 1855 = 35 x 53,
+// 2046 = 66 x 31): the
 // packing, N-point DFT and unpack of row_czt, with the DFT in two levels as
 // in k_c2c_cols_qp below.  Lane a < Q reads the packed values z[a + Q b]
 // (pixels 2 (a + Q b) and 2 (a + Q b) + 1) straight from the frame and runs
 // the P-point direct DFT into LDS row k2, element a; after the twiddles and
 // the P Q-point FFTs, Z[k] is row k % P, element k / P (and the unpack puts
 // X[N] in the spare element Q of row 0).  The LDS rows are unpadded (odd
-// stride 37): 16 KB per row fits 10 workgroups per CU against 9 with the
-// padded layout, 0.5 ms less per 40-frame movie for the two passes.  One
-// row per 64-lane workgroup;
+// stride; at 3710, 16 KB per row fits 10 workgroups per CU against 9 with
+// the padded layout, 0.5 ms less per 40-frame movie for the two passes).
+// One row per workgroup of Q lanes rounded up to whole waves;
 // the rows are numbered XCD-major, so the four rows that share a line of
 // the row-pass scratch are written through one L2.
 template <int Q, int P, int R, int NT, int DT>
@@ -543,23 +544,25 @@ __global__ __launch_bounds__(NT) void k_r2c_rows_qp(R2CGeom g, int f0, int nfram
   }
 }
 
-// Stage-1 column pass at n = Q P without fill rows (Q with a Stockham plan,
-// P odd), for the 3710 = 70 x 53 crop of a 3838 x 3710 frame:
-//   X[k2 + P k1] = sum_a e^{-2 pi i a k1/Q} (e^{-2 pi i a k2/n}
+// Stage-1 column pass at N = Q P without fill rows (Q with a Stockham plan,
+// P odd), for the 3710 = 70 x 53 crop of a 3838 x 3710 frame and the 4092 =
+// 132 x 31 crop of a K3 frame:
+//   X[k2 + P k1] = sum_a e^{-2 pi i a k1/Q} (e^{-2 pi i a k2/N}
 //                  sum_b x[a + Q b] e^{-2 pi i b k2/P}).
 // Lane a < Q loads x[a + Q b] (b < P) and runs the direct P-point DFT
 // (odd_dft) in registers into LDS row k2, element a; a second pass applies
 // the twiddles; then P Q-point FFTs run in LDS, and row k2 element k1 is
-// X[k2 + P k1].  About 0.4x the arithmetic of the 8192-point chirp-z
-// convolution, in a 31 KB LDS buffer (unpadded rows of odd stride 71, 5
-// workgroups per CU).  One column per workgroup (a
+// X[k2 + P k1]; the rows of the Fourier crop to n are stored.  At 3710
+// about 0.4x the arithmetic of the 8192-point chirp-z convolution, in a 31
+// KB LDS buffer (unpadded rows of odd stride 71, 5 workgroups per CU).  One
+// column per workgroup (a
 // persistent loop would hoist the LDS addresses of every FFT stage out of
 // the loop and hold them in registers); the columns are numbered XCD-major,
 // so the four columns of one row-pass line group share an L2.
 template <int Q, int P, int NT, int W>
 __global__ __launch_bounds__(NT, W) void k_c2c_cols_qp(const float2* __restrict__ rt,
                                                        size_t rt_fstride, int nkx, int nitems,
-                                                       float2* __restrict__ spec,
+                                                       int n, float2* __restrict__ spec,
                                                        size_t spec_stride,
                                                        const float2* __restrict__ gtw,
                                                        const float2* __restrict__ twqp) {
@@ -583,7 +586,7 @@ __global__ __launch_bounds__(NT, W) void k_c2c_cols_qp(const float2* __restrict_
     odd_dft<P, -1>(x, [&](int k, float2 v) { buf[k * TS + lix<PB>(a)] = v; });
   }
   __syncthreads();
-  // twiddles e^{-2 pi i a k2/n} (row k2 = 0 has none)
+  // twiddles e^{-2 pi i a k2/N} (row k2 = 0 has none)
   for (int e = Q + threadIdx.x; e < N; e += NT) {
     const int k2 = e / Q, i = e - k2 * Q;
     float2* p = buf + k2 * TS + lix<PB>(i);
@@ -591,8 +594,12 @@ __global__ __launch_bounds__(NT, W) void k_c2c_cols_qp(const float2* __restrict_
   }
   __syncthreads();
   fft_lds_t<Q, NT, -1, P, TS, TwTable, PB>(buf, TwTable{twm, Q}, Q);
-  float2* dst = spec + (size_t)f * spec_stride + (size_t)kx * N;
-  for (int o = threadIdx.x; o < N; o += NT) dst[o] = buf[(o % P) * TS + lix<PB>(o / P)];
+  // the rows of the Fourier crop to n (n = N unbinned)
+  float2* dst = spec + (size_t)f * spec_stride + (size_t)kx * n;
+  for (int o = threadIdx.x; o < n; o += NT) {
+    const int si = o < n / 2 ? o : N - n + o;
+    dst[o] = buf[(si % P) * TS + lix<PB>(si / P)];
+  }
 }
 
 // Pair-stage column pass (any even n): per (pair, kx) item, the weighted
@@ -774,7 +781,7 @@ static hipError_t cols_czt_t(const R2CGeom& g, int nframes, int nkx, const float
 template <int Q, int P>
 static hipError_t rows_qp_t(const R2CGeom& g, int f0, int nframes, int nkx, float2* rt,
                             size_t rt_fstride, float* rs, const float2* twnu, hipStream_t st) {
-  constexpr int R = 1, NT = 64;
+  constexpr int R = 1, NT = (Q + 63) / 64 * 64;
   const float2* gtw = twiddles(Q * P);
   const float2* twqp = qp_twiddles(Q, P);
   if (!gtw || !twqp) return hipErrorOutOfMemory;
@@ -794,15 +801,15 @@ static hipError_t rows_qp_t(const R2CGeom& g, int f0, int nframes, int nkx, floa
 }
 
 template <int Q, int P>
-static hipError_t cols_qp_t(int nframes, int nkx, const float2* rt, size_t rt_fstride,
+static hipError_t cols_qp_t(int nframes, int nkx, const float2* rt, size_t rt_fstride, int n,
                             float2* spectra, size_t spec_stride, hipStream_t st) {
-  constexpr int NT = 128, W = 2;
+  constexpr int NT = (Q + 63) / 64 * 64, W = 2;
   const float2* gtw = twiddles(Q * P);
   const float2* twqp = qp_twiddles(Q, P);
   if (!gtw || !twqp) return hipErrorOutOfMemory;
   const int grid = (nkx * nframes + 7) / 8 * 8;
   hipLaunchKernelGGL((k_c2c_cols_qp<Q, P, NT, W>), dim3(grid), dim3(NT), 0, st, rt, rt_fstride, nkx,
-                     nkx * nframes, spectra, spec_stride, gtw, twqp);
+                     nkx * nframes, n, spectra, spec_stride, gtw, twqp);
   return hipGetLastError();
 }
 
@@ -820,11 +827,15 @@ hipError_t launch_frames_to_spectra_czt(const R2CGeom& g, int f0, int nframes, f
   const float2* twnu = twiddles(g.nu);
   if (!tr || !tc || !twnu) return hipErrorOutOfMemory;
   hipError_t e = hipErrorInvalidValue;
-  // the square 3710 crop (no fill, no binning): two-level transforms,
-  // 35 x 53 for the packed rows and 70 x 53 for the columns
-  const bool qp = !g.pad_mode && g.nu == 3710 && g.n == g.nu && g.h == g.nu && g.w == g.nu;
-  if (qp) {
+  // square canvases of 3710 (K2 crop) and 4092 (K3 crop) without fill:
+  // two-level transforms, 35 x 53 (66 x 31) for the packed rows and 70 x 53
+  // (132 x 31) for the columns, any binning
+  const int qp = (!g.pad_mode && g.h == g.nu && g.w == g.nu && (g.nu == 3710 || g.nu == 4092))
+                     ? g.nu : 0;
+  if (qp == 3710) {
     e = rows_qp_t<35, 53>(gv, f0, nframes, nkx, rt, rt_fstride, rs, twnu, st);
+  } else if (qp == 4092) {
+    e = rows_qp_t<66, 31>(gv, f0, nframes, nkx, rt, rt_fstride, rs, twnu, st);
   } else {
     switch (tr->M) {
 #define MC_R(MM) case MM: e = rows_czt_t<MM>(gv, f0, nframes, nkx, rt, rt_fstride, rs, *tr, twnu, st); break;
@@ -839,8 +850,10 @@ hipError_t launch_frames_to_spectra_czt(const R2CGeom& g, int f0, int nframes, f
   }
   e = hipErrorInvalidValue;
   const float* fl = g.pad_mode ? fill : nullptr;
-  if (qp) {
-    e = cols_qp_t<70, 53>(nframes, nkx, rt, rt_fstride, spectra, spec_stride, st);
+  if (qp == 3710) {
+    e = cols_qp_t<70, 53>(nframes, nkx, rt, rt_fstride, g.n, spectra, spec_stride, st);
+  } else if (qp == 4092) {
+    e = cols_qp_t<132, 31>(nframes, nkx, rt, rt_fstride, g.n, spectra, spec_stride, st);
   } else {
     switch (tc->M) {
 #define MC_C(MM) case MM: e = cols_czt_t<MM>(g, nframes, nkx, rt, rt_fstride, fl, padP, spectra, spec_stride, *tc, st); break;

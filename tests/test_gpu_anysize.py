@@ -1,5 +1,6 @@
 """GPU parity at transform sizes without a Stockham plan (chirp-z path,
-mc_czt.hip): the reference crops to min(nx, ny) rounded to even and plans
+mc_czt.hip; the forward stage-1 transform at the 3710 and 4092 canvases in
+two levels, 35 x 53 / 70 x 53 and 66 x 31 / 132 x 31): the reference crops to min(nx, ny) rounded to even and plans
 cuFFT at that size (func.cpp:80-103, DFAlign.cu:508-515, cufunc.cu:202-215),
 so a 3838 x 3710 K2 frame runs at n = 3710 = 2*5*7*53, a K3 frame binned 2x
 at 4092 -> 2046 = 2*3*11*31, and a non-integer -bin gives sizes such as 640.
@@ -28,7 +29,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CLI = os.path.join(ROOT, "motioncorr1exp_amd", "bin", "dosefgpu_driftcorr")
 
 
-@pytest.mark.parametrize("n", [66, 640, 1000, 1854, 2046, 3710])
+@pytest.mark.parametrize("n", [66, 640, 1000, 1854, 2046, 3710, 4092])
 def test_level0_fft_any_even_size(n):
     rng = np.random.default_rng(n)
     img = rng.standard_normal((n, n)).astype(np.float32)
