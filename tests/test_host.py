@@ -111,3 +111,33 @@ def test_shipped_tuning_table_is_well_formed():
         assert 1 <= (e["r2c"] & 7) <= 3 and 0 <= (e["r2c"] >> 4) < 6
         assert e["chunk"] in (1, 2, 4, 8)
         assert 0 <= e["cols"] <= 3 and 0 <= e["rows"] <= 2
+
+
+@pytest.mark.parametrize("P", [31, 53])
+def test_odd_dft_coefficients_match_libm(tmp_path, P):
+    """The compile-time cos/sin tables of the direct odd-length DFT
+    (OddTrig<P> in mc_device.h, used for the 31- and 53-point stages of the
+    two-level transforms at 4092 and 3710) equal libm's double cos and sin
+    of 2 pi m/P rounded to float, for every m."""
+    import shutil
+    import subprocess
+
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    src = tmp_path / "trig.cpp"
+    src.write_text(
+        '#include "mc_device.h"\n#include <cstdio>\n'
+        "int main() { constexpr mc::OddTrig<%d> T{};\n"
+        '  for (int m = 0; m < %d; m++) printf("%%a %%a\\n", (double)T.c[m], (double)T.s[m]); }\n'
+        % (P, P))
+    exe = tmp_path / "trig"
+    inc = os.path.join(ROOT, "motioncorr1exp_amd", "csrc")
+    subprocess.run([hipcc, "-std=c++17", "-O1", "-I", inc, str(src), "-o", str(exe)],
+                   check=True, capture_output=True, timeout=300)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    got = np.array([float.fromhex(v) for v in out]).reshape(P, 2)
+    m = np.arange(P)
+    want_c = np.cos(2 * np.pi * m / P).astype(np.float32).astype(np.float64)
+    want_s = np.sin(2 * np.pi * m / P).astype(np.float32).astype(np.float64)
+    assert np.array_equal(got[:, 0], want_c) and np.array_equal(got[:, 1], want_s)
