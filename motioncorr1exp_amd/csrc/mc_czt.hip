@@ -675,6 +675,45 @@ __global__ __launch_bounds__(NT, czt_waves<NT>()) void k_cols_inv_czt(const floa
   for (int i = threadIdx.x; i < n; i += NT) dst[(size_t)i * 4] = buf[lix<PB>(i)];
 }
 
+// Sum-image C2R, column pass at N = Q P (the 3710 and 4092 sizes): the
+// inverse of k_c2c_cols_qp, X[k2 + P k1] = sum_a e^{2 pi i a k1/Q}
+// (e^{2 pi i a k2/N} sum_b x[a + Q b] e^{2 pi i b k2/P}), on column kx of the
+// spectrum, stored like k_cols_inv_czt (tmp[kx/4][y][kx%4]).
+template <int Q, int P, int NT>
+__global__ __launch_bounds__(NT, 2) void k_cols_inv_qp(const float2* __restrict__ spec, int nkx,
+                                                       float2* __restrict__ tmp,
+                                                       const float2* __restrict__ gtw,
+                                                       const float2* __restrict__ twqp) {
+  constexpr int N = Q * P;
+  constexpr int PB = 0;
+  constexpr int TS = lds_len<PB>(Q) | 1;
+  static_assert(NT >= Q && NT % 64 == 0, "one lane per P-point DFT");
+  __shared__ float2 buf[P * TS];
+  __shared__ float2 twm[Q];
+  const int kx = (int)(blockIdx.x % 8) * (int)(gridDim.x / 8) + (int)(blockIdx.x / 8);
+  if (kx >= nkx) return;
+  for (int i = threadIdx.x; i < Q; i += NT) twm[i] = gtw[i * P];  // e^{+2 pi i i/Q}
+  const int a = threadIdx.x;
+  if (a < Q) {
+    const auto q = buf_rsrc(spec + (size_t)kx * N, N * 8);
+    float2 x[P];
+#pragma unroll
+    for (int b = 0; b < P; b++) x[b] = buf_ld2(q, 8 * a, 8 * Q * b);
+    odd_dft<P, 1>(x, [&](int k, float2 v) { buf[k * TS + lix<PB>(a)] = v; });
+  }
+  __syncthreads();
+  // twiddles e^{+2 pi i a k2/N} (the table holds e^{-2 pi i a k2/N})
+  for (int e = Q + threadIdx.x; e < N; e += NT) {
+    const int k2 = e / Q, i = e - k2 * Q;
+    float2* p = buf + k2 * TS + lix<PB>(i);
+    *p = cmulc(*p, twqp[e]);
+  }
+  __syncthreads();
+  fft_lds_t<Q, NT, 1, P, TS, TwTable, PB>(buf, TwTable{twm, Q}, Q);
+  float2* dst = tmp + (size_t)(kx >> 2) * N * 4 + (kx & 3);
+  for (int o = threadIdx.x; o < N; o += NT) dst[(size_t)o * 4] = buf[(o % P) * TS + lix<PB>(o / P)];
+}
+
 // Sum-image C2R, row pass (any even n): the n/2+1 bins of row y with the
 // DC/Nyquist projection, packed into n/2 complex values, a Bluestein
 // inverse DFT of length n/2, x *= scale.
@@ -915,6 +954,18 @@ static hipError_t c2r_rows_czt_t(const float2* tmp, int n, float scale, float* i
   return hipGetLastError();
 }
 
+template <int Q, int P>
+static hipError_t cols_inv_qp_t(const float2* spec, float2* tmp, hipStream_t st) {
+  constexpr int NT = (Q + 63) / 64 * 64;
+  const float2* gtw = twiddles(Q * P);
+  const float2* twqp = qp_twiddles(Q, P);
+  if (!gtw || !twqp) return hipErrorOutOfMemory;
+  const int nkx = Q * P / 2 + 1;
+  hipLaunchKernelGGL((k_cols_inv_qp<Q, P, NT>), dim3((nkx + 7) / 8 * 8), dim3(NT), 0, st, spec,
+                     nkx, tmp, gtw, twqp);
+  return hipGetLastError();
+}
+
 hipError_t launch_spectrum_to_image_czt(const float2* spec, int n, float scale, float* img,
                                         size_t img_pitch, float2* tmp, hipStream_t st) {
   if (n < 64 || n > 8192 || n % 2) return hipErrorInvalidValue;
@@ -923,10 +974,16 @@ hipError_t launch_spectrum_to_image_czt(const float2* spec, int n, float scale, 
   const float2* twn = twiddles(n);
   if (!tc || !tr || !twn) return hipErrorOutOfMemory;
   hipError_t e = hipErrorInvalidValue;
-  switch (tc->M) {
+  if (n == 3710) {
+    e = cols_inv_qp_t<70, 53>(spec, tmp, st);
+  } else if (n == 4092) {
+    e = cols_inv_qp_t<132, 31>(spec, tmp, st);
+  } else {
+    switch (tc->M) {
 #define MC_CI(MM) case MM: e = cols_inv_czt_t<MM>(spec, n, tmp, *tc, st); break;
-    MC_CZT_SIZES(MC_CI)
+      MC_CZT_SIZES(MC_CI)
 #undef MC_CI
+    }
   }
   if (e != hipSuccess) return e;
   e = hipErrorInvalidValue;
